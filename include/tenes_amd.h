@@ -1,0 +1,210 @@
+/*
+ * tenes_amd.h -- C ABI of the MI355X-native CTMRG hot path.
+ *
+ * The library replaces TeNeS's CTM environment path (mptensor + LAPACK on
+ * the CPU) with FP64 HIP kernels on one gfx950 GPU.  TeNeS has no FFI for
+ * this path (its only C entry point is the whole program,
+ * src/iTPS/main.cpp:33-39 tenes_itps_main), so each entry point below
+ * replaces one templated C++ function of tenes::itps::core; the file:line of
+ * the function replaced is given with each declaration (paths relative to
+ * the TeNeS source tree).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative status on failure;
+ *    tenes_ctx_last_error() returns the message.  No C++ exception crosses
+ *    the boundary.
+ *  - Tensors are dense, row-major (last leg fastest), real FP64, resident in
+ *    device memory and owned by the library unless created with
+ *    tenes_tensor_from_device (borrowed, caller keeps ownership).
+ *  - Leg orders are those of TeNeS: site tensor (left, up, right, down,
+ *    phys); corners C1..C4 (chi, chi); edges eT (chi, chi, D, D) with the
+ *    leg conventions of src/iTPS/core/ctm.cpp:984-1192.
+ *  - All work of a context is issued on its HIP stream.  Functions returning
+ *    host scalars synchronise that stream.  A context must not be used from
+ *    two host threads at once.
+ */
+#ifndef TENES_AMD_H
+#define TENES_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tenes_ctx tenes_ctx;
+typedef struct tenes_tensor tenes_tensor;
+typedef struct tenes_env tenes_env;
+
+#define TENES_OK 0
+#define TENES_ERR_GENERIC (-1)
+#define TENES_ERR_HIP (-2)
+#define TENES_ERR_ARGUMENT (-3)
+#define TENES_ERR_ALLOC (-4)
+#define TENES_ERR_UNSUPPORTED (-5)
+
+/* Version of this ABI (major*10000 + minor*100 + patch). */
+int tenes_abi_version(void);
+
+/* ---- context ----------------------------------------------------------- */
+/* hip_stream may be NULL (the library creates a non-blocking stream). */
+int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out);
+int tenes_ctx_destroy(tenes_ctx *ctx);
+const char *tenes_ctx_last_error(const tenes_ctx *ctx);
+int tenes_ctx_synchronize(tenes_ctx *ctx);
+int tenes_ctx_set_seed(tenes_ctx *ctx, uint64_t seed);
+/* Accumulated GEMM work issued through this context (2*M*N*K per GEMM). */
+int tenes_ctx_stats(tenes_ctx *ctx, double *gemm_flops, int64_t *gemm_calls);
+int tenes_ctx_reset_stats(tenes_ctx *ctx);
+
+/* ---- tensors (mptensor::Tensor replacement) ------------------------------ */
+int tenes_tensor_from_host(tenes_ctx *ctx, int rank, const int64_t *shape,
+                           const double *data, tenes_tensor **out);
+int tenes_tensor_from_device(tenes_ctx *ctx, int rank, const int64_t *shape,
+                             double *dev_ptr, tenes_tensor **out);
+int tenes_tensor_zeros(tenes_ctx *ctx, int rank, const int64_t *shape,
+                       tenes_tensor **out);
+int tenes_tensor_rank(const tenes_tensor *t, int *rank);
+int tenes_tensor_shape(const tenes_tensor *t, int64_t *shape);
+int tenes_tensor_to_host(const tenes_tensor *t, double *out);
+int tenes_tensor_device_ptr(const tenes_tensor *t, double **out);
+int tenes_tensor_destroy(tenes_tensor *t);
+
+/* mptensor tensordot(A, B, axA, axB) (used throughout src/iTPS/core), with an
+ * optional transpose of the result fused into the store (perm == NULL: none).
+ * Result legs: free legs of A in order, then free legs of B. */
+int tenes_tensordot(const tenes_tensor *A, const tenes_tensor *B, int nax,
+                    const int *axA, const int *axB, const int *perm,
+                    tenes_tensor **out);
+/* mptensor transpose(T, axes): new leg k = old leg axes[k]. */
+int tenes_transpose(const tenes_tensor *T, const int *axes, tenes_tensor **out);
+/* Leading k singular triplets of a 2-D tensor (mptensor svd at
+ * src/iTPS/core/ctm.cpp:273).  U: (m, k), V: (n, k), s: k values, descending.
+ * *k_out receives the number returned. */
+int tenes_svd_top(const tenes_tensor *A, int k, int power_iters,
+                  tenes_tensor **U, tenes_tensor **V, double *s, int *k_out);
+
+/* ---- parameters (PEPS_Parameters.hpp:29-94, CTM fields) ------------------ */
+typedef struct tenes_peps_parameters {
+  int CHI;
+  double Inverse_projector_cut;
+  double CTM_Convergence_Epsilon;
+  int Max_CTM_Iteration;
+  int CTM_Projector_corner;
+  int Use_RSVD;
+  double RSVD_Oversampling_factor;
+  int print_level;
+  int svd_power_iters;  /* GPU range-finder iterations (full-SVD mode) */
+  int rsvd_power_iters; /* GPU range-finder iterations (Use_RSVD mode)  */
+} tenes_peps_parameters;
+/* Defaults of PEPS_Parameters.cpp:29-77. */
+void tenes_peps_parameters_default(tenes_peps_parameters *p);
+
+/* ---- CTM core (src/iTPS/core/ctm.hpp) ------------------------------------- */
+/* Calc_projector_left_block, ctm.cpp:174-318 */
+int tenes_calc_projector_left_block(
+    const tenes_tensor *C1, const tenes_tensor *C4, const tenes_tensor *eT1,
+    const tenes_tensor *eT6, const tenes_tensor *eT7, const tenes_tensor *eT8,
+    const tenes_tensor *Tn1, const tenes_tensor *Tn4,
+    const tenes_peps_parameters *p, tenes_tensor **PU, tenes_tensor **PL);
+/* Calc_projector_updown_blocks, ctm.cpp:320-491; T[16] =
+ * {C1,C2,C3,C4, eT1..eT8, Tn1..Tn4} */
+int tenes_calc_projector_updown_blocks(const tenes_tensor *const *T,
+                                       const tenes_peps_parameters *p,
+                                       tenes_tensor **PU, tenes_tensor **PL);
+/* Calc_Next_CTM, ctm.cpp:493-518 */
+int tenes_calc_next_ctm(const tenes_tensor *C1, const tenes_tensor *C4,
+                        const tenes_tensor *eT1, const tenes_tensor *eT6,
+                        const tenes_tensor *PU, const tenes_tensor *PL,
+                        tenes_tensor **C1_out, tenes_tensor **C4_out);
+/* Calc_Next_eT, ctm.cpp:520-552 */
+int tenes_calc_next_et(const tenes_tensor *eT8, const tenes_tensor *Tn1,
+                       const tenes_tensor *PU, const tenes_tensor *PL,
+                       tenes_tensor **eT_out);
+
+/* ---- environment object: the iTPS arrays Tn, C1..C4, eTt/eTr/eTb/eTl ------
+ * (iTPS.hpp:293-315) on a SquareLattice (SquareLattice.hpp:54-163). */
+#define TENES_C1 0
+#define TENES_C2 1
+#define TENES_C3 2
+#define TENES_C4 3
+#define TENES_ETT 4
+#define TENES_ETR 5
+#define TENES_ETB 6
+#define TENES_ETL 7
+#define TENES_TN 8
+int tenes_env_create(tenes_ctx *ctx, int LX, int LY, int skew,
+                     tenes_env **out);
+int tenes_env_destroy(tenes_env *env);
+/* Set/get one tensor of the environment (kind = TENES_C1 .. TENES_TN).
+ * Getting returns a new handle sharing the data (destroy it when done). */
+int tenes_env_set(tenes_env *env, int kind, int site, const tenes_tensor *t);
+int tenes_env_get(tenes_env *env, int kind, int site, tenes_tensor **out);
+/* Calc_CTM_Environment, ctm.cpp:971-1253 (initialize != 0: build C/eT
+ * from Tn first).  *count = sweeps done, *sig_max = last convergence
+ * measure (either may be NULL). */
+int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
+                               int initialize, int *count, double *sig_max);
+/* Left_move / Right_move / Top_move / Bottom_move, ctm.cpp:562-813.
+ * dir: 0 left, 1 right, 2 top, 3 bottom; coord = ix (left/right) or iy. */
+int tenes_env_move(tenes_env *env, int dir, int coord,
+                   const tenes_peps_parameters *p);
+/* Check_Convergence_CTM, ctm.cpp:815-969, against corners saved with
+ * tenes_env_save_corners.  *converged = 0/1. */
+int tenes_env_save_corners(tenes_env *env);
+int tenes_env_check_convergence(tenes_env *env, const tenes_peps_parameters *p,
+                                int *converged, double *sig_max);
+/* L2-normalised singular spectrum of one corner (diagnostics):
+ * writes min(chi, 128) values, *n_out receives the count. */
+int tenes_env_corner_spectrum(tenes_env *env, int kind, int site, double *out,
+                              int *n_out);
+
+/* ---- environment contractions (contract_itps_ctm.hpp:45-160) -------------
+ * Operators are host arrays, row-major: op (d, d); op12 (d1, d2, d1, d2). */
+/* Contract_one_site_iTPS_CTM, contract_itps_ctm/ctm.cpp:89-113, on the
+ * environment of `site` (onesite_obs.cpp:86-97). */
+int tenes_env_contract_one_site(tenes_env *env, int site, const double *op,
+                                double *out);
+/* Contract_two_sites_{horizontal,vertical}_op12_iTPS_CTM
+ * (contract_itps_ctm/ctm.cpp:213-319) for the nearest-neighbour bond
+ * (source, dx, dy) with |dx| + |dy| == 1, using the tensor selection of
+ * twosite_obs.cpp:146-222 (operator legs swapped when source is the right
+ * or bottom site). */
+int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
+                                      int dy, const double *op12,
+                                      double *out);
+/* Contract_iTPS_CTM (contract_itps_ctm/ctm.cpp:45-87) for a 1x1, 1x2 or 2x1
+ * cluster around the bond (source, dx, dy), with a one-site operator on the
+ * source and one on the target (NULL = identity), twosite_obs.cpp:146-260.
+ * With both operators NULL this is the bond norm of twosite_obs.cpp:168. */
+int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
+                               const double *op_source,
+                               const double *op_target, double *out);
+/* Create_Environment_two_sites, core/full_update.cpp:51-83 */
+int tenes_create_environment_two_sites(const tenes_tensor *const *T,
+                                       tenes_tensor **out);
+/* T[12] = {C1,C2,C3,C4, eT1..eT6, Q1, Q2} */
+
+/* ---- transfer matrix (transfer_matrix.hpp:49-139, arnoldi.hpp:32-71) ---- */
+typedef struct tenes_tm_parameters {
+  int num_eigvals;
+  int maxdim_dense_eigensolver;
+  int arnoldi_maxdim;
+  int arnoldi_restartdim;
+  int arnoldi_maxiter;
+  double arnoldi_rtol;
+} tenes_tm_parameters;
+void tenes_tm_parameters_default(tenes_tm_parameters *p);
+/* TransferMatrix_ctm::eigenvalues, transfer_matrix.cpp:84-142.
+ * eig: 2*num_eigvals doubles (re, im), sorted by |lambda| descending;
+ * *n_out receives the number of eigenvalues written. */
+int tenes_env_transfer_matrix_eigenvalues(tenes_env *env, int dir,
+                                          int fixed_coord,
+                                          const tenes_tm_parameters *p,
+                                          double *eig, int *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENES_AMD_H */

@@ -1,0 +1,132 @@
+"""CTM-environment contractions restated in numpy
+(TeNeS src/iTPS/core/contract_itps_ctm/ctm.cpp, ctm_1x1.cpp, ctm_1x2.cpp,
+ctm_2x1.cpp, src/iTPS/core/full_update.cpp:51-83)."""
+from __future__ import annotations
+
+import numpy as np
+
+td = np.tensordot
+cj = np.conj
+
+
+def trace(A, B, axA, axB):
+    """mptensor trace(A, B, axA, axB): full contraction to a scalar."""
+    return complex(td(A, B, (list(axA), list(axB))))
+
+
+def Contract_one_site_iTPS_CTM(C1, C2, C3, C4, eT1, eT2, eT3, eT4, Tn1, op1):
+    """contract_itps_ctm/ctm.cpp:89-113"""
+    return trace(
+        td(td(td(td(C2, td(C3, eT2, ([0], [1])), ([1], [1])), td(C1, eT1, ([1], [0])), ([0], [1])),
+              td(td(td(C4, eT4, ([1], [0])), eT3, ([0], [1])), cj(Tn1), ([2, 5], [0, 3])),
+              ([0, 2, 3, 5], [2, 5, 0, 4])),
+           Tn1, ([0, 1, 2, 3], [2, 1, 0, 3])),
+        op1, [0, 1], [1, 0])
+
+
+def Contract_two_sites_horizontal_iTPS_CTM(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, Tn1, Tn2,
+                                           op1, op2):
+    """contract_itps_ctm/ctm.cpp:115-163"""
+    rh = td(Tn2, td(td(td(C3, eT4, ([1], [0])), eT3, ([0], [1])), td(cj(Tn2), op2, ([4], [1])),
+                    ([2, 5], [3, 2])), ([2, 3, 4], [3, 1, 6]))
+    R = td(td(td(C2, eT2, ([0], [1])), rh, ([0, 2, 3], [3, 1, 5])), eT5, ([2], [0]))
+    L = td(td(C1, eT1, ([1], [0])), td(C4, eT6, ([1], [0])), ([0], [1]))
+    X = td(Tn1, td(L, td(cj(Tn1), R, ([2, 3], [2, 5])), ([0, 2, 3, 5], [3, 1, 5, 0])),
+           ([0, 1, 2, 3], [1, 0, 3, 4]))
+    return trace(X, op1, [0, 1], [0, 1])
+
+
+def Contract_two_sites_vertical_iTPS_CTM(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, Tn1, Tn2,
+                                         op1, op2):
+    """contract_itps_ctm/ctm.cpp:165-211"""
+    lh = td(Tn2, td(td(td(C4, eT5, ([1], [0])), eT4, ([0], [1])), td(cj(Tn2), op2, ([4], [1])),
+                    ([2, 5], [0, 3])), ([0, 3, 4], [1, 3, 6]))
+    B = td(td(td(C3, eT3, ([0], [1])), lh, ([0, 2, 3], [3, 1, 5])), eT6, ([2], [0]))
+    T = td(td(C2, eT2, ([1], [0])), td(C1, eT1, ([1], [0])), ([0], [1]))
+    X = td(Tn1, td(T, td(cj(Tn1), B, ([0, 3], [5, 2])), ([0, 2, 3, 5], [3, 1, 5, 0])),
+           ([0, 1, 2, 3], [4, 1, 0, 3]))
+    return trace(X, op1, [0, 1], [0, 1])
+
+
+def Contract_two_sites_horizontal_op12_iTPS_CTM(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6,
+                                                Tn1, Tn2, op12):
+    """contract_itps_ctm/ctm.cpp:213-266"""
+    left = td(eT1, td(Tn1, td(cj(Tn1), td(eT5, td(C1, td(C4, eT6, ([1], [0])), ([0], [1])),
+                                           ([1], [1])), ([0, 3], [5, 2])), ([0, 3], [6, 4])),
+              ([0, 2, 3], [7, 0, 3]))
+    right = td(eT2, td(Tn2, td(cj(Tn2), td(eT4, td(C2, td(C3, eT3, ([0], [1])), ([1], [1])),
+                                            ([0], [1])), ([2, 3], [5, 2])), ([2, 3], [6, 4])),
+               ([1, 2, 3], [7, 1, 4]))
+    return trace(op12, td(left, right, ([0, 1, 3, 5], [0, 1, 3, 5])), [0, 1, 2, 3], [0, 2, 1, 3])
+
+
+def Contract_two_sites_vertical_op12_iTPS_CTM(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6,
+                                              Tn1, Tn2, op12):
+    """contract_itps_ctm/ctm.cpp:268-319"""
+    top = td(eT2, td(Tn1, td(cj(Tn1), td(eT6, td(C1, td(C2, eT1, ([0], [1])), ([1], [1])),
+                                          ([1], [0])), ([0, 1], [2, 5])), ([0, 1], [4, 6])),
+             ([0, 2, 3], [7, 0, 3]))
+    bottom = td(eT3, td(Tn2, td(cj(Tn2), td(eT5, td(C3, td(C4, eT4, ([0], [1])), ([1], [1])),
+                                             ([0], [1])), ([0, 3], [2, 5])), ([0, 3], [4, 6])),
+                ([1, 2, 3], [7, 1, 4]))
+    return trace(op12, td(top, bottom, ([0, 1, 3, 5], [0, 1, 3, 5])), [0, 1, 2, 3], [0, 2, 1, 3])
+
+
+def _c1x1(C, eTt, eTr, eTb, eTl, Tn, op):
+    """contract_itps_ctm/ctm_1x1.cpp:27-72"""
+    T = Tn[0][0]
+    top = td(eTt[0], td(C[0], eTl[0], ([0], [1])), ([0], [0]))
+    rest = td(td(C[1], eTr[0], ([1], [0])), td(C[2], td(C[3], eTb[0], ([0], [1])), ([1], [1])),
+              ([1], [0]))
+    X = td(T, td(top, td(cj(T), rest, ([2, 3], [2, 5])), ([0, 2, 3, 5], [3, 1, 5, 0])),
+           ([0, 1, 2, 3], [1, 0, 3, 4]))
+    return trace(op[0][0], X, [0, 1], [0, 1])
+
+
+def _c1x2(C, eTt, eTr, eTb, eTl, Tn, op):
+    """contract_itps_ctm/ctm_1x2.cpp:27-87"""
+    T0, T1 = Tn[0][0], Tn[0][1]
+    renv = td(eTb[1], td(C[1], td(C[2], eTr[0], ([0], [1])), ([1], [1])), ([0], [1]))
+    site1 = td(eTt[1], td(T1, td(td(cj(T1), op[0][1], ([4], [1])), renv, ([2, 3], [5, 2])),
+                          ([2, 3, 4], [6, 4, 2])), ([1, 2, 3], [5, 1, 3]))
+    lower = td(td(C[3], eTb[0], ([0], [1])), site1, ([1], [3]))
+    top = td(eTt[0], td(C[0], eTl[0], ([0], [1])), ([0], [0]))
+    X = td(T0, td(top, td(cj(T0), lower, ([2, 3], [5, 2])), ([0, 2, 3, 5], [5, 1, 3, 0])),
+           ([0, 1, 2, 3], [1, 0, 4, 3]))
+    return trace(op[0][0], X, [0, 1], [0, 1])
+
+
+def _c2x1(C, eTt, eTr, eTb, eTl, Tn, op):
+    """contract_itps_ctm/ctm_2x1.cpp:27-87"""
+    T0, T1 = Tn[0][0], Tn[1][0]
+    benv = td(eTl[1], td(C[2], td(C[3], eTb[0], ([0], [1])), ([1], [1])), ([0], [1]))
+    site1 = td(eTr[1], td(T1, td(td(cj(T1), op[1][0], ([4], [1])), benv, ([0, 3], [2, 5])),
+                          ([0, 3, 4], [4, 6, 2])), ([1, 2, 3], [5, 1, 3]))
+    right = td(td(C[1], eTr[0], ([1], [0])), site1, ([1], [0]))
+    top = td(eTt[0], td(C[0], eTl[0], ([0], [1])), ([0], [0]))
+    X = td(T0, td(top, td(cj(T0), right, ([2, 3], [2, 4])), ([0, 2, 3, 5], [3, 1, 6, 0])),
+           ([0, 1, 2, 3], [1, 0, 3, 4]))
+    return trace(op[0][0], X, [0, 1], [0, 1])
+
+
+def Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op):
+    """contract_itps_ctm/ctm.cpp:45-87 for 1x1, 1x2, 2x1 clusters."""
+    nrow, ncol = len(Tn), len(Tn[0])
+    if (nrow, ncol) == (1, 1):
+        return _c1x1(C, eTt, eTr, eTb, eTl, Tn, op)
+    if (nrow, ncol) == (1, 2):
+        return _c1x2(C, eTt, eTr, eTb, eTl, Tn, op)
+    if (nrow, ncol) == (2, 1):
+        return _c2x1(C, eTt, eTr, eTb, eTl, Tn, op)
+    raise NotImplementedError(f"Contract_{nrow}_{ncol}_iTPS_CTM is not implemented")
+
+
+def Create_Environment_two_sites(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, Q1, Q2):
+    """core/full_update.cpp:51-83; legs (t1, t2, tc1, tc2) of Q1, Q2."""
+    rq = td(td(td(td(C3, eT4, ([1], [0])), eT3, ([0], [1])), cj(Q2), ([2, 5], [3, 2])), Q2,
+            ([1, 3], [3, 2]))
+    right = td(td(C2, eT2, ([0], [1])), rq, ([0, 2, 3], [1, 5, 3]))
+    lq = td(td(td(td(C4, eT6, ([1], [0])), eT5, ([0], [1])), cj(Q1), ([2, 5], [0, 2])), Q1,
+            ([1, 3], [0, 2]))
+    left = td(td(C1, eT1, ([1], [0])), lq, ([0, 2, 3], [0, 4, 2]))
+    return td(right, left, ([0, 1], [0, 1])).transpose(3, 1, 2, 0)

@@ -1,0 +1,128 @@
+"""Measurement drivers restated in numpy: one-site and two-site observables
+and the energy density (TeNeS src/iTPS/onesite_obs.cpp:35-123,
+twosite_obs.cpp:31-288, density.cpp:30-140) and the correlation length
+(correlation_length.cpp:24-124).  Operators are given as plain dicts:
+
+  onesite: {"name", "group", "site", "op" (d, d), "coeff"}
+  twosite: {"name", "group", "source", "dx", "dy", "coeff",
+            "op" (d1, d2, d1, d2) or "ops" (one-site groups [g_source, g_target])}
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import contract as K
+from . import transfer as TM
+
+
+def measure_onesite(env, Tn, lat, onesite_ops):
+    """onesite_obs.cpp:85-98.  Returns ({group: {site: value}}, norms)."""
+    norms = []
+    for i in range(lat.N_UNIT):
+        d = Tn[i].shape[4]
+        norms.append(K.Contract_one_site_iTPS_CTM(env.C1[i], env.C2[i], env.C3[i], env.C4[i], env.eTt[i],
+                                                  env.eTr[i], env.eTb[i], env.eTl[i], Tn[i], np.eye(d)))
+    out = {}
+    for op in onesite_ops:
+        i = op["site"]
+        val = K.Contract_one_site_iTPS_CTM(env.C1[i], env.C2[i], env.C3[i], env.C4[i], env.eTt[i],
+                                           env.eTr[i], env.eTb[i], env.eTl[i], Tn[i], op["op"])
+        out.setdefault(op["group"], {})[i] = op.get("coeff", 1.0) * val / norms[i]
+    return out, norms
+
+
+def cluster(lat, source, dx, dy):
+    """indices[row][col] of twosite_obs.cpp:119-176."""
+    ncol = abs(dx) + 1
+    nrow = abs(dy) + 1
+    source_col, target_col = (0, ncol - 1) if dx >= 0 else (ncol - 1, 0)
+    source_row, target_row = (nrow - 1, 0) if dy >= 0 else (0, nrow - 1)
+    idx = [[lat.other(source, col - source_col, source_row - row) for col in range(ncol)]
+           for row in range(nrow)]
+    return nrow, ncol, source_row, source_col, target_row, target_col, idx
+
+
+def _cluster_args(env, idx, nrow, ncol):
+    C = [env.C1[idx[0][0]], env.C2[idx[0][ncol - 1]], env.C3[idx[nrow - 1][ncol - 1]], env.C4[idx[nrow - 1][0]]]
+    eTt = [env.eTt[idx[0][c]] for c in range(ncol)]
+    eTb = [env.eTb[idx[nrow - 1][c]] for c in range(ncol)]
+    eTl = [env.eTl[idx[r][0]] for r in range(nrow)]
+    eTr = [env.eTr[idx[r][ncol - 1]] for r in range(nrow)]
+    return C, eTt, eTr, eTb, eTl
+
+
+def measure_twosite(env, Tn, lat, twosite_ops, onesite_ops):
+    """twosite_obs.cpp:46-262.  Returns ({group: {(source, dx, dy): value}}, norms)."""
+    site_op = {(o["site"], o["group"]): o["op"] for o in onesite_ops}
+    norms = {}
+    out = {}
+    for op in twosite_ops:
+        source, dx, dy = op["source"], op["dx"], op["dy"]
+        nrow, ncol, srow, scol, trow, tcol, idx = cluster(lat, source, dx, dy)
+        if ncol > 4 or nrow > 4:
+            continue
+        Tn_ = [[Tn[idx[r][c]] for c in range(ncol)] for r in range(nrow)]
+        ops_ = [[np.eye(Tn[idx[r][c]].shape[4]) for c in range(ncol)] for r in range(nrow)]
+        C, eTt, eTr, eTb, eTl = _cluster_args(env, idx, nrow, ncol)
+        key = (idx[nrow - 1][0], nrow - 1, ncol - 1)
+        if key not in norms:
+            norms[key] = K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
+        norm = norms[key]
+        if "op" in op:
+            if nrow * ncol != 2:
+                raise NotImplementedError("explicit two-site operators only on nearest-neighbour bonds")
+            o = op["op"]
+            if nrow == 2:
+                top, bottom = idx[0][0], idx[1][0]
+                o = o if top == source else o.transpose(1, 0, 3, 2)
+                value = K.Contract_two_sites_vertical_op12_iTPS_CTM(
+                    env.C1[top], env.C2[top], env.C3[bottom], env.C4[bottom], env.eTt[top], env.eTr[top],
+                    env.eTr[bottom], env.eTb[bottom], env.eTl[bottom], env.eTl[top], Tn[top], Tn[bottom], o)
+            else:
+                left, right = idx[0][0], idx[0][1]
+                o = o if left == source else o.transpose(1, 0, 3, 2)
+                value = K.Contract_two_sites_horizontal_op12_iTPS_CTM(
+                    env.C1[left], env.C2[right], env.C3[right], env.C4[left], env.eTt[left], env.eTt[right],
+                    env.eTr[right], env.eTb[right], env.eTb[left], env.eTl[left], Tn[left], Tn[right], o)
+        else:
+            g0, g1 = op["ops"]
+            ops_[srow][scol] = site_op[(source, g0)]
+            target = lat.other(source, dx, dy)
+            ops_[trow][tcol] = site_op[(target, g1)]
+            value = K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
+        out.setdefault(op["group"], {})[(source, dx, dy)] = op.get("coeff", 1.0) * value / norm
+    return out, norms
+
+
+def density(onesite_vals, twosite_vals, lat, pdims, onesite_names, twosite_names):
+    """density.cpp:39-76: per-site averages and the energy
+    (site_hamiltonian + bond_hamiltonian groups)."""
+    nsites = sum(1 for i in range(lat.N_UNIT) if pdims[i] > 1)
+    res = {}
+    energy = 0.0
+    for g, name in onesite_names.items():
+        v = sum(val for i, val in onesite_vals.get(g, {}).items() if pdims[i] > 1)
+        res[name] = v / nsites
+        if name == "site_hamiltonian":
+            energy += np.real(v)
+    for g, name in twosite_names.items():
+        v = sum(twosite_vals.get(g, {}).values())
+        res[name] = v / nsites
+        if name == "bond_hamiltonian":
+            energy += np.real(v)
+    res["Energy"] = energy / nsites
+    return res
+
+
+def correlation_lengths(env, Tn, lat, prm: TM.TMParameters):
+    """correlation_length.cpp:24-55, 97-124: [(dir, coord, xi, [e_i])]."""
+    tm = TM.TransferMatrixCTM(lat, Tn, env)
+    out = []
+    for d in (0, 1):
+        W = lat.LY if d == 0 else lat.LX
+        L = lat.LX_noskew if d == 0 else lat.LY_noskew
+        for fixed in range(W):
+            ev = tm.eigenvalues(d, fixed, prm)
+            xi, es = TM.correlation_length(ev, L)
+            out.append((d, fixed, xi, es, ev))
+    return out

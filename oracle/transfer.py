@@ -1,0 +1,169 @@
+"""Transfer matrix of the CTM environment and Arnoldi restated in numpy
+(TeNeS src/iTPS/transfer_matrix.cpp:84-551, src/arnoldi.cpp:34-181,
+src/tensor.cpp:25-115)."""
+from __future__ import annotations
+
+import numpy as np
+
+td = np.tensordot
+
+
+class TMParameters:
+    """transfer_matrix.hpp:32-47 defaults."""
+
+    def __init__(self, **kw):
+        self.num_eigvals = 4
+        self.maxdim_dense_eigensolver = 200
+        self.arnoldi_maxdim = 50
+        self.arnoldi_restartdim = 20
+        self.arnoldi_maxiter = 1
+        self.arnoldi_rtol = 1e-10
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise KeyError(k)
+            setattr(self, k, v)
+
+
+def eigen(A: np.ndarray, nev: int):
+    """tensor.cpp:91-115: eigenvalues (and last components of the unit-norm
+    right eigenvectors) sorted by |lambda| descending, first nev."""
+    w, V = np.linalg.eig(A)
+    order = np.argsort(-np.abs(w), kind="stable")[:nev]
+    return w[order], V[-1, order]
+
+
+class TransferMatrixCTM:
+    def __init__(self, lat, Tn, env):
+        self.lat = lat
+        self.Tn = Tn
+        self.e = env
+
+    def dim(self, d, fixed):
+        lat, e = self.lat, self.e
+        if d == 0:
+            s0 = lat.index(0, fixed)
+            s1 = lat.top(s0)
+            return e.eTt[s0].shape[0] * e.eTb[s1].shape[1]
+        s0 = lat.index(fixed, 0)
+        s1 = lat.left(s0)
+        return e.eTl[s0].shape[0] * e.eTr[s1].shape[1]
+
+    def initial_vector(self, d, fixed):
+        """transfer_matrix.cpp:162-185"""
+        lat, e = self.lat, self.e
+        if d == 0:
+            site = lat.index(0, fixed)
+            v = td(e.C1[site], e.C4[lat.top(site)], ([0], [1]))
+        else:
+            site = lat.index(fixed, 0)
+            v = td(e.C4[site], e.C3[lat.left(site)], ([0], [1]))
+        return v.reshape(-1)
+
+    def matvec(self, d, vec, fixed):
+        """transfer_matrix.cpp:187-308 (double-layer branch)."""
+        lat, e = self.lat, self.e
+        if d == 0:
+            y = fixed
+            s0 = lat.index(0, y)
+            s1 = lat.top(s0)
+            c0, c1 = e.eTt[s0].shape[0], e.eTb[s1].shape[1]
+            out = vec.reshape(c0, c1)
+            for x in range(lat.LX):
+                site = lat.index(x, y)
+                out = td(e.eTt[site], td(e.eTb[lat.top(site)], out, ([1], [1])), ([0, 2, 3], [3, 1, 2]))
+            return out.reshape(-1)
+        x = fixed
+        x0 = x
+        s0 = lat.index(x, 0)
+        s1 = lat.left(s0)
+        c0, c1 = e.eTl[s0].shape[0], e.eTr[s1].shape[1]
+        out = vec.reshape(c0, c1)
+        while True:
+            for y in range(lat.LY):
+                site = lat.index(x, y)
+                out = td(e.eTl[site], td(e.eTr[lat.left(site)], out, ([1], [1])), ([0, 2, 3], [3, 1, 2]))
+            x = (x + lat.skew + lat.LX) % lat.LX
+            if x == x0:
+                break
+        return out.reshape(-1)
+
+    def matrix(self, d, fixed):
+        """transfer_matrix.cpp:379-475 (double-layer branch)."""
+        lat, e = self.lat, self.e
+        if d == 0:
+            y = fixed
+            s0 = lat.index(0, y)
+            s1 = lat.top(s0)
+            c0, c1 = e.eTt[s0].shape[0], e.eTb[s1].shape[1]
+            res = td(e.eTt[s0], e.eTb[s1], ([2, 3], [2, 3])).transpose(0, 3, 1, 2)
+            for x in range(1, lat.LX):
+                site = lat.index(x, y)
+                res = td(res, td(e.eTt[site], e.eTb[lat.top(site)], ([2, 3], [2, 3])), ([2, 3], [0, 3]))
+            return res.reshape(c0 * c1, c0 * c1)
+        x = fixed
+        x0 = x
+        s0 = lat.index(x, 0)
+        s1 = lat.left(s0)
+        c0, c1 = e.eTl[s0].shape[0], e.eTr[s1].shape[1]
+        res = np.eye(c0 * c1, dtype=e.eTl[s0].dtype).reshape(c0, c1, c0, c1)
+        while True:
+            for y in range(lat.LY):
+                site = lat.index(x, y)
+                res = td(res, td(e.eTl[site], e.eTr[lat.left(site)], ([2, 3], [2, 3])), ([2, 3], [0, 3]))
+            x = (x + lat.skew + lat.LX) % lat.LX
+            if x == x0:
+                break
+        return res.reshape(c0 * c1, c0 * c1)
+
+    def eigenvalues(self, d, fixed, prm: TMParameters):
+        """transfer_matrix.cpp:84-142 with Arnoldi (arnoldi.cpp:56-181,
+        maxiter = 1: no restart)."""
+        N = self.dim(d, fixed)
+        nev = min(prm.num_eigvals, N)
+        if N == 1:
+            return np.array([1.0 + 0j])
+        if N <= prm.maxdim_dense_eigensolver:
+            return eigen(self.matrix(d, fixed), nev)[0]
+        maxvec = prm.arnoldi_maxdim
+        if N < maxvec:
+            maxvec = N
+        v0 = self.initial_vector(d, fixed)
+        dt = np.result_type(v0.dtype, np.float64)
+        Q = np.zeros((maxvec + 1, N), dtype=dt)
+        H = np.zeros((maxvec + 1, maxvec), dtype=dt)
+        Q[0] = v0
+
+        def orthonormalize(k):
+            for j in range(k):
+                x = np.vdot(Q[j], Q[k])
+                H[j, k - 1] = x
+                Q[k] = Q[k] - Q[j] * x
+            x = np.sqrt(np.real(np.vdot(Q[k], Q[k])))
+            Q[k] = Q[k] * (1.0 / x)
+            if k > 0:
+                H[k, k - 1] = x
+
+        def residue(k):
+            w, last = eigen(H[:k, :k], nev)
+            return np.abs(H[k, k - 1] * last) / np.abs(w)
+
+        orthonormalize(0)
+        for k in range(1, nev + 1):
+            Q[k] = self.matvec(d, Q[k - 1], fixed)
+            orthonormalize(k)
+        for k in range(nev + 1, maxvec + 1):
+            Q[k] = self.matvec(d, Q[k - 1], fixed)
+            orthonormalize(k)
+            if residue(k)[nev - 1] <= prm.arnoldi_rtol:
+                break
+        return eigen(H[:maxvec, :maxvec], nev)[0]
+
+
+def correlation_length(eigvals, L):
+    """correlation_length.cpp:97-124: xi = -L / log|t1/t0| and
+    e_i = -log|t_i/t0| / L."""
+    if len(eigvals) == 1:
+        return 0.0, []
+    e0 = abs(eigvals[0])
+    xi = -L / np.log(abs(eigvals[1]) / e0)
+    return xi, [-np.log(abs(v) / e0) / L for v in eigvals[1:]]

@@ -1,0 +1,295 @@
+"""Thin Python objects over the C ABI (include/tenes_amd.h).
+
+Every call goes to libtenes_amd.so; nothing here computes on the CPU beyond
+argument marshalling.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+KIND = {"C1": 0, "C2": 1, "C3": 2, "C4": 3, "eTt": 4, "eTr": 5, "eTb": 6, "eTl": 7, "Tn": 8}
+
+
+class TenesError(RuntimeError):
+    pass
+
+
+class Context:
+    """One GPU, one HIP stream (tenes_ctx)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None, seed: int | None = None):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.tenes_ctx_create(device, C.c_void_p(stream or 0), C.byref(h))
+        if rc != 0:
+            msg = self.lib.tenes_ctx_last_error(None)
+            raise TenesError(f"tenes_ctx_create failed ({rc}): {msg.decode() if msg else ''}")
+        self.h = h
+        if seed is not None:
+            self.check(self.lib.tenes_ctx_set_seed(self.h, C.c_uint64(seed)))
+
+    def check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.tenes_ctx_last_error(self.h)
+            raise TenesError(f"tenes error {rc}: {msg.decode() if msg else ''}")
+
+    def synchronize(self):
+        self.check(self.lib.tenes_ctx_synchronize(self.h))
+
+    def stats(self):
+        f = C.c_double()
+        n = C.c_int64()
+        self.check(self.lib.tenes_ctx_stats(self.h, C.byref(f), C.byref(n)))
+        return f.value, n.value
+
+    def reset_stats(self):
+        self.check(self.lib.tenes_ctx_reset_stats(self.h))
+
+    def tensor(self, a: np.ndarray) -> "Tensor":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        shape = np.array(a.shape, dtype=np.int64)
+        h = C.c_void_p()
+        self.check(self.lib.tenes_tensor_from_host(self.h, a.ndim, L.i64ptr(shape), L.dptr(a), C.byref(h)))
+        return Tensor(self, h)
+
+    def from_device(self, ptr: int, shape: Sequence[int]) -> "Tensor":
+        shp = np.array(shape, dtype=np.int64)
+        h = C.c_void_p()
+        self.check(self.lib.tenes_tensor_from_device(self.h, len(shp), L.i64ptr(shp), C.c_void_p(ptr), C.byref(h)))
+        return Tensor(self, h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tenes_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tensor:
+    """Device FP64 row-major tensor handle (tenes_tensor)."""
+
+    def __init__(self, ctx: Context, h: C.c_void_p):
+        self.ctx = ctx
+        self.h = h
+
+    @property
+    def shape(self) -> tuple:
+        lib = self.ctx.lib
+        r = C.c_int()
+        lib.tenes_tensor_rank(self.h, C.byref(r))
+        s = np.zeros(max(r.value, 1), dtype=np.int64)
+        lib.tenes_tensor_shape(self.h, L.i64ptr(s))
+        return tuple(int(x) for x in s[: r.value])
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.tenes_tensor_to_host(self.h, L.dptr(out)))
+        return out
+
+    def device_ptr(self) -> int:
+        p = C.c_void_p()
+        self.ctx.lib.tenes_tensor_device_ptr(self.h, C.byref(p))
+        return p.value or 0
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.ctx.lib.tenes_tensor_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+def tensordot(A: Tensor, B: Tensor, axA: Sequence[int], axB: Sequence[int],
+              perm: Sequence[int] | None = None) -> Tensor:
+    ctx = A.ctx
+    a = np.array(axA, dtype=np.int32)
+    b = np.array(axB, dtype=np.int32)
+    p = np.array(perm, dtype=np.int32) if perm is not None else None
+    h = C.c_void_p()
+    ctx.check(ctx.lib.tenes_tensordot(A.h, B.h, len(a), L.iptr(a), L.iptr(b),
+                                      L.iptr(p) if p is not None else None, C.byref(h)))
+    return Tensor(ctx, h)
+
+
+def transpose(T: Tensor, axes: Sequence[int]) -> Tensor:
+    ctx = T.ctx
+    a = np.array(axes, dtype=np.int32)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.tenes_transpose(T.h, L.iptr(a), C.byref(h)))
+    return Tensor(ctx, h)
+
+
+def svd_top(A: Tensor, k: int, power_iters: int = 4):
+    ctx = A.ctx
+    U = C.c_void_p()
+    V = C.c_void_p()
+    s = np.zeros(max(k, 1), dtype=np.float64)
+    kk = C.c_int()
+    ctx.check(ctx.lib.tenes_svd_top(A.h, k, power_iters, C.byref(U), C.byref(V), L.dptr(s), C.byref(kk)))
+    return Tensor(ctx, U), s[: kk.value].copy(), Tensor(ctx, V)
+
+
+def peps_parameters(**kw) -> L.PepsParameters:
+    """PEPS_Parameters (CTM fields) with TeNeS defaults, overridden by kw."""
+    p = L.PepsParameters()
+    L.load().tenes_peps_parameters_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def tm_parameters(**kw) -> L.TmParameters:
+    p = L.TmParameters()
+    L.load().tenes_tm_parameters_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def calc_projector_left_block(C1, C4, eT1, eT6, eT7, eT8, Tn1, Tn4, params):
+    ctx = C1.ctx
+    PU = C.c_void_p()
+    PL = C.c_void_p()
+    ctx.check(ctx.lib.tenes_calc_projector_left_block(
+        C1.h, C4.h, eT1.h, eT6.h, eT7.h, eT8.h, Tn1.h, Tn4.h, C.byref(params), C.byref(PU), C.byref(PL)))
+    return Tensor(ctx, PU), Tensor(ctx, PL)
+
+
+def calc_projector_updown_blocks(tensors: Sequence[Tensor], params):
+    ctx = tensors[0].ctx
+    arr = (C.c_void_p * 16)(*[t.h for t in tensors])
+    PU = C.c_void_p()
+    PL = C.c_void_p()
+    ctx.check(ctx.lib.tenes_calc_projector_updown_blocks(arr, C.byref(params), C.byref(PU), C.byref(PL)))
+    return Tensor(ctx, PU), Tensor(ctx, PL)
+
+
+def calc_next_ctm(C1, C4, eT1, eT6, PU, PL):
+    ctx = C1.ctx
+    a = C.c_void_p()
+    b = C.c_void_p()
+    ctx.check(ctx.lib.tenes_calc_next_ctm(C1.h, C4.h, eT1.h, eT6.h, PU.h, PL.h, C.byref(a), C.byref(b)))
+    return Tensor(ctx, a), Tensor(ctx, b)
+
+
+def calc_next_et(eT8, Tn1, PU, PL):
+    ctx = eT8.ctx
+    a = C.c_void_p()
+    ctx.check(ctx.lib.tenes_calc_next_et(eT8.h, Tn1.h, PU.h, PL.h, C.byref(a)))
+    return Tensor(ctx, a)
+
+
+def create_environment_two_sites(tensors: Sequence[Tensor]) -> Tensor:
+    ctx = tensors[0].ctx
+    arr = (C.c_void_p * 12)(*[t.h for t in tensors])
+    out = C.c_void_p()
+    ctx.check(ctx.lib.tenes_create_environment_two_sites(arr, C.byref(out)))
+    return Tensor(ctx, out)
+
+
+class Environment:
+    """The iTPS tensor arrays (Tn, C1..C4, eTt/eTr/eTb/eTl) on a square
+    lattice unit cell, resident on the GPU (tenes_env)."""
+
+    def __init__(self, ctx: Context, LX: int, LY: int, skew: int = 0):
+        self.ctx = ctx
+        self.LX, self.LY, self.skew = LX, LY, skew
+        h = C.c_void_p()
+        ctx.check(ctx.lib.tenes_env_create(ctx.h, LX, LY, skew, C.byref(h)))
+        self.h = h
+        self.N = LX * LY
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.ctx.lib.tenes_env_destroy(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+    def set(self, kind: str, site: int, t):
+        if isinstance(t, np.ndarray):
+            t = self.ctx.tensor(t)
+        self.ctx.check(self.ctx.lib.tenes_env_set(self.h, KIND[kind], site, t.h))
+
+    def get(self, kind: str, site: int) -> Tensor:
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.tenes_env_get(self.h, KIND[kind], site, C.byref(h)))
+        return Tensor(self.ctx, h)
+
+    def set_site_tensors(self, Tn: Sequence[np.ndarray]):
+        for i, t in enumerate(Tn):
+            self.set("Tn", i, t)
+
+    def calc_ctm_environment(self, params, initialize: bool = True):
+        n = C.c_int()
+        sig = C.c_double()
+        self.ctx.check(self.ctx.lib.tenes_calc_ctm_environment(
+            self.h, C.byref(params), 1 if initialize else 0, C.byref(n), C.byref(sig)))
+        return n.value, sig.value
+
+    def move(self, direction: int, coord: int, params):
+        self.ctx.check(self.ctx.lib.tenes_env_move(self.h, direction, coord, C.byref(params)))
+
+    def save_corners(self):
+        self.ctx.check(self.ctx.lib.tenes_env_save_corners(self.h))
+
+    def check_convergence(self, params):
+        c = C.c_int()
+        s = C.c_double()
+        self.ctx.check(self.ctx.lib.tenes_env_check_convergence(self.h, C.byref(params), C.byref(c), C.byref(s)))
+        return bool(c.value), s.value
+
+    def corner_spectrum(self, kind: str, site: int) -> np.ndarray:
+        out = np.zeros(256, dtype=np.float64)
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.tenes_env_corner_spectrum(self.h, KIND[kind], site, L.dptr(out), C.byref(n)))
+        return out[: n.value].copy()
+
+    def contract_one_site(self, site: int, op: np.ndarray | None = None) -> float:
+        out = C.c_double()
+        o = np.ascontiguousarray(op, dtype=np.float64) if op is not None else None
+        self.ctx.check(self.ctx.lib.tenes_env_contract_one_site(
+            self.h, site, L.dptr(o) if o is not None else None, C.byref(out)))
+        return out.value
+
+    def contract_two_sites_op12(self, source: int, dx: int, dy: int, op12: np.ndarray) -> float:
+        out = C.c_double()
+        o = np.ascontiguousarray(op12, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.tenes_env_contract_two_sites_op12(
+            self.h, source, dx, dy, L.dptr(o), C.byref(out)))
+        return out.value
+
+    def contract_cluster(self, source: int, dx: int, dy: int,
+                         op_source: np.ndarray | None = None,
+                         op_target: np.ndarray | None = None) -> float:
+        out = C.c_double()
+        a = np.ascontiguousarray(op_source, dtype=np.float64) if op_source is not None else None
+        b = np.ascontiguousarray(op_target, dtype=np.float64) if op_target is not None else None
+        self.ctx.check(self.ctx.lib.tenes_env_contract_cluster(
+            self.h, source, dx, dy, L.dptr(a) if a is not None else None,
+            L.dptr(b) if b is not None else None, C.byref(out)))
+        return out.value
+
+    def transfer_matrix_eigenvalues(self, direction: int, fixed: int, params=None) -> np.ndarray:
+        params = params or tm_parameters()
+        out = np.zeros(2 * max(params.num_eigvals, 1), dtype=np.float64)
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.tenes_env_transfer_matrix_eigenvalues(
+            self.h, direction, fixed, C.byref(params), L.dptr(out), C.byref(n)))
+        return out[0: 2 * n.value: 2] + 1j * out[1: 2 * n.value: 2]

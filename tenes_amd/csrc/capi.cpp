@@ -1,0 +1,654 @@
+// extern "C" boundary: handles around tn::Context / tn::Tensor and the
+// environment arrays, exception-free, one status code per call.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tenes_amd.h"
+#include "contract.hpp"
+#include "ctm.hpp"
+#include "linalg.hpp"
+#include "transfer.hpp"
+
+struct tenes_ctx {
+  std::unique_ptr<tn::Context> c;
+  std::string err;
+};
+
+struct tenes_tensor {
+  tn::Tensor t;
+  tenes_ctx *owner;
+};
+
+struct tenes_env {
+  tenes_ctx *ctx;
+  tenes::itps::SquareLattice lat;
+  std::vector<tn::Tensor> arr[9];  // C1..C4, eTt, eTr, eTb, eTl, Tn
+  std::vector<tn::Tensor> saved[4];
+};
+
+namespace {
+
+using tenes::itps::PEPS_Parameters;
+
+thread_local std::string g_noctx_err;
+
+template <class F>
+int guard(tenes_ctx *ctx, F &&f) {
+  try {
+    f();
+    return TENES_OK;
+  } catch (const tn::Error &e) {
+    (ctx ? ctx->err : g_noctx_err) = e.what();
+    return e.code < 0 ? e.code : TENES_ERR_GENERIC;
+  } catch (const std::exception &e) {
+    (ctx ? ctx->err : g_noctx_err) = e.what();
+    return TENES_ERR_GENERIC;
+  } catch (...) {
+    (ctx ? ctx->err : g_noctx_err) = "unknown error";
+    return TENES_ERR_GENERIC;
+  }
+}
+
+tenes_tensor *wrap(tenes_ctx *ctx, const tn::Tensor &t) {
+  return new tenes_tensor{t, ctx};
+}
+
+tn::Shape to_shape(int rank, const int64_t *shape) {
+  TN_REQUIRE(rank >= 0 && rank <= 16, "rank out of range");
+  TN_REQUIRE(rank == 0 || shape != nullptr, "null shape");
+  return tn::Shape(shape, shape + rank);
+}
+
+PEPS_Parameters to_params(const tenes_peps_parameters *p) {
+  PEPS_Parameters q;
+  if (!p) return q;
+  q.CHI = p->CHI;
+  q.Inverse_projector_cut = p->Inverse_projector_cut;
+  q.CTM_Convergence_Epsilon = p->CTM_Convergence_Epsilon;
+  q.Max_CTM_Iteration = p->Max_CTM_Iteration;
+  q.CTM_Projector_corner = p->CTM_Projector_corner != 0;
+  q.Use_RSVD = p->Use_RSVD != 0;
+  q.RSVD_Oversampling_factor = p->RSVD_Oversampling_factor;
+  q.print_level = p->print_level;
+  q.svd_power_iters = p->svd_power_iters;
+  q.rsvd_power_iters = p->rsvd_power_iters;
+  TN_REQUIRE(q.CHI >= 1, "CHI must be positive");
+  TN_REQUIRE(q.RSVD_Oversampling_factor >= 1.0,
+             "RSVD oversampling factor must be >= 1");
+  return q;
+}
+
+const tn::Tensor &T(const tenes_tensor *t) {
+  TN_REQUIRE(t != nullptr && t->t.defined(), "null tensor handle");
+  return t->t;
+}
+
+tenes_ctx *ctx_of(const tenes_tensor *t) {
+  TN_REQUIRE(t != nullptr, "null tensor handle");
+  return t->owner;
+}
+
+tn::Tensor host_op(tenes_ctx *ctx, const double *op, int64_t d) {
+  TN_REQUIRE(op != nullptr, "null operator");
+  return tn::from_host(*ctx->c, tn::Shape{d, d}, op);
+}
+
+tn::Tensor identity_op(tenes_ctx *ctx, int64_t d) {
+  std::vector<double> id((size_t)(d * d), 0.0);
+  for (int64_t i = 0; i < d; ++i) id[(size_t)(i * d + i)] = 1.0;
+  return tn::from_host(*ctx->c, tn::Shape{d, d}, id.data());
+}
+
+void check_env(const tenes_env *env) {
+  TN_REQUIRE(env != nullptr, "null environment");
+  for (int k = 0; k < 9; ++k)
+    for (const auto &t : env->arr[k])
+      TN_REQUIRE(t.defined(), "environment tensor not set (kind " +
+                                  std::to_string(k) + ")");
+}
+
+// indices[row][col] of twosite_obs.cpp:119-176 for a bond (source, dx, dy)
+struct Cluster {
+  int nrow, ncol, source_row, source_col, target_row, target_col;
+  std::vector<std::vector<int>> idx;
+};
+
+Cluster cluster_of(const tenes::itps::SquareLattice &lat, int source, int dx,
+                   int dy) {
+  Cluster c;
+  c.ncol = std::abs(dx) + 1;
+  c.nrow = std::abs(dy) + 1;
+  if (dx >= 0) {
+    c.source_col = 0;
+    c.target_col = c.ncol - 1;
+  } else {
+    c.source_col = c.ncol - 1;
+    c.target_col = 0;
+  }
+  if (dy >= 0) {
+    c.source_row = c.nrow - 1;
+    c.target_row = 0;
+  } else {
+    c.source_row = 0;
+    c.target_row = c.nrow - 1;
+  }
+  c.idx.assign(c.nrow, std::vector<int>(c.ncol));
+  for (int row = 0; row < c.nrow; ++row)
+    for (int col = 0; col < c.ncol; ++col)
+      c.idx[row][col] =
+          lat.other(source, col - c.source_col, c.source_row - row);
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tenes_abi_version(void) { return 100; }
+
+int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
+  return guard(nullptr, [&] {
+    TN_REQUIRE(out != nullptr, "null output pointer");
+    auto *c = new tenes_ctx;
+    try {
+      c->c.reset(new tn::Context(device, static_cast<hipStream_t>(hip_stream)));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int tenes_ctx_destroy(tenes_ctx *ctx) {
+  delete ctx;
+  return TENES_OK;
+}
+
+const char *tenes_ctx_last_error(const tenes_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_noctx_err.c_str();
+}
+
+int tenes_ctx_synchronize(tenes_ctx *ctx) {
+  return guard(ctx, [&] { ctx->c->sync(); });
+}
+
+int tenes_ctx_set_seed(tenes_ctx *ctx, uint64_t seed) {
+  return guard(ctx, [&] { ctx->c->seed = seed; });
+}
+
+int tenes_ctx_stats(tenes_ctx *ctx, double *gemm_flops, int64_t *gemm_calls) {
+  return guard(ctx, [&] {
+    if (gemm_flops) *gemm_flops = ctx->c->gemm_flops;
+    if (gemm_calls) *gemm_calls = ctx->c->gemm_calls;
+  });
+}
+
+int tenes_ctx_reset_stats(tenes_ctx *ctx) {
+  return guard(ctx, [&] {
+    ctx->c->gemm_flops = 0.0;
+    ctx->c->gemm_calls = 0;
+  });
+}
+
+int tenes_tensor_from_host(tenes_ctx *ctx, int rank, const int64_t *shape,
+                           const double *data, tenes_tensor **out) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(out && data, "null pointer");
+    *out = wrap(ctx, tn::from_host(*ctx->c, to_shape(rank, shape), data));
+  });
+}
+
+int tenes_tensor_from_device(tenes_ctx *ctx, int rank, const int64_t *shape,
+                             double *dev_ptr, tenes_tensor **out) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(out && dev_ptr, "null pointer");
+    *out = wrap(ctx, tn::borrow(*ctx->c, to_shape(rank, shape), dev_ptr));
+  });
+}
+
+int tenes_tensor_zeros(tenes_ctx *ctx, int rank, const int64_t *shape,
+                       tenes_tensor **out) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(out, "null pointer");
+    *out = wrap(ctx, tn::zeros(*ctx->c, to_shape(rank, shape)));
+  });
+}
+
+int tenes_tensor_rank(const tenes_tensor *t, int *rank) {
+  if (!t || !rank) return TENES_ERR_ARGUMENT;
+  *rank = t->t.rank();
+  return TENES_OK;
+}
+
+int tenes_tensor_shape(const tenes_tensor *t, int64_t *shape) {
+  if (!t || !shape) return TENES_ERR_ARGUMENT;
+  for (int i = 0; i < t->t.rank(); ++i) shape[i] = t->t.shape[i];
+  return TENES_OK;
+}
+
+int tenes_tensor_to_host(const tenes_tensor *t, double *out) {
+  if (!t) return TENES_ERR_ARGUMENT;
+  return guard(t->owner, [&] {
+    TN_REQUIRE(out != nullptr, "null output");
+    tn::to_host(t->t, out);
+  });
+}
+
+int tenes_tensor_device_ptr(const tenes_tensor *t, double **out) {
+  if (!t || !out) return TENES_ERR_ARGUMENT;
+  *out = t->t.data();
+  return TENES_OK;
+}
+
+int tenes_tensor_destroy(tenes_tensor *t) {
+  delete t;
+  return TENES_OK;
+}
+
+int tenes_tensordot(const tenes_tensor *A, const tenes_tensor *B, int nax,
+                    const int *axA, const int *axB, const int *perm,
+                    tenes_tensor **out) {
+  if (!A || !B) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(A);
+  return guard(ctx, [&] {
+    TN_REQUIRE(out != nullptr, "null output");
+    TN_REQUIRE(nax >= 0 && (nax == 0 || (axA && axB)), "bad axes");
+    tn::Axes a(axA, axA + nax), b(axB, axB + nax), pm;
+    if (perm) {
+      const int rr = T(A).rank() + T(B).rank() - 2 * nax;
+      pm.assign(perm, perm + rr);
+    }
+    *out = wrap(ctx, tn::tensordot_t(T(A), T(B), a, b, pm));
+  });
+}
+
+int tenes_transpose(const tenes_tensor *Tt, const int *axes,
+                    tenes_tensor **out) {
+  if (!Tt) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(Tt);
+  return guard(ctx, [&] {
+    TN_REQUIRE(out && axes, "null pointer");
+    tn::Axes ax(axes, axes + T(Tt).rank());
+    *out = wrap(ctx, tn::transpose(T(Tt), ax));
+  });
+}
+
+int tenes_svd_top(const tenes_tensor *A, int k, int power_iters,
+                  tenes_tensor **U, tenes_tensor **V, double *s, int *k_out) {
+  if (!A) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(A);
+  return guard(ctx, [&] {
+    tn::SvdOptions o;
+    o.power_iters = power_iters;
+    tn::SvdResult r = tn::svd_top(*ctx->c, T(A), k, o);
+    if (U) *U = wrap(ctx, r.U);
+    if (V) *V = wrap(ctx, r.V);
+    if (s) std::memcpy(s, r.s.data(), r.s.size() * sizeof(double));
+    if (k_out) *k_out = (int)r.s.size();
+  });
+}
+
+void tenes_peps_parameters_default(tenes_peps_parameters *p) {
+  if (!p) return;
+  PEPS_Parameters q;
+  p->CHI = q.CHI;
+  p->Inverse_projector_cut = q.Inverse_projector_cut;
+  p->CTM_Convergence_Epsilon = q.CTM_Convergence_Epsilon;
+  p->Max_CTM_Iteration = q.Max_CTM_Iteration;
+  p->CTM_Projector_corner = q.CTM_Projector_corner ? 1 : 0;
+  p->Use_RSVD = q.Use_RSVD ? 1 : 0;
+  p->RSVD_Oversampling_factor = q.RSVD_Oversampling_factor;
+  p->print_level = q.print_level;
+  p->svd_power_iters = q.svd_power_iters;
+  p->rsvd_power_iters = q.rsvd_power_iters;
+}
+
+int tenes_calc_projector_left_block(
+    const tenes_tensor *C1, const tenes_tensor *C4, const tenes_tensor *eT1,
+    const tenes_tensor *eT6, const tenes_tensor *eT7, const tenes_tensor *eT8,
+    const tenes_tensor *Tn1, const tenes_tensor *Tn4,
+    const tenes_peps_parameters *p, tenes_tensor **PU, tenes_tensor **PL) {
+  if (!C1) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(C1);
+  return guard(ctx, [&] {
+    TN_REQUIRE(PU && PL, "null output");
+    tn::Tensor pu, pl;
+    tenes::itps::core::Calc_projector_left_block(
+        T(C1), T(C4), T(eT1), T(eT6), T(eT7), T(eT8), T(Tn1), T(Tn4),
+        to_params(p), pu, pl);
+    *PU = wrap(ctx, pu);
+    *PL = wrap(ctx, pl);
+  });
+}
+
+int tenes_calc_projector_updown_blocks(const tenes_tensor *const *Ts,
+                                       const tenes_peps_parameters *p,
+                                       tenes_tensor **PU, tenes_tensor **PL) {
+  if (!Ts || !Ts[0]) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(Ts[0]);
+  return guard(ctx, [&] {
+    TN_REQUIRE(PU && PL, "null output");
+    tn::Tensor pu, pl;
+    tenes::itps::core::Calc_projector_updown_blocks(
+        T(Ts[0]), T(Ts[1]), T(Ts[2]), T(Ts[3]), T(Ts[4]), T(Ts[5]), T(Ts[6]),
+        T(Ts[7]), T(Ts[8]), T(Ts[9]), T(Ts[10]), T(Ts[11]), T(Ts[12]),
+        T(Ts[13]), T(Ts[14]), T(Ts[15]), to_params(p), pu, pl);
+    *PU = wrap(ctx, pu);
+    *PL = wrap(ctx, pl);
+  });
+}
+
+int tenes_calc_next_ctm(const tenes_tensor *C1, const tenes_tensor *C4,
+                        const tenes_tensor *eT1, const tenes_tensor *eT6,
+                        const tenes_tensor *PU, const tenes_tensor *PL,
+                        tenes_tensor **C1_out, tenes_tensor **C4_out) {
+  if (!C1) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(C1);
+  return guard(ctx, [&] {
+    TN_REQUIRE(C1_out && C4_out, "null output");
+    tn::Tensor a, b;
+    tenes::itps::core::Calc_Next_CTM(T(C1), T(C4), T(eT1), T(eT6), T(PU),
+                                     T(PL), a, b);
+    *C1_out = wrap(ctx, a);
+    *C4_out = wrap(ctx, b);
+  });
+}
+
+int tenes_calc_next_et(const tenes_tensor *eT8, const tenes_tensor *Tn1,
+                       const tenes_tensor *PU, const tenes_tensor *PL,
+                       tenes_tensor **eT_out) {
+  if (!eT8) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(eT8);
+  return guard(ctx, [&] {
+    TN_REQUIRE(eT_out, "null output");
+    tn::Tensor e;
+    tenes::itps::core::Calc_Next_eT(T(eT8), T(Tn1), T(PU), T(PL), e);
+    *eT_out = wrap(ctx, e);
+  });
+}
+
+int tenes_env_create(tenes_ctx *ctx, int LX, int LY, int skew,
+                     tenes_env **out) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(out != nullptr, "null output");
+    auto *e = new tenes_env;
+    e->ctx = ctx;
+    e->lat = tenes::itps::SquareLattice(LX, LY, skew);
+    for (int k = 0; k < 9; ++k) e->arr[k].resize(e->lat.N_UNIT);
+    *out = e;
+  });
+}
+
+int tenes_env_destroy(tenes_env *env) {
+  delete env;
+  return TENES_OK;
+}
+
+int tenes_env_set(tenes_env *env, int kind, int site, const tenes_tensor *t) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    TN_REQUIRE(kind >= 0 && kind < 9, "bad tensor kind");
+    TN_REQUIRE(site >= 0 && site < env->lat.N_UNIT, "bad site index");
+    env->arr[kind][site] = T(t);
+  });
+}
+
+int tenes_env_get(tenes_env *env, int kind, int site, tenes_tensor **out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    TN_REQUIRE(out != nullptr, "null output");
+    TN_REQUIRE(kind >= 0 && kind < 9, "bad tensor kind");
+    TN_REQUIRE(site >= 0 && site < env->lat.N_UNIT, "bad site index");
+    TN_REQUIRE(env->arr[kind][site].defined(), "tensor not set");
+    *out = wrap(env->ctx, env->arr[kind][site]);
+  });
+}
+
+int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
+                               int initialize, int *count, double *sig_max) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    for (const auto &t : env->arr[TENES_TN])
+      TN_REQUIRE(t.defined(), "site tensor not set");
+    double sm = 0.0;
+    auto &a = env->arr;
+    const int n = tenes::itps::core::Calc_CTM_Environment(
+        a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], to_params(p),
+        env->lat, initialize != 0, &sm);
+    if (count) *count = n;
+    if (sig_max) *sig_max = sm;
+  });
+}
+
+int tenes_env_move(tenes_env *env, int dir, int coord,
+                   const tenes_peps_parameters *p) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    auto &a = env->arr;
+    const auto q = to_params(p);
+    switch (dir) {
+      case 0:
+        tenes::itps::core::Left_move(a[0], a[1], a[2], a[3], a[4], a[5], a[6],
+                                     a[7], a[8], coord, q, env->lat);
+        break;
+      case 1:
+        tenes::itps::core::Right_move(a[0], a[1], a[2], a[3], a[4], a[5], a[6],
+                                      a[7], a[8], coord, q, env->lat);
+        break;
+      case 2:
+        tenes::itps::core::Top_move(a[0], a[1], a[2], a[3], a[4], a[5], a[6],
+                                    a[7], a[8], coord, q, env->lat);
+        break;
+      case 3:
+        tenes::itps::core::Bottom_move(a[0], a[1], a[2], a[3], a[4], a[5],
+                                       a[6], a[7], a[8], coord, q, env->lat);
+        break;
+      default:
+        throw tn::Error("bad move direction", TENES_ERR_ARGUMENT);
+    }
+  });
+}
+
+int tenes_env_save_corners(tenes_env *env) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    for (int k = 0; k < 4; ++k) env->saved[k] = env->arr[k];
+  });
+}
+
+int tenes_env_check_convergence(tenes_env *env, const tenes_peps_parameters *p,
+                                int *converged, double *sig_max) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    for (int k = 0; k < 4; ++k)
+      TN_REQUIRE(env->saved[k].size() == env->arr[k].size(),
+                 "corners were not saved");
+    double sm = 0.0;
+    const bool c = tenes::itps::core::Check_Convergence_CTM(
+        env->arr[0], env->arr[1], env->arr[2], env->arr[3], env->saved[0],
+        env->saved[1], env->saved[2], env->saved[3], to_params(p), env->lat,
+        sm);
+    if (converged) *converged = c ? 1 : 0;
+    if (sig_max) *sig_max = sm;
+  });
+}
+
+int tenes_env_corner_spectrum(tenes_env *env, int kind, int site, double *out,
+                              int *n_out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    TN_REQUIRE(kind >= 0 && kind < 4, "kind must be a corner");
+    TN_REQUIRE(site >= 0 && site < env->lat.N_UNIT, "bad site index");
+    auto lam = tenes::itps::core::normalized_spectrum(env->arr[kind][site]);
+    if (out) std::memcpy(out, lam.data(), lam.size() * sizeof(double));
+    if (n_out) *n_out = (int)lam.size();
+  });
+}
+
+int tenes_env_contract_one_site(tenes_env *env, int site, const double *op,
+                                double *out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(site >= 0 && site < env->lat.N_UNIT, "bad site index");
+    TN_REQUIRE(out != nullptr, "null output");
+    auto &a = env->arr;
+    const int64_t d = a[8][site].shape[4];
+    tn::Tensor o = op ? host_op(env->ctx, op, d) : identity_op(env->ctx, d);
+    *out = tenes::itps::core::Contract_one_site_iTPS_CTM(
+        a[0][site], a[1][site], a[2][site], a[3][site], a[4][site], a[5][site],
+        a[6][site], a[7][site], a[8][site], o);
+  });
+}
+
+int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
+                                      int dy, const double *op12,
+                                      double *out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(std::abs(dx) + std::abs(dy) == 1,
+               "op12 contraction needs a nearest-neighbour bond");
+    TN_REQUIRE(op12 && out, "null pointer");
+    auto &a = env->arr;
+    Cluster c = cluster_of(env->lat, source, dx, dy);
+    if (c.nrow == 2) {
+      const int top = c.idx[0][0], bottom = c.idx[1][0];
+      const int64_t d1 = a[8][top].shape[4], d2 = a[8][bottom].shape[4];
+      tn::Tensor o;
+      if (top == source)
+        o = tn::from_host(*env->ctx->c, tn::Shape{d1, d2, d1, d2}, op12);
+      else
+        o = tn::transpose(
+            tn::from_host(*env->ctx->c, tn::Shape{d2, d1, d2, d1}, op12),
+            tn::Axes{1, 0, 3, 2});
+      *out = tenes::itps::core::Contract_two_sites_vertical_op12_iTPS_CTM(
+          a[0][top], a[1][top], a[2][bottom], a[3][bottom], a[4][top],
+          a[5][top], a[5][bottom], a[6][bottom], a[7][bottom], a[7][top],
+          a[8][top], a[8][bottom], o);
+    } else {
+      const int left = c.idx[0][0], right = c.idx[0][1];
+      const int64_t d1 = a[8][left].shape[4], d2 = a[8][right].shape[4];
+      tn::Tensor o;
+      if (left == source)
+        o = tn::from_host(*env->ctx->c, tn::Shape{d1, d2, d1, d2}, op12);
+      else
+        o = tn::transpose(
+            tn::from_host(*env->ctx->c, tn::Shape{d2, d1, d2, d1}, op12),
+            tn::Axes{1, 0, 3, 2});
+      *out = tenes::itps::core::Contract_two_sites_horizontal_op12_iTPS_CTM(
+          a[0][left], a[1][right], a[2][right], a[3][left], a[4][left],
+          a[4][right], a[5][right], a[6][right], a[6][left], a[7][left],
+          a[8][left], a[8][right], o);
+    }
+  });
+}
+
+int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
+                               const double *op_source,
+                               const double *op_target, double *out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(out != nullptr, "null output");
+    auto &a = env->arr;
+    Cluster c = cluster_of(env->lat, source, dx, dy);
+    std::vector<const tn::Tensor *> C(4), eTt(c.ncol), eTr(c.nrow),
+        eTb(c.ncol), eTl(c.nrow);
+    std::vector<std::vector<const tn::Tensor *>> Tn(
+        c.nrow, std::vector<const tn::Tensor *>(c.ncol)),
+        op(c.nrow, std::vector<const tn::Tensor *>(c.ncol));
+    std::vector<tn::Tensor> keep;
+    keep.reserve(c.nrow * c.ncol + 2);
+    for (int row = 0; row < c.nrow; ++row) {
+      for (int col = 0; col < c.ncol; ++col) {
+        const int idx = c.idx[row][col];
+        Tn[row][col] = &a[8][idx];
+        keep.push_back(identity_op(env->ctx, a[8][idx].shape[4]));
+        op[row][col] = &keep.back();
+      }
+      eTl[row] = &a[7][c.idx[row][0]];
+      eTr[row] = &a[5][c.idx[row][c.ncol - 1]];
+    }
+    for (int col = 0; col < c.ncol; ++col) {
+      eTt[col] = &a[4][c.idx[0][col]];
+      eTb[col] = &a[6][c.idx[c.nrow - 1][col]];
+    }
+    C[0] = &a[0][c.idx[0][0]];
+    C[1] = &a[1][c.idx[0][c.ncol - 1]];
+    C[2] = &a[2][c.idx[c.nrow - 1][c.ncol - 1]];
+    C[3] = &a[3][c.idx[c.nrow - 1][0]];
+    if (op_source) {
+      const int s = c.idx[c.source_row][c.source_col];
+      keep.push_back(host_op(env->ctx, op_source, a[8][s].shape[4]));
+      op[c.source_row][c.source_col] = &keep.back();
+    }
+    if (op_target) {
+      const int t = c.idx[c.target_row][c.target_col];
+      keep.push_back(host_op(env->ctx, op_target, a[8][t].shape[4]));
+      op[c.target_row][c.target_col] = &keep.back();
+    }
+    *out = tenes::itps::core::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op);
+  });
+}
+
+int tenes_create_environment_two_sites(const tenes_tensor *const *Ts,
+                                       tenes_tensor **out) {
+  if (!Ts || !Ts[0]) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(Ts[0]);
+  return guard(ctx, [&] {
+    TN_REQUIRE(out != nullptr, "null output");
+    *out = wrap(ctx, tenes::itps::core::Create_Environment_two_sites(
+                         T(Ts[0]), T(Ts[1]), T(Ts[2]), T(Ts[3]), T(Ts[4]),
+                         T(Ts[5]), T(Ts[6]), T(Ts[7]), T(Ts[8]), T(Ts[9]),
+                         T(Ts[10]), T(Ts[11])));
+  });
+}
+
+void tenes_tm_parameters_default(tenes_tm_parameters *p) {
+  if (!p) return;
+  tenes::itps::TransferMatrix_Parameters q;
+  p->num_eigvals = q.num_eigvals;
+  p->maxdim_dense_eigensolver = q.maxdim_dense_eigensolver;
+  p->arnoldi_maxdim = q.arnoldi_maxdim;
+  p->arnoldi_restartdim = q.arnoldi_restartdim;
+  p->arnoldi_maxiter = q.arnoldi_maxiter;
+  p->arnoldi_rtol = q.arnoldi_rtol;
+}
+
+int tenes_env_transfer_matrix_eigenvalues(tenes_env *env, int dir,
+                                          int fixed_coord,
+                                          const tenes_tm_parameters *p,
+                                          double *eig, int *n_out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(dir == 0 || dir == 1, "dir must be 0 (x) or 1 (y)");
+    TN_REQUIRE(eig != nullptr, "null output");
+    tenes::itps::TransferMatrix_Parameters q;
+    if (p) {
+      q.num_eigvals = p->num_eigvals;
+      q.maxdim_dense_eigensolver = p->maxdim_dense_eigensolver;
+      q.arnoldi_maxdim = p->arnoldi_maxdim;
+      q.arnoldi_restartdim = p->arnoldi_restartdim;
+      q.arnoldi_maxiter = p->arnoldi_maxiter;
+      q.arnoldi_rtol = p->arnoldi_rtol;
+    }
+    TN_REQUIRE(q.num_eigvals >= 1, "num_eigvals must be positive");
+    auto &a = env->arr;
+    tenes::itps::TransferMatrix_ctm tm(env->lat, a[8], a[0], a[1], a[2], a[3],
+                                       a[7], a[4], a[5], a[6]);
+    auto ev = tm.eigenvalues(dir, fixed_coord, q);
+    for (size_t i = 0; i < ev.size(); ++i) {
+      eig[2 * i] = ev[i].real();
+      eig[2 * i + 1] = ev[i].imag();
+    }
+    if (n_out) *n_out = (int)ev.size();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,302 @@
+// CTM-environment contractions (observables, full-update environment).
+// Each follows the contraction tree of the TeNeS function it replaces; the
+// final scalar is a device dot product read back once.
+#include "contract.hpp"
+
+#include <sstream>
+
+namespace tenes {
+namespace itps {
+namespace core {
+
+using tn::Axes;
+using tn::conj;
+using tn::tensordot;
+using tn::trace;
+
+// contract_itps_ctm/ctm.cpp:89-113
+double Contract_one_site_iTPS_CTM(const tensor &C1, const tensor &C2,
+                                  const tensor &C3, const tensor &C4,
+                                  const tensor &eT1, const tensor &eT2,
+                                  const tensor &eT3, const tensor &eT4,
+                                  const tensor &Tn1, const tensor &op1) {
+  tensor upper =
+      tensordot(tensordot(C2, tensordot(C3, eT2, Axes{0}, Axes{1}), Axes{1},
+                          Axes{1}),
+                tensordot(C1, eT1, Axes{1}, Axes{0}), Axes{0}, Axes{1});
+  tensor lower = tensordot(
+      tensordot(tensordot(C4, eT4, Axes{1}, Axes{0}), eT3, Axes{0}, Axes{1}),
+      conj(Tn1), Axes{2, 5}, Axes{0, 3});
+  tensor X = tensordot(tensordot(upper, lower, Axes{0, 2, 3, 5},
+                                 Axes{2, 5, 0, 4}),
+                       Tn1, Axes{0, 1, 2, 3}, Axes{2, 1, 0, 3});
+  return trace(X, op1, Axes{0, 1}, Axes{1, 0});
+}
+
+// contract_itps_ctm/ctm.cpp:115-163
+double Contract_two_sites_horizontal_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op1, const tensor &op2) {
+  tensor right_half = tensordot(
+      Tn2,
+      tensordot(tensordot(tensordot(C3, eT4, Axes{1}, Axes{0}), eT3, Axes{0},
+                          Axes{1}),
+                tensordot(conj(Tn2), op2, Axes{4}, Axes{1}), Axes{2, 5},
+                Axes{3, 2}),
+      Axes{2, 3, 4}, Axes{3, 1, 6});
+  tensor R = tensordot(
+      tensordot(tensordot(C2, eT2, Axes{0}, Axes{1}), right_half,
+                Axes{0, 2, 3}, Axes{3, 1, 5}),
+      eT5, Axes{2}, Axes{0});
+  tensor L = tensordot(tensordot(C1, eT1, Axes{1}, Axes{0}),
+                       tensordot(C4, eT6, Axes{1}, Axes{0}), Axes{0}, Axes{1});
+  tensor X = tensordot(
+      Tn1,
+      tensordot(L, tensordot(conj(Tn1), R, Axes{2, 3}, Axes{2, 5}),
+                Axes{0, 2, 3, 5}, Axes{3, 1, 5, 0}),
+      Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
+  return trace(X, op1, Axes{0, 1}, Axes{0, 1});
+}
+
+// contract_itps_ctm/ctm.cpp:165-211
+double Contract_two_sites_vertical_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op1, const tensor &op2) {
+  tensor lower_half = tensordot(
+      Tn2,
+      tensordot(tensordot(tensordot(C4, eT5, Axes{1}, Axes{0}), eT4, Axes{0},
+                          Axes{1}),
+                tensordot(conj(Tn2), op2, Axes{4}, Axes{1}), Axes{2, 5},
+                Axes{0, 3}),
+      Axes{0, 3, 4}, Axes{1, 3, 6});
+  tensor B = tensordot(
+      tensordot(tensordot(C3, eT3, Axes{0}, Axes{1}), lower_half,
+                Axes{0, 2, 3}, Axes{3, 1, 5}),
+      eT6, Axes{2}, Axes{0});
+  tensor T = tensordot(tensordot(C2, eT2, Axes{1}, Axes{0}),
+                       tensordot(C1, eT1, Axes{1}, Axes{0}), Axes{0}, Axes{1});
+  tensor X = tensordot(
+      Tn1,
+      tensordot(T, tensordot(conj(Tn1), B, Axes{0, 3}, Axes{5, 2}),
+                Axes{0, 2, 3, 5}, Axes{3, 1, 5, 0}),
+      Axes{0, 1, 2, 3}, Axes{4, 1, 0, 3});
+  return trace(X, op1, Axes{0, 1}, Axes{0, 1});
+}
+
+// contract_itps_ctm/ctm.cpp:213-266
+double Contract_two_sites_horizontal_op12_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op12) {
+  tensor left = tensordot(
+      eT1,
+      tensordot(Tn1,
+                tensordot(conj(Tn1),
+                          tensordot(eT5,
+                                    tensordot(C1,
+                                              tensordot(C4, eT6, Axes{1},
+                                                        Axes{0}),
+                                              Axes{0}, Axes{1}),
+                                    Axes{1}, Axes{1}),
+                          Axes{0, 3}, Axes{5, 2}),
+                Axes{0, 3}, Axes{6, 4}),
+      Axes{0, 2, 3}, Axes{7, 0, 3});
+  tensor right = tensordot(
+      eT2,
+      tensordot(Tn2,
+                tensordot(conj(Tn2),
+                          tensordot(eT4,
+                                    tensordot(C2,
+                                              tensordot(C3, eT3, Axes{0},
+                                                        Axes{1}),
+                                              Axes{1}, Axes{1}),
+                                    Axes{0}, Axes{1}),
+                          Axes{2, 3}, Axes{5, 2}),
+                Axes{2, 3}, Axes{6, 4}),
+      Axes{1, 2, 3}, Axes{7, 1, 4});
+  tensor X = tensordot(left, right, Axes{0, 1, 3, 5}, Axes{0, 1, 3, 5});
+  return trace(op12, X, Axes{0, 1, 2, 3}, Axes{0, 2, 1, 3});
+}
+
+// contract_itps_ctm/ctm.cpp:268-319
+double Contract_two_sites_vertical_op12_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op12) {
+  tensor top = tensordot(
+      eT2,
+      tensordot(Tn1,
+                tensordot(conj(Tn1),
+                          tensordot(eT6,
+                                    tensordot(C1,
+                                              tensordot(C2, eT1, Axes{0},
+                                                        Axes{1}),
+                                              Axes{1}, Axes{1}),
+                                    Axes{1}, Axes{0}),
+                          Axes{0, 1}, Axes{2, 5}),
+                Axes{0, 1}, Axes{4, 6}),
+      Axes{0, 2, 3}, Axes{7, 0, 3});
+  tensor bottom = tensordot(
+      eT3,
+      tensordot(Tn2,
+                tensordot(conj(Tn2),
+                          tensordot(eT5,
+                                    tensordot(C3,
+                                              tensordot(C4, eT4, Axes{0},
+                                                        Axes{1}),
+                                              Axes{1}, Axes{1}),
+                                    Axes{0}, Axes{1}),
+                          Axes{0, 3}, Axes{2, 5}),
+                Axes{0, 3}, Axes{4, 6}),
+      Axes{1, 2, 3}, Axes{7, 1, 4});
+  tensor X = tensordot(top, bottom, Axes{0, 1, 3, 5}, Axes{0, 1, 3, 5});
+  return trace(op12, X, Axes{0, 1, 2, 3}, Axes{0, 2, 1, 3});
+}
+
+namespace {
+
+// contract_itps_ctm/ctm_1x1.cpp:27-72
+double contract_1x1(const std::vector<const tensor *> &C,
+                    const std::vector<const tensor *> &eTt,
+                    const std::vector<const tensor *> &eTr,
+                    const std::vector<const tensor *> &eTb,
+                    const std::vector<const tensor *> &eTl,
+                    const std::vector<std::vector<const tensor *>> &Tn,
+                    const std::vector<std::vector<const tensor *>> &op) {
+  const tensor &T = *Tn[0][0];
+  tensor top = tensordot(*eTt[0], tensordot(*C[0], *eTl[0], Axes{0}, Axes{1}),
+                         Axes{0}, Axes{0});
+  tensor rest = tensordot(
+      tensordot(*C[1], *eTr[0], Axes{1}, Axes{0}),
+      tensordot(*C[2], tensordot(*C[3], *eTb[0], Axes{0}, Axes{1}), Axes{1},
+                Axes{1}),
+      Axes{1}, Axes{0});
+  tensor X = tensordot(
+      T,
+      tensordot(top, tensordot(conj(T), rest, Axes{2, 3}, Axes{2, 5}),
+                Axes{0, 2, 3, 5}, Axes{3, 1, 5, 0}),
+      Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
+  return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
+}
+
+// contract_itps_ctm/ctm_1x2.cpp:27-87
+double contract_1x2(const std::vector<const tensor *> &C,
+                    const std::vector<const tensor *> &eTt,
+                    const std::vector<const tensor *> &eTr,
+                    const std::vector<const tensor *> &eTb,
+                    const std::vector<const tensor *> &eTl,
+                    const std::vector<std::vector<const tensor *>> &Tn,
+                    const std::vector<std::vector<const tensor *>> &op) {
+  const tensor &T0 = *Tn[0][0];
+  const tensor &T1 = *Tn[0][1];
+  tensor right_env =
+      tensordot(*eTb[1],
+                tensordot(*C[1], tensordot(*C[2], *eTr[0], Axes{0}, Axes{1}),
+                          Axes{1}, Axes{1}),
+                Axes{0}, Axes{1});
+  tensor site1 = tensordot(
+      *eTt[1],
+      tensordot(T1,
+                tensordot(tensordot(conj(T1), *op[0][1], Axes{4}, Axes{1}),
+                          right_env, Axes{2, 3}, Axes{5, 2}),
+                Axes{2, 3, 4}, Axes{6, 4, 2}),
+      Axes{1, 2, 3}, Axes{5, 1, 3});
+  tensor lower = tensordot(tensordot(*C[3], *eTb[0], Axes{0}, Axes{1}), site1,
+                           Axes{1}, Axes{3});
+  tensor top = tensordot(*eTt[0], tensordot(*C[0], *eTl[0], Axes{0}, Axes{1}),
+                         Axes{0}, Axes{0});
+  tensor X = tensordot(
+      T0,
+      tensordot(top, tensordot(conj(T0), lower, Axes{2, 3}, Axes{5, 2}),
+                Axes{0, 2, 3, 5}, Axes{5, 1, 3, 0}),
+      Axes{0, 1, 2, 3}, Axes{1, 0, 4, 3});
+  return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
+}
+
+// contract_itps_ctm/ctm_2x1.cpp:27-87
+double contract_2x1(const std::vector<const tensor *> &C,
+                    const std::vector<const tensor *> &eTt,
+                    const std::vector<const tensor *> &eTr,
+                    const std::vector<const tensor *> &eTb,
+                    const std::vector<const tensor *> &eTl,
+                    const std::vector<std::vector<const tensor *>> &Tn,
+                    const std::vector<std::vector<const tensor *>> &op) {
+  const tensor &T0 = *Tn[0][0];
+  const tensor &T1 = *Tn[1][0];
+  tensor bottom_env =
+      tensordot(*eTl[1],
+                tensordot(*C[2], tensordot(*C[3], *eTb[0], Axes{0}, Axes{1}),
+                          Axes{1}, Axes{1}),
+                Axes{0}, Axes{1});
+  tensor site1 = tensordot(
+      *eTr[1],
+      tensordot(T1,
+                tensordot(tensordot(conj(T1), *op[1][0], Axes{4}, Axes{1}),
+                          bottom_env, Axes{0, 3}, Axes{2, 5}),
+                Axes{0, 3, 4}, Axes{4, 6, 2}),
+      Axes{1, 2, 3}, Axes{5, 1, 3});
+  tensor right = tensordot(tensordot(*C[1], *eTr[0], Axes{1}, Axes{0}), site1,
+                           Axes{1}, Axes{0});
+  tensor top = tensordot(*eTt[0], tensordot(*C[0], *eTl[0], Axes{0}, Axes{1}),
+                         Axes{0}, Axes{0});
+  tensor X = tensordot(
+      T0,
+      tensordot(top, tensordot(conj(T0), right, Axes{2, 3}, Axes{2, 4}),
+                Axes{0, 2, 3, 5}, Axes{3, 1, 6, 0}),
+      Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
+  return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
+}
+
+}  // namespace
+
+double Contract_iTPS_CTM(const std::vector<const tensor *> &C,
+                         const std::vector<const tensor *> &eTt,
+                         const std::vector<const tensor *> &eTr,
+                         const std::vector<const tensor *> &eTb,
+                         const std::vector<const tensor *> &eTl,
+                         const std::vector<std::vector<const tensor *>> &Tn,
+                         const std::vector<std::vector<const tensor *>> &op) {
+  const size_t nrow = Tn.size();
+  const size_t ncol = nrow ? Tn[0].size() : 0;
+  if (nrow == 1 && ncol == 1) return contract_1x1(C, eTt, eTr, eTb, eTl, Tn, op);
+  if (nrow == 1 && ncol == 2) return contract_1x2(C, eTt, eTr, eTb, eTl, Tn, op);
+  if (nrow == 2 && ncol == 1) return contract_2x1(C, eTt, eTr, eTb, eTl, Tn, op);
+  std::ostringstream ss;
+  ss << "Contract_" << nrow << "_" << ncol << "_iTPS_CTM is not implemented";
+  throw tn::Error(ss.str(), -5);
+}
+
+// core/full_update.cpp:51-83
+tensor Create_Environment_two_sites(const tensor &C1, const tensor &C2,
+                                    const tensor &C3, const tensor &C4,
+                                    const tensor &eT1, const tensor &eT2,
+                                    const tensor &eT3, const tensor &eT4,
+                                    const tensor &eT5, const tensor &eT6,
+                                    const tensor &Q1, const tensor &Q2) {
+  tensor right_q = tensordot(
+      tensordot(tensordot(tensordot(C3, eT4, Axes{1}, Axes{0}), eT3, Axes{0},
+                          Axes{1}),
+                conj(Q2), Axes{2, 5}, Axes{3, 2}),
+      Q2, Axes{1, 3}, Axes{3, 2});
+  tensor right = tensordot(tensordot(C2, eT2, Axes{0}, Axes{1}), right_q,
+                           Axes{0, 2, 3}, Axes{1, 5, 3});
+  tensor left_q = tensordot(
+      tensordot(tensordot(tensordot(C4, eT6, Axes{1}, Axes{0}), eT5, Axes{0},
+                          Axes{1}),
+                conj(Q1), Axes{2, 5}, Axes{0, 2}),
+      Q1, Axes{1, 3}, Axes{0, 2});
+  tensor left = tensordot(tensordot(C1, eT1, Axes{1}, Axes{0}), left_q,
+                          Axes{0, 2, 3}, Axes{0, 4, 2});
+  return tn::tensordot_t(right, left, Axes{0, 1}, Axes{0, 1},
+                         Axes{3, 1, 2, 0});
+}
+
+}  // namespace core
+}  // namespace itps
+}  // namespace tenes

@@ -1,0 +1,65 @@
+// CTM-environment contractions for observables and the full-update bond
+// environment, with the call surface of TeNeS
+// (src/iTPS/core/contract_itps_ctm.hpp:45-160, core/full_update.hpp:33-79).
+#pragma once
+
+#include <vector>
+
+#include "tensor.hpp"
+
+namespace tenes {
+namespace itps {
+namespace core {
+
+using tensor = tn::Tensor;
+
+double Contract_one_site_iTPS_CTM(const tensor &C1, const tensor &C2,
+                                  const tensor &C3, const tensor &C4,
+                                  const tensor &eT1, const tensor &eT2,
+                                  const tensor &eT3, const tensor &eT4,
+                                  const tensor &Tn1, const tensor &op1);
+
+double Contract_two_sites_horizontal_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op1, const tensor &op2);
+
+double Contract_two_sites_vertical_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op1, const tensor &op2);
+
+double Contract_two_sites_horizontal_op12_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op12);
+
+double Contract_two_sites_vertical_op12_iTPS_CTM(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
+    const tensor &op12);
+
+// Contract_iTPS_CTM dispatcher (contract_itps_ctm/ctm.cpp:45-87) for the
+// 1x1, 1x2 and 2x1 clusters; other cluster shapes raise an error.
+double Contract_iTPS_CTM(const std::vector<const tensor *> &C,
+                         const std::vector<const tensor *> &eTt,
+                         const std::vector<const tensor *> &eTr,
+                         const std::vector<const tensor *> &eTb,
+                         const std::vector<const tensor *> &eTl,
+                         const std::vector<std::vector<const tensor *>> &Tn,
+                         const std::vector<std::vector<const tensor *>> &op);
+
+tensor Create_Environment_two_sites(const tensor &C1, const tensor &C2,
+                                    const tensor &C3, const tensor &C4,
+                                    const tensor &eT1, const tensor &eT2,
+                                    const tensor &eT3, const tensor &eT4,
+                                    const tensor &eT5, const tensor &eT6,
+                                    const tensor &Q1, const tensor &Q2);
+
+}  // namespace core
+}  // namespace itps
+}  // namespace tenes

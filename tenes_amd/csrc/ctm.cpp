@@ -1,0 +1,705 @@
+// CTMRG on the GPU: projectors, renormalisation, directional moves,
+// convergence check and the environment driver.  Each function follows the
+// contraction order of the TeNeS function it replaces (cited per function);
+// every tensordot runs as one gather-GEMM launch on the context stream.
+#include "ctm.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+
+#include "kernels.hpp"
+
+namespace tenes {
+namespace itps {
+
+using tn::Axes;
+using tn::Shape;
+
+// ---------------------------------------------------------------------------
+// SquareLattice (src/SquareLattice.cpp:48-137)
+// ---------------------------------------------------------------------------
+static int gcd_i(int a, int b) {
+  while (b) {
+    int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+SquareLattice::SquareLattice(int X, int Y, int skew_in) {
+  TN_REQUIRE(X > 0 && Y > 0, "SquareLattice: LX and LY must be positive");
+  LX = X;
+  LY = Y;
+  N_UNIT = LX * LY;
+  skew = ((skew_in % LX) + LX) % LX;
+  LX_noskew = LX;
+  LY_noskew = skew == 0 ? LY : LY * ((LX / gcd_i(LX, skew) * skew) / skew);
+  N_UNIT_noskew = LX_noskew * LY_noskew;
+  nn_.resize(N_UNIT);
+  for (int iy = 0; iy < LY; ++iy)
+    for (int ix = 0; ix < LX; ++ix) {
+      const int i = ix + iy * LX;
+      nn_[i][0] = ((ix - 1 + LX) % LX) + iy * LX;
+      nn_[i][1] = ix + ((iy + 1) % LY) * LX;
+      nn_[i][2] = ((ix + 1) % LX) + iy * LX;
+      nn_[i][3] = ix + ((iy - 1 + LY) % LY) * LX;
+    }
+  if (skew != 0) {
+    for (int ix = 0; ix < LX; ++ix) {
+      nn_[ix][3] = ((ix + LX + skew) % LX) + (LY - 1) * LX;
+      nn_[N_UNIT + ix - LX][1] = ((ix + LX - skew) % LX);
+    }
+  }
+  parity_.assign(N_UNIT, 0);
+  for (int ix = 0; ix < LX; ++ix) parity_[ix] = (ix % 2 != 0) ? -1 : 1;
+  for (int iy = 1; iy < LY; ++iy)
+    for (int ix = 0; ix < LX; ++ix) {
+      const int site = index(ix, iy);
+      parity_[site] = -parity_[bottom(site)];
+    }
+}
+
+int SquareLattice::index(int x, int y) const {
+  int y_offset = 0;
+  if (y >= LY)
+    y_offset = y / LY;
+  else if (y < 0)
+    y_offset = (y + 1) / LY - 1;
+  y -= LY * y_offset;
+  x -= skew * y_offset;
+  x %= LX;
+  if (x < 0) x += LX;
+  return x + y * LX;
+}
+
+int SquareLattice::other(int idx, int dx, int dy) const {
+  while (dx != 0) {
+    if (dx > 0) {
+      idx = right(idx);
+      --dx;
+    } else {
+      idx = left(idx);
+      ++dx;
+    }
+  }
+  while (dy != 0) {
+    if (dy > 0) {
+      idx = top(idx);
+      --dy;
+    } else {
+      idx = bottom(idx);
+      ++dy;
+    }
+  }
+  return idx;
+}
+
+namespace core {
+
+namespace {
+
+tn::SvdOptions svd_opts(const PEPS_Parameters &p, bool rsvd) {
+  tn::SvdOptions o;
+  o.power_iters = rsvd ? p.rsvd_power_iters : p.svd_power_iters;
+  return o;
+}
+
+// Truncation of ctm.cpp:275-288: keep s[i]/s[0] > cut, at most `cut` values;
+// returns the kept count and fills s_c = 1/sqrt(s).
+int truncate_spectrum(const std::vector<double> &s, int cut, double eps,
+                      std::vector<double> &s_c) {
+  s_c.clear();
+  if (s.empty()) return 0;
+  const double denom = s[0];
+  for (int i = 0; i < cut; ++i) {
+    if (s[i] / denom > eps) {
+      s_c.push_back(1.0 / std::sqrt(s[i]));
+    } else {
+      cut = i;
+      break;
+    }
+  }
+  return cut;
+}
+
+tensor identity_projector(tn::Context &ctx, int64_t e78, int64_t t41) {
+  std::vector<double> id((size_t)(e78 * e78), 0.0);
+  for (int64_t i = 0; i < e78; ++i) id[(size_t)(i * e78 + i)] = 1.0;
+  tensor I = tn::from_host(ctx, Shape{e78, e78}, id.data());
+  return tn::reshape(I, Shape{e78, t41, t41, e78});
+}
+
+// Enlarged corner of ctm.cpp:207-212 (and its rotated uses):
+// LT = Tn1*(((C1*eT1)*eT8)*Tn1c), legs (t12, t41, e12, e78, tc12, tc41).
+tensor enlarged_LT(const tensor &C1, const tensor &eT1, const tensor &eT8,
+                   const tensor &Tn1) {
+  return tn::tensordot(
+      Tn1,
+      tn::tensordot(tn::tensordot(tn::tensordot(C1, eT1, Axes{1}, Axes{0}),
+                                  eT8, Axes{0}, Axes{1}),
+                    tn::conj(Tn1), Axes{2, 5}, Axes{1, 0}),
+      Axes{0, 1, 4}, Axes{3, 1, 6});
+}
+// ctm.cpp:222-227: LB = Tn4*(((C4*eT7)*eT6)*Tn4c),
+// legs (t41, t34, e78, e56, tc41, tc34).
+tensor enlarged_LB(const tensor &C4, const tensor &eT7, const tensor &eT6,
+                   const tensor &Tn4) {
+  return tn::tensordot(
+      Tn4,
+      tn::tensordot(tn::tensordot(tn::tensordot(C4, eT7, Axes{1}, Axes{0}),
+                                  eT6, Axes{0}, Axes{1}),
+                    tn::conj(Tn4), Axes{2, 5}, Axes{0, 3}),
+      Axes{0, 3, 4}, Axes{1, 3, 6});
+}
+
+// From the SVD of the (rows x cols) matrix `Rm`, build the oblique projector
+// pair: PU = Bblk * (VT_c^dagger s^-1/2), PL = Ablk * (U_c^* s^-1/2), both
+// transposed to (e_in, t, tc, cut) (ctm.cpp:290-301 / 466-475).
+void projectors_from_matrix(tn::Context &ctx, const tensor &Rm, int64_t r0,
+                            int64_t r1, int64_t c0, int64_t c1, int chi_cap,
+                            const tensor &Ablk, const Axes &axA,
+                            const tensor &Bblk, const Axes &axB,
+                            const PEPS_Parameters &pp, bool rsvd, tensor &PU,
+                            tensor &PL) {
+  const int64_t nr = r0 * r1 * r0, nc = c0 * c1 * c0;
+  const int kmax = (int)std::min<int64_t>(std::min<int64_t>(nr, nc), chi_cap);
+  tn::SvdResult sv =
+      tn::svd_top(ctx, tn::reshape(Rm, Shape{nr, nc}), kmax, svd_opts(pp, rsvd));
+  std::vector<double> s_c;
+  const int cut =
+      truncate_spectrum(sv.s, std::min<int>(kmax, (int)sv.s.size()),
+                        pp.Inverse_projector_cut, s_c);
+  TN_REQUIRE(cut >= 1, "projector: no singular value above the cutoff");
+  tensor sc = tn::from_host(ctx, Shape{cut}, s_c.data());
+  tensor Uc = tn::slice(sv.U, 1, 0, cut);
+  tensor Vc = tn::slice(sv.V, 1, 0, cut);
+  tn::multiply_vector_dev(Uc, sc.data(), 1);
+  tn::multiply_vector_dev(Vc, sc.data(), 1);
+  Uc = tn::reshape(Uc, Shape{r0, r1, r0, cut});
+  Vc = tn::reshape(Vc, Shape{c0, c1, c0, cut});
+  PU = tn::tensordot_t(Bblk, Vc, axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+  PL = tn::tensordot_t(Ablk, Uc, axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+}
+
+}  // namespace
+
+// ctm.cpp:174-318
+void Calc_projector_left_block(const tensor &C1, const tensor &C4,
+                               const tensor &eT1, const tensor &eT6,
+                               const tensor &eT7, const tensor &eT8,
+                               const tensor &Tn1, const tensor &Tn4,
+                               const PEPS_Parameters &pp, tensor &PU,
+                               tensor &PL) {
+  tn::Context &ctx = *C1.ctx();
+  const int64_t e12 = eT1.shape[1];
+  const int64_t e56 = eT6.shape[0];
+  const int64_t e78 = eT8.shape[0];
+  const int64_t t12 = Tn1.shape[2];
+  const int64_t t41 = Tn1.shape[3];
+  const int64_t t34 = Tn4.shape[2];
+  if (t41 == 1) {
+    PU = identity_projector(ctx, e78, t41);
+    PL = PU;
+    return;
+  }
+  tensor LT = enlarged_LT(C1, eT1, eT8, Tn1);
+  tensor LB = enlarged_LB(C4, eT7, eT6, Tn4);
+  const bool rsvd = (t12 != 1 && t34 != 1) && pp.Use_RSVD;
+  int cap = pp.CHI;
+  if (rsvd)
+    cap = (int)std::min<int64_t>(
+        std::min<int64_t>(std::min<int64_t>(pp.CHI, e78 * t41 * t41),
+                          e12 * t12 * t12),
+        e56 * t34 * t34);
+  // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273)
+  tensor R = tn::tensordot(LT, LB, Axes{1, 3, 5}, Axes{0, 2, 4});
+  projectors_from_matrix(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 2, 4},
+                         LB, Axes{1, 3, 5}, pp, rsvd, PU, PL);
+}
+
+// ctm.cpp:320-491
+void Calc_projector_updown_blocks(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &eT7, const tensor &eT8,
+    const tensor &Tn1, const tensor &Tn2, const tensor &Tn3, const tensor &Tn4,
+    const PEPS_Parameters &pp, tensor &PU, tensor &PL) {
+  tn::Context &ctx = *C1.ctx();
+  const int64_t e34 = eT3.shape[1];
+  const int64_t e78 = eT7.shape[1];
+  const int64_t t23 = Tn2.shape[3];
+  const int64_t t41 = Tn4.shape[1];
+  if (t41 == 1) {
+    PU = identity_projector(ctx, e78, t41);
+    PL = PU;
+    return;
+  }
+  tensor LT = enlarged_LT(C1, eT1, eT8, Tn1);
+  // RT legs (t12, t23, e34, e12, tc12, tc23)  (ctm.cpp:364-369)
+  tensor RT = tn::tensordot(
+      Tn2,
+      tn::tensordot(tn::tensordot(tn::tensordot(C2, eT3, Axes{1}, Axes{0}),
+                                  eT2, Axes{0}, Axes{1}),
+                    tn::conj(Tn2), Axes{2, 5}, Axes{2, 1}),
+      Axes{1, 2, 4}, Axes{3, 1, 6});
+  // RB legs (t34, t23, e56, e34, tc34, tc23)  (ctm.cpp:378-383)
+  tensor RB = tn::tensordot(
+      Tn3,
+      tn::tensordot(tn::tensordot(tn::tensordot(C3, eT5, Axes{1}, Axes{0}),
+                                  eT4, Axes{0}, Axes{1}),
+                    tn::conj(Tn3), Axes{2, 5}, Axes{3, 2}),
+      Axes{2, 3, 4}, Axes{3, 1, 6});
+  tensor LB = enlarged_LB(C4, eT7, eT6, Tn4);
+  const bool rsvd = t23 != 1 && pp.Use_RSVD;
+  int cap = pp.CHI;
+  if (rsvd) cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * t23);
+  // R1 = RT*LT, R2 = RB*LB, legs (t23, e34, tc23, t41, e78, tc41)
+  tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 2, 4});
+  tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{1, 3, 5});
+  tensor M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
+  projectors_from_matrix(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2},
+                         R2, Axes{0, 1, 2}, pp, rsvd, PU, PL);
+}
+
+// ctm.cpp:493-518
+void Calc_Next_CTM(const tensor &C1, const tensor &C4, const tensor &eT1,
+                   const tensor &eT6, const tensor &PU, const tensor &PL,
+                   tensor &C1_out, tensor &C4_out) {
+  C1_out = tn::tensordot(PU, tn::tensordot(C1, eT1, Axes{1}, Axes{0}),
+                         Axes{0, 1, 2}, Axes{0, 2, 3});
+  C4_out = tn::tensordot(tn::tensordot(eT6, C4, Axes{1}, Axes{0}), PL,
+                         Axes{3, 1, 2}, Axes{0, 1, 2});
+  tn::normalize_max_abs(C1_out);
+  tn::normalize_max_abs(C4_out);
+}
+
+// ctm.cpp:520-552: ((Tn1*(Tn1c*(eT8*PL)))*PU), then transpose (3,2,0,1)
+// fused into the last contraction's store.
+void Calc_Next_eT(const tensor &eT8, const tensor &Tn1, const tensor &PU,
+                  const tensor &PL, tensor &eT_out) {
+  tensor X = tn::tensordot(eT8, PL, Axes{1}, Axes{0});
+  tensor Y = tn::tensordot(tn::conj(Tn1), X, Axes{0, 1}, Axes{2, 4});
+  tensor Z = tn::tensordot(Tn1, Y, Axes{0, 1, 4}, Axes{4, 5, 2});
+  eT_out = tn::tensordot_t(Z, PU, Axes{1, 3, 4}, Axes{1, 2, 0},
+                           Axes{3, 2, 0, 1});
+  tn::normalize_max_abs(eT_out);
+}
+
+// ctm.cpp:562-616
+void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
+               const std::vector<tensor> &C3, std::vector<tensor> &C4,
+               const std::vector<tensor> &eTt, const std::vector<tensor> &eTr,
+               const std::vector<tensor> &eTb, std::vector<tensor> &eTl,
+               const std::vector<tensor> &Tn, int ix,
+               const PEPS_Parameters &pp, const SquareLattice &lat) {
+  const int L = lat.LY_noskew;
+  std::vector<tensor> PUs(L), PLs(L);
+  for (int iy = 0; iy < L; ++iy) {
+    const int i = lat.index(ix, iy);
+    const int j = lat.right(i);
+    const int k = lat.bottom(j);
+    const int l = lat.left(k);
+    if (pp.CTM_Projector_corner)
+      Calc_projector_left_block(C1[i], C4[l], eTt[i], eTb[l], eTl[l], eTl[i],
+                                Tn[i], Tn[l], pp, PUs[iy], PLs[iy]);
+    else
+      Calc_projector_updown_blocks(C1[i], C2[j], C3[k], C4[l], eTt[i], eTt[j],
+                                   eTr[j], eTr[k], eTb[k], eTb[l], eTl[l],
+                                   eTl[i], Tn[i], Tn[j], Tn[k], Tn[l], pp,
+                                   PUs[iy], PLs[iy]);
+  }
+  const std::vector<tensor> C1_bak = C1, C4_bak = C4, eTl_bak = eTl;
+  for (int iy = 0; iy < L; ++iy) {
+    const int i = lat.index(ix, iy);
+    const int j = lat.right(i);
+    const int k = lat.bottom(j);
+    const int l = lat.left(k);
+    const int iy_up = (iy + 1) % L;
+    const int iy_down = (iy - 1 + L) % L;
+    Calc_Next_CTM(C1_bak[i], C4_bak[l], eTt[i], eTb[l], PUs[iy_up],
+                  PLs[iy_down], C1[j], C4[k]);
+    Calc_Next_eT(eTl_bak[i], Tn[i], PUs[iy], PLs[iy_up], eTl[j]);
+    Calc_Next_eT(eTl_bak[l], Tn[l], PUs[iy_down], PLs[iy], eTl[k]);
+  }
+}
+
+// ctm.cpp:618-681
+void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
+                std::vector<tensor> &C3, const std::vector<tensor> &C4,
+                const std::vector<tensor> &eTt, std::vector<tensor> &eTr,
+                const std::vector<tensor> &eTb, const std::vector<tensor> &eTl,
+                const std::vector<tensor> &Tn, int ix,
+                const PEPS_Parameters &pp, const SquareLattice &lat) {
+  const int L = lat.LY_noskew;
+  const Axes rot{2, 3, 0, 1, 4};
+  std::vector<tensor> PUs(L), PLs(L);
+  for (int iy = 0; iy < L; ++iy) {
+    const int k = lat.index(ix, iy);
+    const int l = lat.left(k);
+    const int i = lat.top(l);
+    const int j = lat.right(i);
+    if (pp.CTM_Projector_corner)
+      Calc_projector_left_block(C3[k], C2[j], eTb[k], eTt[j], eTr[j], eTr[k],
+                                tn::transpose(Tn[k], rot),
+                                tn::transpose(Tn[j], rot), pp, PUs[iy],
+                                PLs[iy]);
+    else
+      Calc_projector_updown_blocks(
+          C3[k], C4[l], C1[i], C2[j], eTb[k], eTb[l], eTl[l], eTl[i], eTt[i],
+          eTt[j], eTr[j], eTr[k], tn::transpose(Tn[k], rot),
+          tn::transpose(Tn[l], rot), tn::transpose(Tn[i], rot),
+          tn::transpose(Tn[j], rot), pp, PUs[iy], PLs[iy]);
+  }
+  const std::vector<tensor> C2_bak = C2, C3_bak = C3, eTr_bak = eTr;
+  for (int iy = 0; iy < L; ++iy) {
+    const int k = lat.index(ix, iy);
+    const int l = lat.left(k);
+    const int i = lat.top(l);
+    const int j = lat.right(i);
+    const int iy_up = (iy + 1) % L;
+    const int iy_down = (iy - 1 + L) % L;
+    Calc_Next_CTM(C3_bak[k], C2_bak[j], eTb[k], eTt[j], PUs[iy_down],
+                  PLs[iy_up], C3[l], C2[i]);
+    Calc_Next_eT(eTr_bak[k], tn::transpose(Tn[k], rot), PUs[iy],
+                 PLs[iy_down], eTr[l]);
+    Calc_Next_eT(eTr_bak[j], tn::transpose(Tn[j], rot), PUs[iy_up], PLs[iy],
+                 eTr[i]);
+  }
+}
+
+// ctm.cpp:683-746
+void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
+              const std::vector<tensor> &C3, const std::vector<tensor> &C4,
+              std::vector<tensor> &eTt, const std::vector<tensor> &eTr,
+              const std::vector<tensor> &eTb, const std::vector<tensor> &eTl,
+              const std::vector<tensor> &Tn, int iy, const PEPS_Parameters &pp,
+              const SquareLattice &lat) {
+  const int L = lat.LX_noskew;
+  const Axes rot{1, 2, 3, 0, 4};
+  std::vector<tensor> PUs(L), PLs(L);
+  for (int ix = 0; ix < L; ++ix) {
+    const int j = lat.index(ix, iy);
+    const int k = lat.bottom(j);
+    const int l = lat.left(k);
+    const int i = lat.top(l);
+    if (pp.CTM_Projector_corner)
+      Calc_projector_left_block(C2[j], C1[i], eTr[j], eTl[i], eTt[i], eTt[j],
+                                tn::transpose(Tn[j], rot),
+                                tn::transpose(Tn[i], rot), pp, PUs[ix],
+                                PLs[ix]);
+    else
+      Calc_projector_updown_blocks(
+          C2[j], C3[k], C4[l], C1[i], eTr[j], eTr[k], eTb[k], eTb[l], eTl[l],
+          eTl[i], eTt[i], eTt[j], tn::transpose(Tn[j], rot),
+          tn::transpose(Tn[k], rot), tn::transpose(Tn[l], rot),
+          tn::transpose(Tn[i], rot), pp, PUs[ix], PLs[ix]);
+  }
+  const std::vector<tensor> C1_bak = C1, C2_bak = C2, eTt_bak = eTt;
+  for (int ix = 0; ix < L; ++ix) {
+    const int j = lat.index(ix, iy);
+    const int k = lat.bottom(j);
+    const int l = lat.left(k);
+    const int i = lat.top(l);
+    const int ix_right = (ix + 1) % L;
+    const int ix_left = (ix - 1 + L) % L;
+    Calc_Next_CTM(C2_bak[j], C1_bak[i], eTr[j], eTl[i], PUs[ix_right],
+                  PLs[ix_left], C2[k], C1[l]);
+    Calc_Next_eT(eTt_bak[j], tn::transpose(Tn[j], rot), PUs[ix],
+                 PLs[ix_right], eTt[k]);
+    Calc_Next_eT(eTt_bak[i], tn::transpose(Tn[i], rot), PUs[ix_left], PLs[ix],
+                 eTt[l]);
+  }
+}
+
+// ctm.cpp:748-813
+void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
+                 std::vector<tensor> &C3, std::vector<tensor> &C4,
+                 const std::vector<tensor> &eTt,
+                 const std::vector<tensor> &eTr, std::vector<tensor> &eTb,
+                 const std::vector<tensor> &eTl, const std::vector<tensor> &Tn,
+                 int iy, const PEPS_Parameters &pp, const SquareLattice &lat) {
+  const int L = lat.LX_noskew;
+  const Axes rot{3, 0, 1, 2, 4};
+  std::vector<tensor> PUs(L), PLs(L);
+  for (int ix = 0; ix < L; ++ix) {
+    const int l = lat.index(ix, iy);
+    const int i = lat.top(l);
+    const int j = lat.right(i);
+    const int k = lat.bottom(j);
+    if (pp.CTM_Projector_corner)
+      Calc_projector_left_block(C4[l], C3[k], eTl[l], eTr[k], eTb[k], eTb[l],
+                                tn::transpose(Tn[l], rot),
+                                tn::transpose(Tn[k], rot), pp, PUs[ix],
+                                PLs[ix]);
+    else
+      Calc_projector_updown_blocks(
+          C4[l], C1[i], C2[j], C3[k], eTl[l], eTl[i], eTt[i], eTt[j], eTr[j],
+          eTr[k], eTb[k], eTb[l], tn::transpose(Tn[l], rot),
+          tn::transpose(Tn[i], rot), tn::transpose(Tn[j], rot),
+          tn::transpose(Tn[k], rot), pp, PUs[ix], PLs[ix]);
+  }
+  const std::vector<tensor> C3_bak = C3, C4_bak = C4, eTb_bak = eTb;
+  for (int ix = 0; ix < L; ++ix) {
+    const int l = lat.index(ix, iy);
+    const int i = lat.top(l);
+    const int j = lat.right(i);
+    const int k = lat.bottom(j);
+    const int ix_right = (ix + 1) % L;
+    const int ix_left = (ix - 1 + L) % L;
+    Calc_Next_CTM(C4_bak[l], C3_bak[k], eTl[l], eTr[k], PUs[ix_left],
+                  PLs[ix_right], C4[i], C3[j]);
+    Calc_Next_eT(eTb_bak[l], tn::transpose(Tn[l], rot), PUs[ix], PLs[ix_left],
+                 eTb[i]);
+    Calc_Next_eT(eTb_bak[k], tn::transpose(Tn[k], rot), PUs[ix_right],
+                 PLs[ix], eTb[j]);
+  }
+}
+
+namespace {
+
+// L2-normalised descending spectra of a list of matrices, batched by shape.
+std::vector<std::vector<double>> spectra(const std::vector<const tensor *> &Cs) {
+  std::vector<std::vector<double>> out(Cs.size());
+  if (Cs.empty()) return out;
+  tn::Context &ctx = *Cs[0]->ctx();
+  std::map<Shape, std::vector<int>> groups;
+  for (size_t i = 0; i < Cs.size(); ++i) groups[Cs[i]->shape].push_back((int)i);
+  for (auto &g : groups) {
+    std::vector<tensor> mats;
+    for (int i : g.second) mats.push_back(*Cs[i]);
+    bool big = g.first[0] > tn::kern::jacobi_max_dim() ||
+               g.first[1] > tn::kern::jacobi_max_dim();
+    TN_REQUIRE(!big, "convergence check: corner matrices above 128 x 128 are "
+                     "not supported");
+    tensor sv = tn::svd_values_batched(ctx, mats);
+    const int64_t k = sv.shape[1];
+    std::vector<double> h((size_t)sv.size());
+    tn::to_host(sv, h.data());
+    for (size_t q = 0; q < g.second.size(); ++q) {
+      std::vector<double> lam(h.begin() + q * k, h.begin() + (q + 1) * k);
+      double norm = 0.0;
+      for (double v : lam) norm += v * v;
+      norm = std::sqrt(norm);
+      for (double &v : lam) v /= norm;
+      out[g.second[q]] = lam;
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+std::vector<double> normalized_spectrum(const tensor &C) {
+  return spectra({&C})[0];
+}
+
+// ctm.cpp:815-969
+bool Check_Convergence_CTM(const std::vector<tensor> &C1,
+                           const std::vector<tensor> &C2,
+                           const std::vector<tensor> &C3,
+                           const std::vector<tensor> &C4,
+                           const std::vector<tensor> &C1_old,
+                           const std::vector<tensor> &C2_old,
+                           const std::vector<tensor> &C3_old,
+                           const std::vector<tensor> &C4_old,
+                           const PEPS_Parameters &pp,
+                           const SquareLattice &lat, double &sig_max) {
+  const int N = lat.N_UNIT;
+  std::vector<const tensor *> all;
+  for (int i = 0; i < N; ++i) {
+    all.push_back(&C1[i]);
+    all.push_back(&C1_old[i]);
+    all.push_back(&C2[i]);
+    all.push_back(&C2_old[i]);
+    all.push_back(&C3[i]);
+    all.push_back(&C3_old[i]);
+    all.push_back(&C4[i]);
+    all.push_back(&C4_old[i]);
+  }
+  auto lam = spectra(all);
+  sig_max = 0.0;
+  for (int i = 0; i < N; ++i) {
+    for (int c = 0; c < 4; ++c) {
+      const auto &ln = lam[8 * i + 2 * c];
+      const auto &lo = lam[8 * i + 2 * c + 1];
+      double sig = 0.0;
+      for (size_t j = 0; j < ln.size(); ++j) {
+        // the reference reads lam_old[j] for j < lam_new.size(); a missing
+        // entry (shorter old spectrum) counts as zero here.
+        const double o = j < lo.size() ? lo[j] : 0.0;
+        sig += (ln[j] - o) * (ln[j] - o);
+      }
+      sig = std::sqrt(sig);
+      if (sig > pp.CTM_Convergence_Epsilon) {
+        sig_max = sig;
+        return false;
+      } else if (sig > sig_max) {
+        sig_max = sig;
+      }
+    }
+  }
+  return true;
+}
+
+namespace {
+// reshape(...) then extend/slice to (CHI, CHI[, d, d]) (ctm.cpp:998-1012)
+tensor fit_chi(const tensor &T, int64_t chi) {
+  tensor r = T;
+  const int64_t d1 = T.shape[0], d2 = T.shape[1];
+  if (d1 > chi) r = tn::slice(r, 0, 0, chi);
+  if (d2 > chi) r = tn::slice(r, 1, 0, chi);
+  if (r.shape[0] < chi || r.shape[1] < chi) {
+    Shape s = r.shape;
+    s[0] = chi;
+    s[1] = chi;
+    r = tn::extend(r, s);
+  }
+  return r;
+}
+}  // namespace
+
+// ctm.cpp:984-1193
+void Initialize_CTM(std::vector<tensor> &C1, std::vector<tensor> &C2,
+                    std::vector<tensor> &C3, std::vector<tensor> &C4,
+                    std::vector<tensor> &eTt, std::vector<tensor> &eTr,
+                    std::vector<tensor> &eTb, std::vector<tensor> &eTl,
+                    const std::vector<tensor> &Tn, const PEPS_Parameters &pp,
+                    const SquareLattice &lat) {
+  const int N = lat.N_UNIT;
+  const int64_t chi = pp.CHI;
+  C1.resize(N);
+  C2.resize(N);
+  C3.resize(N);
+  C4.resize(N);
+  eTt.resize(N);
+  eTr.resize(N);
+  eTb.resize(N);
+  eTl.resize(N);
+  for (int i = 0; i < N; ++i) {
+    int num = lat.top(lat.left(i));
+    {
+      const auto &T = Tn[num];
+      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{0, 1, 4}, Axes{0, 1, 4},
+                                 Axes{1, 3, 0, 2});
+      C1[i] = fit_chi(tn::reshape(c, Shape{T.shape[3] * T.shape[3],
+                                           T.shape[2] * T.shape[2]}),
+                      chi);
+    }
+    num = lat.top(lat.right(i));
+    {
+      const auto &T = Tn[num];
+      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{1, 2, 4}, Axes{1, 2, 4},
+                                 Axes{0, 2, 1, 3});
+      C2[i] = fit_chi(tn::reshape(c, Shape{T.shape[0] * T.shape[0],
+                                           T.shape[3] * T.shape[3]}),
+                      chi);
+    }
+    num = lat.bottom(lat.right(i));
+    {
+      const auto &T = Tn[num];
+      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{2, 3, 4}, Axes{2, 3, 4},
+                                 Axes{1, 3, 0, 2});
+      C3[i] = fit_chi(tn::reshape(c, Shape{T.shape[1] * T.shape[1],
+                                           T.shape[0] * T.shape[0]}),
+                      chi);
+    }
+    num = lat.bottom(lat.left(i));
+    {
+      const auto &T = Tn[num];
+      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{0, 3, 4}, Axes{0, 3, 4},
+                                 Axes{1, 3, 0, 2});
+      C4[i] = fit_chi(tn::reshape(c, Shape{T.shape[2] * T.shape[2],
+                                           T.shape[1] * T.shape[1]}),
+                      chi);
+    }
+    num = lat.top(i);
+    {
+      const auto &T = Tn[num];
+      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{1, 4}, Axes{1, 4},
+                                 Axes{0, 3, 1, 4, 2, 5});
+      eTt[i] = fit_chi(tn::reshape(e, Shape{T.shape[0] * T.shape[0],
+                                            T.shape[2] * T.shape[2],
+                                            T.shape[3], T.shape[3]}),
+                       chi);
+    }
+    num = lat.right(i);
+    {
+      const auto &T = Tn[num];
+      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{2, 4}, Axes{2, 4},
+                                 Axes{1, 4, 2, 5, 0, 3});
+      eTr[i] = fit_chi(tn::reshape(e, Shape{T.shape[1] * T.shape[1],
+                                            T.shape[3] * T.shape[3],
+                                            T.shape[0], T.shape[0]}),
+                       chi);
+    }
+    num = lat.bottom(i);
+    {
+      const auto &T = Tn[num];
+      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{3, 4}, Axes{3, 4},
+                                 Axes{2, 5, 0, 3, 1, 4});
+      eTb[i] = fit_chi(tn::reshape(e, Shape{T.shape[2] * T.shape[2],
+                                            T.shape[0] * T.shape[0],
+                                            T.shape[1], T.shape[1]}),
+                       chi);
+    }
+    num = lat.left(i);
+    {
+      const auto &T = Tn[num];
+      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{0, 4}, Axes{0, 4},
+                                 Axes{2, 5, 0, 3, 1, 4});
+      eTl[i] = fit_chi(tn::reshape(e, Shape{T.shape[3] * T.shape[3],
+                                            T.shape[1] * T.shape[1],
+                                            T.shape[2], T.shape[2]}),
+                       chi);
+    }
+  }
+}
+
+// ctm.cpp:971-1253
+int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
+                         std::vector<tensor> &C3, std::vector<tensor> &C4,
+                         std::vector<tensor> &eTt, std::vector<tensor> &eTr,
+                         std::vector<tensor> &eTb, std::vector<tensor> &eTl,
+                         const std::vector<tensor> &Tn,
+                         const PEPS_Parameters &pp, const SquareLattice &lat,
+                         bool initialize, double *sig_max_out) {
+  if (initialize) Initialize_CTM(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, pp, lat);
+  bool convergence = false;
+  int count = 0;
+  std::vector<tensor> C1_old = C1, C2_old = C2, C3_old = C3, C4_old = C4;
+  double sig_max = 0.0;
+  const int LX = lat.LX_noskew, LY = lat.LY_noskew;
+  while (!convergence && count < pp.Max_CTM_Iteration) {
+    for (int ix = 0; ix < LX; ++ix)
+      Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, pp, lat);
+    for (int ix = 0; ix > -LX; --ix)
+      Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX, pp,
+                 lat);
+    for (int iy = 0; iy > -LY; --iy)
+      Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY, pp,
+               lat);
+    for (int iy = 0; iy < LY; ++iy)
+      Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, pp, lat);
+    convergence = Check_Convergence_CTM(C1, C2, C3, C4, C1_old, C2_old,
+                                        C3_old, C4_old, pp, lat, sig_max);
+    count += 1;
+    C1_old = C1;
+    C2_old = C2;
+    C3_old = C3;
+    C4_old = C4;
+    if (pp.print_level >= 3)
+      std::printf("CTM: count, sig_max %d %.17g\n", count, sig_max);
+  }
+  if (!convergence && pp.print_level >= 2)
+    std::printf("Warning: CTM did not converge! count, sig_max = %d %.17g\n",
+                count, sig_max);
+  if (sig_max_out) *sig_max_out = sig_max;
+  return count;
+}
+
+}  // namespace core
+}  // namespace itps
+}  // namespace tenes

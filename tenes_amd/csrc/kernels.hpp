@@ -1,0 +1,116 @@
+// Launch interfaces of the hand-written gfx950 kernels.  Host code in the
+// library calls only these; every kernel runs on the stream it is given.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tn {
+namespace kern {
+
+// ---------------------------------------------------------------------------
+// Gather-GEMM:  C[m,n] = alpha * sum_k A[m,k] * B[k,n] + beta * C[m,n]
+// with every operand addressed through offset tables, so any tensor
+// permutation (tensordot leg order, output transpose) is fused into the
+// tile loads and the epilogue store instead of a separate permute pass:
+//   A[m,k] = A[a_m[m] + a_k[k]],  B[k,n] = B[b_k[k] + b_n[n]],
+//   C[m,n] = C[c_m[m] + c_n[n]].
+// a_mfast / b_nfast tell the loader which index walks contiguous memory.
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  const double *A;
+  const int *a_m;
+  const int *a_k;
+  const double *B;
+  const int *b_k;
+  const int *b_n;
+  double *C;
+  const int *c_m;
+  const int *c_n;
+  int M, N, K;
+  double alpha, beta;
+  bool a_mfast, b_nfast;
+  // Affine fast path hints: when an offset table is an arithmetic progression
+  // the loader computes the offset instead of reading the table.  0 = use the
+  // table, otherwise the stride.
+  int a_m_stride, a_k_stride, b_k_stride, b_n_stride;
+};
+
+// Scratch needed for split-K on the given shape (bytes, 0 if none).
+size_t gemm_workspace_bytes(int M, int N, int K);
+void gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
+              hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Elementwise / reduction helpers
+// ---------------------------------------------------------------------------
+void fill(double *p, int64_t n, double v, hipStream_t s);
+// dst[dst_off(i)] = src[src_off(i)] for a rank<=8 row-major index space
+// `shape` (used for transpose / slice / extend copies).
+void strided_copy(const double *src, double *dst, int rank,
+                  const int64_t *shape, const int64_t *src_strides,
+                  const int64_t *dst_strides, int64_t n, hipStream_t s);
+// out[0] = max_i |x_i|  (device scalar, two-pass, deterministic)
+void max_abs(const double *x, int64_t n, double *out, double *partials,
+             hipStream_t s);
+int max_abs_partials(int64_t n);
+// x[i] /= *d  (d is a device scalar)
+void scale_by_inv(double *x, int64_t n, const double *d, hipStream_t s);
+void scale(double *x, int64_t n, double a, hipStream_t s);
+// out[0] = sum_i x[i]*y[i]
+void dot(const double *x, const double *y, int64_t n, double *out,
+         double *partials, hipStream_t s);
+int dot_partials(int64_t n);
+// y[i*inner + j + outer*...] *= v[(i / inner) % dim]  : scale along one axis
+void scale_axis(double *x, int64_t n, const double *v, int64_t dim,
+                int64_t inner, hipStream_t s);
+// y += a * x
+void axpy(double *y, const double *x, int64_t n, double a, hipStream_t s);
+// y += (*a_dev) * x * sign
+void axpy_dev(double *y, const double *x, int64_t n, const double *a_dev,
+              double sign, hipStream_t s);
+// out[r*k + q] = src[r*rs + perm[q]*cs] / (inv_scale ? scale[q] : 1)
+// (a zero scale gives a zero column).  rows x k output, row-major.
+void gather_cols(const double *src, int64_t rs, int64_t cs, const int *perm,
+                 const double *scale, int rows, int k, double *out,
+                 hipStream_t s);
+// Split-K reduction: C[c_m[m]+c_n[n]] = alpha*sum_z P[z][m][n] + beta*C
+void splitk_reduce(const double *P, int splits, int M, int N, double alpha,
+                   double beta, double *C, const int *c_m, const int *c_n,
+                   hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Dense linear algebra building blocks (FP64)
+// ---------------------------------------------------------------------------
+// Counter-based standard normal numbers (SplitMix64 hash + Box-Muller).
+void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
+           hipStream_t s);
+// In-place shifted Cholesky of a symmetric p x p matrix G (row-major,
+// p <= 128) using one workgroup: G + shift*I = R^T R, R upper triangular.
+// Also writes Rinv = R^{-1} (upper triangular).  shift is computed on the
+// device as shift_coef * trace(G).
+void chol_inv_upper(const double *G, int p, double shift_coef, double *R,
+                    double *Rinv, hipStream_t s);
+// One-sided (Hestenes) Jacobi SVD of a batch of p x p matrices with
+// rows = m (m, p <= 128), held in LDS by one workgroup per matrix.
+// W is column-major per matrix (col j at W + j*m); on exit its columns are
+// mutually orthogonal.  sigma[j] = ||W[:,j]||.  rotations (c,s) of every
+// step are appended to rot (size max_sweeps*(p-1)*(p/2) pairs of doubles
+// per matrix) unless rot == nullptr; nsteps[b] = number of recorded steps.
+int jacobi_max_dim();
+void jacobi_svd(double *W, int m, int p, int batch, double tol,
+                int max_sweeps, double *sigma, double *rot, int *nsteps,
+                hipStream_t s);
+// Rebuild the accumulated rotation matrix J (p x p, row-major) from the
+// rotation stream written by jacobi_svd (batch of 1).
+void jacobi_accumulate(const double *rot, const int *nsteps, int p, double *J,
+                       hipStream_t s);
+// sorted[b][r] = sigma[b][perm[b][r]], descending (ties keep index order).
+void sort_desc(const double *sigma, int p, int batch, double *sorted,
+               int *perm, hipStream_t s);
+// One-time per-device kernel attribute setup (large dynamic LDS).
+void linalg_init_device();
+
+}  // namespace kern
+}  // namespace tn

@@ -1,0 +1,189 @@
+// GPU FP64 SVD for the CTM projectors: range finder + CholeskyQR + one-sided
+// Jacobi, all on HBM-resident data, only the sorted singular values travel to
+// the host (they decide the projector cut, as in TeNeS ctm.cpp:281-288).
+#include "linalg.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace tn {
+
+namespace {
+
+constexpr int kJac = 128;  // LDS-resident Jacobi limit
+
+// Raw device buffer of ints carried by a Tensor (for lifetime management).
+struct IntBuf {
+  Tensor t;
+  int *ptr = nullptr;
+  IntBuf(Context &ctx, int64_t n) {
+    t = empty(ctx, Shape{std::max<int64_t>((n + 1) / 2, 1)});
+    ptr = reinterpret_cast<int *>(t.data());
+  }
+};
+
+double jacobi_tol_for(const SvdOptions &opt, int m) {
+  if (opt.jacobi_tol > 0.0) return opt.jacobi_tol;
+  return std::max(1.0, std::sqrt((double)m)) * 2.220446049250313e-16;
+}
+
+// Jacobi SVD of the rows x cols matrix held column-major in W (consumed).
+// Returns sorted sigma (device, length cols), perm (device), J (cols x cols,
+// row-major) if want_J.
+struct JacobiOut {
+  Tensor sigma_sorted;
+  IntBuf *perm = nullptr;
+  Tensor J;
+  Tensor W;
+  ~JacobiOut() { delete perm; }
+};
+
+void run_jacobi(Context &ctx, Tensor &W, int rows, int cols, bool want_J,
+                const SvdOptions &opt, JacobiOut &out) {
+  hipStream_t s = ctx.stream();
+  const int pe = cols + (cols & 1);
+  Tensor sigma = empty(ctx, Shape{cols});
+  IntBuf nsteps(ctx, 1);
+  Tensor rot;
+  if (want_J)
+    rot = empty(ctx, Shape{(int64_t)opt.max_sweeps * (pe - 1) * (pe / 2) * 2});
+  kern::jacobi_svd(W.data(), rows, cols, 1, jacobi_tol_for(opt, rows),
+                   opt.max_sweeps, sigma.data(), want_J ? rot.data() : nullptr,
+                   nsteps.ptr, s);
+  out.sigma_sorted = empty(ctx, Shape{cols});
+  out.perm = new IntBuf(ctx, cols);
+  kern::sort_desc(sigma.data(), cols, 1, out.sigma_sorted.data(),
+                  out.perm->ptr, s);
+  if (want_J) {
+    out.J = empty(ctx, Shape{cols, cols});
+    kern::jacobi_accumulate(rot.data(), nsteps.ptr, cols, out.J.data(), s);
+  }
+  out.W = W;
+}
+
+}  // namespace
+
+Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out) {
+  TN_REQUIRE(Y_in.rank() == 2, "orthonormalize: 2-D input expected");
+  const int64_t m = Y_in.shape[0], p = Y_in.shape[1];
+  TN_REQUIRE(p >= 1 && p <= kJac, "orthonormalize: block width > 128");
+  Tensor Y = Y_in;
+  Tensor Rtot;
+  const double u = 1.1102230246251565e-16;
+  for (int pass = 0; pass < 3; ++pass) {
+    Tensor G = tensordot(Y, Y, Axes{0}, Axes{0});  // p x p
+    Tensor R = empty(ctx, Shape{p, p});
+    Tensor Ri = empty(ctx, Shape{p, p});
+    const double coef =
+        pass == 0 ? 11.0 * (double)(m * p + p * (p + 1)) * u : 0.0;
+    kern::chol_inv_upper(G.data(), (int)p, coef, R.data(), Ri.data(),
+                         ctx.stream());
+    Y = tensordot(Y, Ri, Axes{1}, Axes{0});
+    if (R_out) Rtot = Rtot.defined() ? tensordot(R, Rtot, Axes{1}, Axes{0}) : R;
+  }
+  if (R_out) *R_out = Rtot;
+  return Y;
+}
+
+SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
+                  const SvdOptions &opt) {
+  TN_REQUIRE(A.rank() == 2, "svd_top: 2-D input expected");
+  const int m = (int)A.shape[0], n = (int)A.shape[1];
+  TN_REQUIRE(m >= 1 && n >= 1, "svd_top: empty matrix");
+  TN_REQUIRE(k_target >= 1, "svd_top: k must be positive");
+  TN_REQUIRE(k_target <= kJac,
+             "svd_top: at most 128 singular triplets are supported");
+  hipStream_t s = ctx.stream();
+
+  // A ~= Lq * X * Rq^T with orthonormal Lq (m x r), Rq (n x c); X is r x c.
+  Tensor Lq, Rq, X;
+  bool have_L = false, have_R = false;
+  if (m <= kJac && n <= kJac) {
+    X = A;
+  } else if (n <= kJac) {
+    Tensor Rf;
+    Lq = orthonormalize(ctx, A, &Rf);  // A = Lq Rf, Rf n x n
+    have_L = true;
+    X = Rf;
+  } else if (m <= kJac) {
+    Tensor Rf;
+    Tensor At = transpose(A, Axes{1, 0});  // n x m
+    Rq = orthonormalize(ctx, At, &Rf);     // A^T = Rq Rf -> A = Rf^T Rq^T
+    have_R = true;
+    X = transpose(Rf, Axes{1, 0});
+  } else {
+    const int p = kJac;
+    Tensor Om = empty(ctx, Shape{n, p});
+    kern::randn(Om.data(), Om.size(), ctx.seed,
+                ctx.next_rng_counter((uint64_t)Om.size()), s);
+    Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr);
+    for (int it = 0; it < opt.power_iters; ++it) {
+      Tensor Z = orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}), nullptr);
+      Q = orthonormalize(ctx, tensordot(A, Z, Axes{1}, Axes{0}), nullptr);
+    }
+    Tensor R2;
+    Tensor Bt = tensordot(A, Q, Axes{0}, Axes{0});  // n x p = A^T Q
+    Rq = orthonormalize(ctx, Bt, &R2);              // Bt = Rq R2
+    Lq = Q;
+    have_L = have_R = true;
+    X = transpose(R2, Axes{1, 0});  // B = Q^T A = R2^T Rq^T
+  }
+
+  const int rows = (int)X.shape[0], cols = (int)X.shape[1];
+  // Column-major copy of X = row-major X^T.
+  Tensor W = transpose(X, Axes{1, 0});
+  JacobiOut jo;
+  run_jacobi(ctx, W, rows, cols, true, opt, jo);
+  const int k = std::min(k_target, std::min(rows, cols));
+
+  // Xhat = W(:,perm)/sigma (rows x k), Jp = J(:,perm) (cols x k)
+  Tensor What = empty(ctx, Shape{rows, k});
+  kern::gather_cols(jo.W.data(), 1, rows, jo.perm->ptr, jo.sigma_sorted.data(),
+                    rows, k, What.data(), s);
+  Tensor Jp = empty(ctx, Shape{cols, k});
+  kern::gather_cols(jo.J.data(), cols, 1, jo.perm->ptr, nullptr, cols, k,
+                    Jp.data(), s);
+  SvdResult r;
+  r.U = have_L ? tensordot(Lq, What, Axes{1}, Axes{0}) : What;
+  r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
+  r.s_dev = slice(jo.sigma_sorted, 0, 0, k);
+  r.s.resize(k);
+  TN_HIP_CHECK(hipMemcpyAsync(r.s.data(), r.s_dev.data(), k * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  ctx.sync();
+  return r;
+}
+
+Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats) {
+  TN_REQUIRE(!mats.empty(), "svd_values_batched: empty batch");
+  const int m = (int)mats[0].shape[0], n = (int)mats[0].shape[1];
+  TN_REQUIRE(m <= kJac && n <= kJac,
+             "svd_values_batched: matrices larger than 128 x 128");
+  const int b = (int)mats.size();
+  hipStream_t s = ctx.stream();
+  // Jacobi acts on the columns of the stored column-major matrix.  Use the
+  // shorter side as the column length: for m >= n store X column-major.
+  const bool tr = m < n;  // then orthogonalise the columns of X^T
+  const int rows = tr ? n : m, cols = tr ? m : n;
+  Tensor W = empty(ctx, Shape{b, cols, rows});
+  for (int i = 0; i < b; ++i) {
+    TN_REQUIRE(mats[i].shape == mats[0].shape,
+               "svd_values_batched: shape mismatch in batch");
+    Tensor src = tr ? mats[i] : transpose(mats[i], Axes{1, 0});
+    TN_HIP_CHECK(hipMemcpyAsync(W.data() + (int64_t)i * rows * cols,
+                                src.data(), sizeof(double) * rows * cols,
+                                hipMemcpyDeviceToDevice, s));
+  }
+  Tensor sigma = empty(ctx, Shape{b, cols});
+  SvdOptions opt;
+  kern::jacobi_svd(W.data(), rows, cols, b, jacobi_tol_for(opt, rows),
+                   opt.max_sweeps, sigma.data(), nullptr, nullptr, s);
+  Tensor sorted = empty(ctx, Shape{b, cols});
+  IntBuf perm(ctx, (int64_t)b * cols);
+  kern::sort_desc(sigma.data(), cols, b, sorted.data(), perm.ptr, s);
+  return sorted;
+}
+
+}  // namespace tn

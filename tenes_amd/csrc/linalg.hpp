@@ -1,0 +1,40 @@
+// GPU FP64 SVD used by the CTM projectors and the convergence check.
+#pragma once
+
+#include <vector>
+
+#include "tensor.hpp"
+
+namespace tn {
+
+struct SvdOptions {
+  int power_iters = 4;        // subspace iterations for the range finder
+  int max_sweeps = 30;        // Jacobi sweep cap
+  double jacobi_tol = 0.0;    // 0: choose from the dimension
+};
+
+struct SvdResult {
+  Tensor U;                 // (m, k): left singular vectors as columns
+  Tensor V;                 // (n, k): right singular vectors as columns
+  Tensor s_dev;             // (k): singular values, descending
+  std::vector<double> s;    // host copy of s_dev
+};
+
+// Leading k singular triplets of the m x n matrix A (row-major 2-D tensor).
+// Exact one-sided Jacobi SVD when min(m, n) <= 128 (after a CholeskyQR
+// reduction when max(m, n) > 128); otherwise a randomized subspace iteration
+// with a 128-wide block followed by the same Jacobi SVD of the projected
+// problem.  The returned k is min(k_target, rank-space size).
+SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
+                  const SvdOptions &opt);
+
+// Singular values (descending, all min(m,n) of them) of a batch of equal
+// shaped matrices, all of them <= 128 x 128.  Result: device tensor
+// (batch, min(m,n)).
+Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats);
+
+// Orthonormal basis of the column space of Y (m x p row-major) by shifted
+// CholeskyQR3.  If R != nullptr the p x p upper factor (Y = Q R) is returned.
+Tensor orthonormalize(Context &ctx, const Tensor &Y, Tensor *R);
+
+}  // namespace tn

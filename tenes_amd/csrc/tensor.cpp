@@ -1,0 +1,555 @@
+// Device tensor operations: allocator, context, tensordot planning.
+#include "tensor.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+#include "kernels.hpp"
+
+namespace tn {
+
+// ---------------------------------------------------------------------------
+// Allocator
+// ---------------------------------------------------------------------------
+size_t Allocator::round(size_t bytes) {
+  if (bytes < 512) return 512;
+  // four size classes per power of two
+  size_t p = 1;
+  while (p < bytes) p <<= 1;
+  const size_t q = p >> 3;  // eighth of the next power of two
+  return ((bytes + q - 1) / q) * q;
+}
+
+void *Allocator::alloc(size_t bytes) {
+  std::lock_guard<std::mutex> lk(mu_);
+  const size_t b = round(bytes);
+  auto it = free_.lower_bound(b);
+  if (it != free_.end() && it->first <= b + b / 4) {
+    void *p = it->second;
+    cached_ -= it->first;
+    in_use_ += it->first;
+    free_.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    // release the cache and retry once
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached_ = 0;
+    e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error("device allocation of " + std::to_string(b) +
+                      " bytes failed: " + hipGetErrorString(e),
+                  -4);
+    }
+  }
+  in_use_ += b;
+  return p;
+}
+
+void Allocator::free(void *p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
+  const size_t b = round(bytes);
+  in_use_ -= b;
+  cached_ += b;
+  free_.emplace(b, p);
+}
+
+void Allocator::release_cached() {
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto &kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+  cached_ = 0;
+}
+
+Allocator::~Allocator() {
+  for (auto &kv : free_) (void)hipFree(kv.second);
+}
+
+Storage::~Storage() {
+  if (owned && ptr && ctx) ctx->allocator().free(ptr, bytes);
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+Context::Context(int device, hipStream_t stream)
+    : device_(device), stream_(stream), alloc_(device) {
+  TN_HIP_CHECK(hipSetDevice(device));
+  if (stream_ == nullptr) {
+    TN_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  kern::linalg_init_device();
+  TN_HIP_CHECK(hipMalloc(&scal_, 4096 * sizeof(double)));
+}
+
+Context::~Context() {
+  (void)hipStreamSynchronize(stream_);
+  for (auto &kv : plans_) (void)hipFree(kv.second.d_tab);
+  if (ws_) alloc_.free(ws_, ws_bytes_);
+  if (scal_) (void)hipFree(scal_);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void *Context::workspace(size_t bytes) {
+  if (bytes <= ws_bytes_) return ws_;
+  if (ws_) alloc_.free(ws_, ws_bytes_);
+  ws_bytes_ = std::max(bytes, ws_bytes_ * 2);
+  ws_ = alloc_.alloc(ws_bytes_);
+  return ws_;
+}
+
+double *Context::scalars(int n) {
+  TN_REQUIRE(n <= 4096, "scalar area exhausted");
+  return scal_;
+}
+
+void Context::sync() { TN_HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+std::string shape_str(const Shape &s) {
+  std::ostringstream os;
+  os << "(";
+  for (size_t i = 0; i < s.size(); ++i) os << (i ? "," : "") << s[i];
+  os << ")";
+  return os.str();
+}
+
+static std::vector<int64_t> row_major_strides(const Shape &s) {
+  std::vector<int64_t> st(s.size());
+  int64_t acc = 1;
+  for (int i = (int)s.size() - 1; i >= 0; --i) {
+    st[i] = acc;
+    acc *= s[i];
+  }
+  return st;
+}
+
+// Offsets of a row-major multi-index over `dims` with the given strides.
+static void build_offsets(const std::vector<int64_t> &dims,
+                          const std::vector<int64_t> &strides,
+                          std::vector<int> &out) {
+  int64_t n = 1;
+  for (auto d : dims) n *= d;
+  out.resize((size_t)n);
+  const int r = (int)dims.size();
+  std::vector<int64_t> idx(r, 0);
+  int64_t off = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    TN_REQUIRE(off <= INT32_MAX, "tensor too large for 32-bit offsets");
+    out[(size_t)i] = (int)off;
+    for (int d = r - 1; d >= 0; --d) {
+      idx[d]++;
+      off += strides[d];
+      if (idx[d] < dims[d]) break;
+      off -= strides[d] * dims[d];
+      idx[d] = 0;
+    }
+  }
+}
+
+static int affine_stride(const std::vector<int> &tab) {
+  if (tab.size() <= 1) return 1;
+  if (tab[0] != 0) return 0;
+  const int s = tab[1];
+  if (s <= 0) return 0;
+  for (size_t i = 2; i < tab.size(); ++i)
+    if ((int64_t)tab[i] != (int64_t)i * s) return 0;
+  return s;
+}
+
+const ContractPlan &Context::plan(const Shape &sa, const Shape &sb,
+                                  const Axes &axa, const Axes &axb,
+                                  const Axes &perm) {
+  std::ostringstream key;
+  key << shape_str(sa) << shape_str(sb) << "|";
+  for (auto a : axa) key << a << ",";
+  key << "|";
+  for (auto a : axb) key << a << ",";
+  key << "|";
+  for (auto a : perm) key << a << ",";
+  auto it = plans_.find(key.str());
+  if (it != plans_.end()) return it->second;
+
+  const int ra = (int)sa.size(), rb = (int)sb.size(), nc = (int)axa.size();
+  TN_REQUIRE((int)axb.size() == nc, "tensordot: axes length mismatch");
+  std::vector<int> useda(ra, 0), usedb(rb, 0);
+  for (int i = 0; i < nc; ++i) {
+    TN_REQUIRE(axa[i] >= 0 && axa[i] < ra && axb[i] >= 0 && axb[i] < rb,
+               "tensordot: axis out of range");
+    TN_REQUIRE(!useda[axa[i]] && !usedb[axb[i]], "tensordot: repeated axis");
+    useda[axa[i]] = usedb[axb[i]] = 1;
+    TN_REQUIRE(sa[axa[i]] == sb[axb[i]],
+               "tensordot: contracted dimension mismatch " + shape_str(sa) +
+                   " x " + shape_str(sb));
+  }
+  const auto stA = row_major_strides(sa), stB = row_major_strides(sb);
+  std::vector<int> freeA, freeB;
+  for (int i = 0; i < ra; ++i)
+    if (!useda[i]) freeA.push_back(i);
+  for (int i = 0; i < rb; ++i)
+    if (!usedb[i]) freeB.push_back(i);
+  Shape res;
+  for (int i : freeA) res.push_back(sa[i]);
+  for (int i : freeB) res.push_back(sb[i]);
+  const int rr = (int)res.size();
+  Axes pm = perm;
+  if (pm.empty()) {
+    pm.resize(rr);
+    std::iota(pm.begin(), pm.end(), 0);
+  }
+  TN_REQUIRE((int)pm.size() == rr, "tensordot: output permutation rank");
+  Shape outs(rr);
+  {
+    std::vector<int> seen(rr, 0);
+    for (int k = 0; k < rr; ++k) {
+      TN_REQUIRE(pm[k] >= 0 && pm[k] < rr && !seen[pm[k]],
+                 "tensordot: bad output permutation");
+      seen[pm[k]] = 1;
+      outs[k] = res[pm[k]];
+    }
+  }
+  const auto stO = row_major_strides(outs);
+  std::vector<int64_t> cstride(rr);
+  for (int k = 0; k < rr; ++k) cstride[pm[k]] = stO[k];
+
+  // Order contracted pairs so that the operand whose unit-stride leg is
+  // contracted walks contiguous memory along k.
+  std::vector<int> order(nc);
+  std::iota(order.begin(), order.end(), 0);
+  const bool a_last_contracted = ra > 0 && useda[ra - 1];
+  const bool b_last_contracted = rb > 0 && usedb[rb - 1];
+  if (a_last_contracted) {
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return stA[axa[x]] > stA[axa[y]];
+    });
+  } else if (b_last_contracted) {
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return stB[axb[x]] > stB[axb[y]];
+    });
+  }
+
+  std::vector<int64_t> dm, sm, dk, sak, sbk, dn, sn, cm, cn;
+  for (size_t q = 0; q < freeA.size(); ++q) {
+    dm.push_back(sa[freeA[q]]);
+    sm.push_back(stA[freeA[q]]);
+    cm.push_back(cstride[q]);
+  }
+  for (int q : order) {
+    dk.push_back(sa[axa[q]]);
+    sak.push_back(stA[axa[q]]);
+    sbk.push_back(stB[axb[q]]);
+  }
+  for (size_t q = 0; q < freeB.size(); ++q) {
+    dn.push_back(sb[freeB[q]]);
+    sn.push_back(stB[freeB[q]]);
+    cn.push_back(cstride[freeA.size() + q]);
+  }
+  std::vector<int> t_am, t_ak, t_bk, t_bn, t_cm, t_cn;
+  build_offsets(dm, sm, t_am);
+  build_offsets(dk, sak, t_ak);
+  build_offsets(dk, sbk, t_bk);
+  build_offsets(dn, sn, t_bn);
+  build_offsets(dm, cm, t_cm);
+  build_offsets(dn, cn, t_cn);
+
+  ContractPlan p;
+  p.M = (int)t_am.size();
+  p.K = (int)t_ak.size();
+  p.N = (int)t_bn.size();
+  p.out_shape = outs;
+  p.a_m_off = 0;
+  p.a_k_off = p.a_m_off + p.M;
+  p.b_k_off = p.a_k_off + p.K;
+  p.b_n_off = p.b_k_off + p.K;
+  p.c_m_off = p.b_n_off + p.N;
+  p.c_n_off = p.c_m_off + p.M;
+  const size_t total = (size_t)p.c_n_off + p.N;
+  std::vector<int> all;
+  all.reserve(total);
+  all.insert(all.end(), t_am.begin(), t_am.end());
+  all.insert(all.end(), t_ak.begin(), t_ak.end());
+  all.insert(all.end(), t_bk.begin(), t_bk.end());
+  all.insert(all.end(), t_bn.begin(), t_bn.end());
+  all.insert(all.end(), t_cm.begin(), t_cm.end());
+  all.insert(all.end(), t_cn.begin(), t_cn.end());
+  TN_HIP_CHECK(hipMalloc(&p.d_tab, std::max<size_t>(total, 1) * sizeof(int)));
+  if (total)
+    TN_HIP_CHECK(hipMemcpy(p.d_tab, all.data(), total * sizeof(int),
+                           hipMemcpyHostToDevice));
+  p.a_m_stride = affine_stride(t_am);
+  p.a_k_stride = affine_stride(t_ak);
+  p.b_k_stride = affine_stride(t_bk);
+  p.b_n_stride = affine_stride(t_bn);
+  const bool am1 = p.M > 1 && t_am[1] - t_am[0] == 1;
+  const bool ak1 = p.K > 1 && t_ak[1] - t_ak[0] == 1;
+  p.a_mfast = am1 && !(ak1 && p.K >= 16 && p.M < 16);
+  const bool bn1 = p.N > 1 && t_bn[1] - t_bn[0] == 1;
+  const bool bk1 = p.K > 1 && t_bk[1] - t_bk[0] == 1;
+  p.b_nfast = bn1 && !(bk1 && p.K >= 16 && p.N < 16);
+  return plans_.emplace(key.str(), p).first->second;
+}
+
+// ---------------------------------------------------------------------------
+// Construction
+// ---------------------------------------------------------------------------
+static Tensor make(Context &ctx, const Shape &shape) {
+  Tensor t;
+  t.shape = shape;
+  auto st = std::make_shared<Storage>();
+  st->ctx = &ctx;
+  int64_t n = 1;
+  for (auto d : shape) {
+    TN_REQUIRE(d >= 0, "negative dimension");
+    n *= d;
+  }
+  st->bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(double);
+  st->ptr = static_cast<double *>(ctx.allocator().alloc(st->bytes));
+  t.st = st;
+  return t;
+}
+
+Tensor empty(Context &ctx, const Shape &shape) { return make(ctx, shape); }
+
+Tensor zeros(Context &ctx, const Shape &shape) {
+  Tensor t = make(ctx, shape);
+  kern::fill(t.data(), t.size(), 0.0, ctx.stream());
+  return t;
+}
+
+Tensor from_host(Context &ctx, const Shape &shape, const double *host) {
+  Tensor t = make(ctx, shape);
+  if (t.size())
+    TN_HIP_CHECK(hipMemcpyAsync(t.data(), host, t.size() * sizeof(double),
+                                hipMemcpyHostToDevice, ctx.stream()));
+  ctx.sync();
+  return t;
+}
+
+void to_host(const Tensor &t, double *host) {
+  TN_REQUIRE(t.defined(), "to_host: undefined tensor");
+  Context &ctx = *t.ctx();
+  if (t.size())
+    TN_HIP_CHECK(hipMemcpyAsync(host, t.data(), t.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx.stream()));
+  ctx.sync();
+}
+
+Tensor borrow(Context &ctx, const Shape &shape, double *dev) {
+  Tensor t;
+  t.shape = shape;
+  auto st = std::make_shared<Storage>();
+  st->ctx = &ctx;
+  st->ptr = dev;
+  st->owned = false;
+  int64_t n = 1;
+  for (auto d : shape) n *= d;
+  st->bytes = (size_t)n * sizeof(double);
+  t.st = st;
+  return t;
+}
+
+Tensor clone(const Tensor &t) {
+  Context &ctx = *t.ctx();
+  Tensor r = make(ctx, t.shape);
+  if (t.size())
+    TN_HIP_CHECK(hipMemcpyAsync(r.data(), t.data(), t.size() * sizeof(double),
+                                hipMemcpyDeviceToDevice, ctx.stream()));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Shape operations
+// ---------------------------------------------------------------------------
+Tensor reshape(const Tensor &t, const Shape &shape) {
+  int64_t n = 1;
+  for (auto d : shape) n *= d;
+  TN_REQUIRE(n == t.size(), "reshape: size mismatch " + shape_str(t.shape) +
+                                " -> " + shape_str(shape));
+  Tensor r = t;
+  r.shape = shape;
+  return r;
+}
+
+Tensor transpose(const Tensor &t, const Axes &axes) {
+  const int r = t.rank();
+  TN_REQUIRE((int)axes.size() == r, "transpose: rank mismatch");
+  Context &ctx = *t.ctx();
+  const auto st = row_major_strides(t.shape);
+  Shape ns(r);
+  std::vector<int64_t> src_str(r);
+  for (int k = 0; k < r; ++k) {
+    TN_REQUIRE(axes[k] >= 0 && axes[k] < r, "transpose: bad axis");
+    ns[k] = t.shape[axes[k]];
+    src_str[k] = st[axes[k]];
+  }
+  Tensor out = make(ctx, ns);
+  const auto dst_str = row_major_strides(ns);
+  kern::strided_copy(t.data(), out.data(), r, ns.data(), src_str.data(),
+                     dst_str.data(), out.size(), ctx.stream());
+  return out;
+}
+
+void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
+                    const Axes &axB, const Axes &perm, Tensor &C, double alpha,
+                    double beta) {
+  Context &ctx = *A.ctx();
+  const ContractPlan &p = ctx.plan(A.shape, B.shape, axA, axB, perm);
+  TN_REQUIRE(C.shape == p.out_shape, "tensordot_into: output shape mismatch");
+  kern::GemmArgs g;
+  g.A = A.data();
+  g.B = B.data();
+  g.C = C.data();
+  g.a_m = p.d_tab + p.a_m_off;
+  g.a_k = p.d_tab + p.a_k_off;
+  g.b_k = p.d_tab + p.b_k_off;
+  g.b_n = p.d_tab + p.b_n_off;
+  g.c_m = p.d_tab + p.c_m_off;
+  g.c_n = p.d_tab + p.c_n_off;
+  g.M = p.M;
+  g.N = p.N;
+  g.K = p.K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.a_mfast = p.a_mfast;
+  g.b_nfast = p.b_nfast;
+  g.a_m_stride = p.a_m_stride;
+  g.a_k_stride = p.a_k_stride;
+  g.b_k_stride = p.b_k_stride;
+  g.b_n_stride = p.b_n_stride;
+  const size_t ws = kern::gemm_workspace_bytes(p.M, p.N, p.K);
+  void *w = ws ? ctx.workspace(ws) : nullptr;
+  kern::gemm_f64(g, w, ws, ctx.stream());
+  ctx.gemm_flops += 2.0 * p.M * (double)p.N * p.K;
+  ctx.gemm_calls++;
+}
+
+Tensor tensordot_t(const Tensor &A, const Tensor &B, const Axes &axA,
+                   const Axes &axB, const Axes &perm) {
+  TN_REQUIRE(A.defined() && B.defined(), "tensordot: undefined operand");
+  Context &ctx = *A.ctx();
+  const ContractPlan &p = ctx.plan(A.shape, B.shape, axA, axB, perm);
+  Tensor C = make(ctx, p.out_shape);
+  tensordot_into(A, B, axA, axB, perm, C, 1.0, 0.0);
+  return C;
+}
+
+Tensor tensordot(const Tensor &A, const Tensor &B, const Axes &axA,
+                 const Axes &axB) {
+  return tensordot_t(A, B, axA, axB, Axes{});
+}
+
+Tensor slice(const Tensor &t, int axis, int64_t begin, int64_t end) {
+  const int r = t.rank();
+  TN_REQUIRE(axis >= 0 && axis < r, "slice: bad axis");
+  TN_REQUIRE(0 <= begin && begin <= end && end <= t.shape[axis],
+             "slice: bad range");
+  Context &ctx = *t.ctx();
+  Shape ns = t.shape;
+  ns[axis] = end - begin;
+  Tensor out = make(ctx, ns);
+  const auto st = row_major_strides(t.shape);
+  const auto dst = row_major_strides(ns);
+  const double *src = t.data() + begin * st[axis];
+  kern::strided_copy(src, out.data(), r, ns.data(), st.data(), dst.data(),
+                     out.size(), ctx.stream());
+  return out;
+}
+
+Tensor extend(const Tensor &t, const Shape &shape) {
+  const int r = t.rank();
+  TN_REQUIRE((int)shape.size() == r, "extend: rank mismatch");
+  for (int i = 0; i < r; ++i)
+    TN_REQUIRE(shape[i] >= t.shape[i], "extend: target smaller than source");
+  Context &ctx = *t.ctx();
+  Tensor out = zeros(ctx, shape);
+  const auto st = row_major_strides(t.shape);
+  const auto dst = row_major_strides(shape);
+  kern::strided_copy(t.data(), out.data(), r, t.shape.data(), st.data(),
+                     dst.data(), t.size(), ctx.stream());
+  return out;
+}
+
+Tensor conj(const Tensor &t) { return t; }
+
+void trace_dev(const Tensor &A, const Tensor &B, const Axes &axA,
+               const Axes &axB, double *out) {
+  TN_REQUIRE(A.rank() == (int)axA.size() && B.rank() == (int)axB.size() &&
+                 axA.size() == axB.size(),
+             "trace: every leg must be contracted");
+  Context &ctx = *A.ctx();
+  // Bring A into B's leg order: A' leg axB[k] = A leg axA[k].
+  const int r = A.rank();
+  Axes perm(r);
+  for (int k = 0; k < r; ++k) perm[axB[k]] = axA[k];
+  bool ident = true;
+  for (int k = 0; k < r; ++k) ident = ident && perm[k] == k;
+  Tensor Ap = ident ? A : transpose(A, perm);
+  TN_REQUIRE(Ap.shape == B.shape, "trace: shape mismatch");
+  const int np = kern::dot_partials(B.size());
+  double *part = static_cast<double *>(ctx.workspace(np * sizeof(double)));
+  kern::dot(Ap.data(), B.data(), B.size(), out, part, ctx.stream());
+}
+
+double trace(const Tensor &A, const Tensor &B, const Axes &axA,
+             const Axes &axB) {
+  Context &ctx = *A.ctx();
+  double *d = ctx.scalars(1);
+  trace_dev(A, B, axA, axB, d);
+  double h = 0.0;
+  TN_HIP_CHECK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx.stream()));
+  ctx.sync();
+  return h;
+}
+
+double max_abs(const Tensor &t) {
+  Context &ctx = *t.ctx();
+  double *d = ctx.scalars(1);
+  const int np = kern::max_abs_partials(t.size());
+  double *part = static_cast<double *>(ctx.workspace(np * sizeof(double)));
+  kern::max_abs(t.data(), t.size(), d, part, ctx.stream());
+  double h = 0.0;
+  TN_HIP_CHECK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx.stream()));
+  ctx.sync();
+  return h;
+}
+
+void normalize_max_abs(Tensor &t) {
+  Context &ctx = *t.ctx();
+  double *d = ctx.scalars(1);
+  const int np = kern::max_abs_partials(t.size());
+  double *part = static_cast<double *>(ctx.workspace(np * sizeof(double)));
+  kern::max_abs(t.data(), t.size(), d, part, ctx.stream());
+  kern::scale_by_inv(t.data(), t.size(), d, ctx.stream());
+}
+
+void multiply_vector_dev(Tensor &t, const double *v_dev, int axis) {
+  TN_REQUIRE(axis >= 0 && axis < t.rank(), "multiply_vector: bad axis");
+  int64_t inner = 1;
+  for (int i = axis + 1; i < t.rank(); ++i) inner *= t.shape[i];
+  kern::scale_axis(t.data(), t.size(), v_dev, t.shape[axis], inner,
+                   t.ctx()->stream());
+}
+
+void scale_inplace(Tensor &t, double a) {
+  kern::scale(t.data(), t.size(), a, t.ctx()->stream());
+}
+
+Tensor add(const Tensor &a, const Tensor &b, double alpha, double beta) {
+  TN_REQUIRE(a.shape == b.shape, "add: shape mismatch");
+  Tensor r = clone(a);
+  if (alpha != 1.0) scale_inplace(r, alpha);
+  kern::axpy(r.data(), b.data(), r.size(), beta, r.ctx()->stream());
+  return r;
+}
+
+}  // namespace tn

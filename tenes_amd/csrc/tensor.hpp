@@ -1,0 +1,176 @@
+// Device tensor type and the mptensor-like operation subset the CTMRG path
+// uses (tensordot, transpose, reshape, slice, extend, max_abs, trace,
+// multiply_vector), restated on HBM-resident FP64 row-major tensors.
+//
+// Semantics follow how TeNeS uses mptensor (src/iTPS/core/ctm.cpp and
+// contract_itps_ctm/*.cpp call sites):
+//   * tensordot(A, B, axA, axB): result legs = free legs of A (in order), then
+//     free legs of B (in order);
+//   * transpose(T, axes): numpy order, new leg k = old leg axes[k];
+//   * reshape: row-major merge/split of legs;
+//   * tensors have value semantics.  Here every operation returns a fresh
+//     tensor and storage is never mutated once shared, so sharing storage
+//     between copies is equivalent to the reference's deep copies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace tn {
+
+using Shape = std::vector<int64_t>;
+using Axes = std::vector<int>;
+
+class Context;
+
+// Size-bucketed caching allocator.  All work of a Context runs on one HIP
+// stream, so a freed block may be handed to the next allocation without a
+// device synchronisation: the stream orders the reuse after the last use.
+class Allocator {
+ public:
+  explicit Allocator(int device) : device_(device) {}
+  ~Allocator();
+  void *alloc(size_t bytes);
+  void free(void *p, size_t bytes);
+  void release_cached();
+  size_t bytes_in_use() const { return in_use_; }
+  size_t bytes_cached() const { return cached_; }
+
+ private:
+  static size_t round(size_t bytes);
+  int device_;
+  std::multimap<size_t, void *> free_;
+  size_t in_use_ = 0, cached_ = 0;
+  std::mutex mu_;
+};
+
+struct Storage {
+  double *ptr = nullptr;
+  size_t bytes = 0;
+  Context *ctx = nullptr;
+  bool owned = true;
+  ~Storage();
+};
+
+class Tensor {
+ public:
+  Tensor() = default;
+  Shape shape;
+  std::shared_ptr<Storage> st;
+
+  int rank() const { return (int)shape.size(); }
+  int64_t size() const {
+    int64_t n = 1;
+    for (auto d : shape) n *= d;
+    return n;
+  }
+  double *data() const { return st ? st->ptr : nullptr; }
+  Context *ctx() const { return st ? st->ctx : nullptr; }
+  bool defined() const { return (bool)st; }
+};
+
+// Offset tables of one tensordot (device resident, cached per signature).
+struct ContractPlan {
+  int M = 0, N = 0, K = 0;
+  Shape out_shape;
+  int *d_tab = nullptr;  // a_m | a_k | b_k | b_n | c_m | c_n
+  int a_m_off, a_k_off, b_k_off, b_n_off, c_m_off, c_n_off;
+  bool a_mfast = false, b_nfast = false;
+  int a_m_stride = 0, a_k_stride = 0, b_k_stride = 0, b_n_stride = 0;
+};
+
+class Context {
+ public:
+  explicit Context(int device, hipStream_t stream = nullptr);
+  ~Context();
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  void set_stream(hipStream_t s) { stream_ = s; }
+  Allocator &allocator() { return alloc_; }
+
+  // Scratch (grows on demand, stream ordered).
+  void *workspace(size_t bytes);
+  double *scalars(int n);  // small device scalar area
+
+  const ContractPlan &plan(const Shape &sa, const Shape &sb, const Axes &axa,
+                           const Axes &axb, const Axes &perm);
+  void sync();
+  uint64_t next_rng_counter(uint64_t n) {
+    uint64_t c = rng_counter_;
+    rng_counter_ += n;
+    return c;
+  }
+  uint64_t seed = 0x5EEDC7A1ull;
+
+  std::string last_error;
+
+  // accumulated statistics (for benchmarks)
+  double gemm_flops = 0.0;
+  int64_t gemm_calls = 0;
+
+ private:
+  int device_;
+  hipStream_t stream_;
+  bool own_stream_ = false;
+  Allocator alloc_;
+  void *ws_ = nullptr;
+  size_t ws_bytes_ = 0;
+  double *scal_ = nullptr;
+  std::map<std::string, ContractPlan> plans_;
+  uint64_t rng_counter_ = 0;
+};
+
+// ---- construction / transfer ----------------------------------------------
+Tensor empty(Context &ctx, const Shape &shape);
+Tensor zeros(Context &ctx, const Shape &shape);
+Tensor from_host(Context &ctx, const Shape &shape, const double *host);
+void to_host(const Tensor &t, double *host);  // synchronous
+// Wrap caller-owned device memory (not freed by the library).
+Tensor borrow(Context &ctx, const Shape &shape, double *dev);
+Tensor clone(const Tensor &t);
+
+// ---- mptensor-subset -------------------------------------------------------
+Tensor reshape(const Tensor &t, const Shape &shape);  // shares storage
+Tensor transpose(const Tensor &t, const Axes &axes);
+Tensor tensordot(const Tensor &A, const Tensor &B, const Axes &axA,
+                 const Axes &axB);
+// tensordot followed by a transpose of the result, fused into the store.
+Tensor tensordot_t(const Tensor &A, const Tensor &B, const Axes &axA,
+                   const Axes &axB, const Axes &perm);
+// C(+)= tensordot into an existing tensor with the given output perm.
+void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
+                    const Axes &axB, const Axes &perm, Tensor &C, double alpha,
+                    double beta);
+Tensor slice(const Tensor &t, int axis, int64_t begin, int64_t end);
+Tensor extend(const Tensor &t, const Shape &shape);  // zero padding
+Tensor conj(const Tensor &t);                         // real: identity
+// Full contraction of A and B over all legs (A leg axA[k] with B leg axB[k])
+// to a scalar (mptensor trace(A, B, axA, axB)).  Synchronous.
+double trace(const Tensor &A, const Tensor &B, const Axes &axA,
+             const Axes &axB);
+// Same, result left on the device in *out.
+void trace_dev(const Tensor &A, const Tensor &B, const Axes &axA,
+               const Axes &axB, double *out);
+double max_abs(const Tensor &t);  // synchronous
+// t /= max_abs(t), computed and applied on the device.
+void normalize_max_abs(Tensor &t);
+// Scale leg `axis` by the device vector v (length shape[axis]).
+void multiply_vector_dev(Tensor &t, const double *v_dev, int axis);
+void scale_inplace(Tensor &t, double a);
+Tensor add(const Tensor &a, const Tensor &b, double alpha = 1.0,
+           double beta = 1.0);
+
+std::string shape_str(const Shape &s);
+
+}  // namespace tn

@@ -1,0 +1,242 @@
+"""GPU CTMRG path against the oracle on the same seeded inputs.
+
+Renormalisation steps without an SVD (Calc_Next_CTM, Calc_Next_eT, the
+environment contractions, the transfer matrix) are compared entry by entry.
+Steps whose output depends on the SVD gauge (projectors, moves, whole
+sweeps) are compared through gauge-invariant quantities: the oblique
+projector PU*PL^T, L2-normalised corner spectra and observables.
+Tolerances: 1e-11 relative for direct FP64 results, 1e-10 for observables
+after whole CTM runs (FP64 mode target of the path)."""
+import numpy as np
+import pytest
+
+import tenes_amd as ta
+from oracle import contract as K, ctm as O, itps, transfer as TMo
+from oracle.lattice import SquareLattice
+
+from ctm_cases import random_tensors, su_tensors
+from oracle import models
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ("C1", "C2", "C3", "C4", "eTt", "eTr", "eTb", "eTl")
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def random_env_pieces(rng, chi, D, d=2, cut=None):
+    cut = cut or chi
+    eT = rng.standard_normal((chi, chi, D, D))
+    Tn = rng.standard_normal((D, D, D, D, d))
+    PU = rng.standard_normal((chi, D, D, cut))
+    PL = rng.standard_normal((chi, D, D, cut))
+    return eT, Tn, PU, PL
+
+
+@pytest.mark.parametrize("chi,D,cut", [(6, 2, 5), (20, 3, 18), (100, 5, 100)])
+def test_calc_next_et(gpu_ctx, chi, D, cut):
+    rng = np.random.default_rng(chi + D)
+    eT, Tn, PU, PL = random_env_pieces(rng, chi, D, cut=cut)
+    ref = O.Calc_Next_eT(eT, Tn, PU, PL)
+    g = ta.calc_next_et(gpu_ctx.tensor(eT), gpu_ctx.tensor(Tn), gpu_ctx.tensor(PU), gpu_ctx.tensor(PL))
+    assert g.shape == ref.shape
+    assert rel(g.numpy(), ref) < 1e-11
+
+
+@pytest.mark.parametrize("chi,D,cut", [(6, 2, 5), (40, 4, 33), (100, 5, 100)])
+def test_calc_next_ctm(gpu_ctx, chi, D, cut):
+    rng = np.random.default_rng(3 * chi + D)
+    C1 = rng.standard_normal((chi, chi))
+    C4 = rng.standard_normal((chi, chi))
+    eT1 = rng.standard_normal((chi, chi, D, D))
+    eT6 = rng.standard_normal((chi, chi, D, D))
+    PU = rng.standard_normal((chi, D, D, cut))
+    PL = rng.standard_normal((chi, D, D, cut))
+    r1, r4 = O.Calc_Next_CTM(C1, C4, eT1, eT6, PU, PL)
+    t = gpu_ctx.tensor
+    g1, g4 = ta.calc_next_ctm(t(C1), t(C4), t(eT1), t(eT6), t(PU), t(PL))
+    assert rel(g1.numpy(), r1) < 1e-11
+    assert rel(g4.numpy(), r4) < 1e-11
+
+
+def oracle_env(model, D, chi, steps, sweeps):
+    Tn, ham = su_tensors(model, D, steps)
+    Tn = list(Tn)
+    lat = SquareLattice(2, 2)
+    p = O.PEPSParameters(CHI=chi, Max_CTM_Iteration=sweeps, CTM_Convergence_Epsilon=0.0)
+    env, _, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    return Tn, lat, env, ham
+
+
+def upload_env(ctx, Tn, env):
+    genv = ta.Environment(ctx, 2, 2)
+    for i in range(4):
+        genv.set("Tn", i, Tn[i])
+        for k in KINDS:
+            genv.set(k, i, getattr(env, k)[i])
+    return genv
+
+
+@pytest.mark.parametrize("model,D,chi", [("heisenberg", 2, 10), ("tfi", 3, 20)])
+def test_projector_left_block(gpu_ctx, model, D, chi):
+    Tn, lat, env, _ = oracle_env(model, D, chi, 40, 2)
+    p = O.PEPSParameters(CHI=chi)
+    i, l = 0, lat.bottom(0)
+    args = (env.C1[i], env.C4[l], env.eTt[i], env.eTb[l], env.eTl[l], env.eTl[i], Tn[i], Tn[l])
+    PU, PL = O.Calc_projector_left_block(*args, p)
+    gp = ta.peps_parameters(CHI=chi)
+    gPU, gPL = ta.calc_projector_left_block(*[gpu_ctx.tensor(a) for a in args], gp)
+    assert gPU.shape == PU.shape and gPL.shape == PL.shape
+    # the SVD gauge cancels in the oblique projector PU PL^T
+    P = np.tensordot(PU, PL, ([3], [3]))
+    gP = np.tensordot(gPU.numpy(), gPL.numpy(), ([3], [3]))
+    assert rel(gP, P) < 1e-10
+
+
+def test_projector_updown_blocks(gpu_ctx):
+    Tn, lat, env, _ = oracle_env("heisenberg", 2, 8, 40, 2)
+    p = O.PEPSParameters(CHI=8, CTM_Projector_corner=False)
+    i = 0
+    j = lat.right(i)
+    k = lat.bottom(j)
+    l = lat.left(k)
+    args = (env.C1[i], env.C2[j], env.C3[k], env.C4[l], env.eTt[i], env.eTt[j], env.eTr[j], env.eTr[k],
+            env.eTb[k], env.eTb[l], env.eTl[l], env.eTl[i], Tn[i], Tn[j], Tn[k], Tn[l])
+    PU, PL = O.Calc_projector_updown_blocks(*args, p)
+    gPU, gPL = ta.calc_projector_updown_blocks([gpu_ctx.tensor(a) for a in args],
+                                               ta.peps_parameters(CHI=8, CTM_Projector_corner=0))
+    P = np.tensordot(PU, PL, ([3], [3]))
+    gP = np.tensordot(gPU.numpy(), gPL.numpy(), ([3], [3]))
+    assert rel(gP, P) < 1e-10
+
+
+def observables_gpu(genv, lat, ham):
+    E = 0.0
+    for s, leg in models.square_bonds(lat):
+        dx, dy = (1, 0) if leg == 2 else (0, 1)
+        E += genv.contract_two_sites_op12(s, dx, dy, ham) / genv.contract_cluster(s, dx, dy)
+    mz = [genv.contract_one_site(i, models.Sz) / genv.contract_one_site(i) for i in range(4)]
+    return E / 4, np.array(mz)
+
+
+def observables_oracle(env, Tn, lat, ham):
+    one = [{"name": "Sz", "group": 0, "site": i, "op": models.Sz} for i in range(4)]
+    two = []
+    for s, leg in models.square_bonds(lat):
+        dx, dy = (1, 0) if leg == 2 else (0, 1)
+        two.append({"name": "bond_hamiltonian", "group": 0, "source": s, "dx": dx, "dy": dy, "op": ham})
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    E = np.real(sum(w[0].values())) / 4
+    return E, np.real(np.array([o[0][i] for i in range(4)]))
+
+
+def test_environment_contractions_same_env(gpu_ctx):
+    """One-site, two-site op12, 1x2/2x1 clusters and the transfer matrix on
+    an identical (uploaded) environment: entry-level agreement."""
+    Tn, lat, env, ham = oracle_env("heisenberg", 3, 12, 60, 4)
+    genv = upload_env(gpu_ctx, Tn, env)
+    E, mz = observables_oracle(env, Tn, lat, ham)
+    gE, gmz = observables_gpu(genv, lat, ham)
+    assert abs(gE - E) < 1e-12 * abs(E)
+    assert np.max(np.abs(gmz - mz)) < 1e-12
+    # product operators on a vertical bond through the 2x1 cluster
+    v = genv.contract_cluster(0, 0, 1, models.Sz, models.Sz)
+    nrow, ncol, sr, sc, tr, tc, idx = itps.cluster(lat, 0, 0, 1)
+    C, eTt, eTr, eTb, eTl = itps._cluster_args(env, idx, nrow, ncol)
+    ops = [[np.eye(2)] for _ in range(2)]
+    ops[sr][sc] = models.Sz
+    ops[tr][tc] = models.Sz
+    ref = K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, [[Tn[idx[r][0]]] for r in range(2)], ops)
+    assert abs(v - ref.real) < 1e-12 * abs(ref)
+    # transfer matrix: dense (chi^2 = 144 <= 200) eigenvalues
+    tm = TMo.TransferMatrixCTM(lat, Tn, env)
+    for d in (0, 1):
+        ref = tm.eigenvalues(d, 0, TMo.TMParameters())
+        got = genv.transfer_matrix_eigenvalues(d, 0)
+        assert rel(np.abs(got), np.abs(ref)) < 1e-11
+
+
+def test_transfer_matrix_arnoldi(gpu_ctx):
+    """chi^2 > 200: the Arnoldi path, compared with the oracle's Arnoldi and
+    with a dense eigendecomposition of the whole transfer matrix."""
+    Tn, lat, env, _ = oracle_env("heisenberg", 2, 20, 60, 6)
+    genv = upload_env(gpu_ctx, Tn, env)
+    tm = TMo.TransferMatrixCTM(lat, Tn, env)
+    prm = TMo.TMParameters(num_eigvals=4)
+    for d in (0, 1):
+        ref = tm.eigenvalues(d, 0, prm)
+        dense = TMo.eigen(tm.matrix(d, 0), 4)[0]
+        got = genv.transfer_matrix_eigenvalues(d, 0, ta.tm_parameters(num_eigvals=4))
+        assert rel(np.abs(got), np.abs(ref)) < 1e-9
+        # leading eigenvalue is converged to the dense answer
+        assert abs(abs(got[0]) - abs(dense[0])) < 1e-10 * abs(dense[0])
+
+
+@pytest.mark.parametrize("model,D,chi,sweeps", [("heisenberg", 2, 10, 6), ("tfi", 2, 16, 5),
+                                                ("heisenberg", 3, 20, 4)])
+def test_ctm_environment_observables(gpu_ctx, model, D, chi, sweeps):
+    """Calc_CTM_Environment from the same site tensors: energies, <Sz> and
+    normalised corner spectra agree with the oracle."""
+    Tn, lat, env, ham = oracle_env(model, D, chi, 60, sweeps)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=chi, Max_CTM_Iteration=sweeps,
+                                                        CTM_Convergence_Epsilon=0.0))
+    assert n == sweeps
+    E, mz = observables_oracle(env, Tn, lat, ham)
+    gE, gmz = observables_gpu(genv, lat, ham)
+    assert abs(gE - E) < 1e-10 * abs(E)
+    assert np.max(np.abs(gmz - mz)) < 1e-10
+    for i in range(4):
+        for k in ("C1", "C2", "C3", "C4"):
+            ref = O.normalized_spectrum(getattr(env, k)[i])
+            got = genv.corner_spectrum(k, i)
+            m = min(ref.size, got.size)
+            assert np.max(np.abs(got[:m] - ref[:m])) < 1e-10
+
+
+def test_product_state_closed_form(gpu_ctx):
+    """Embedded product state (nonzero only at virtual index 0): the
+    environment is exact at any D and chi, <S> and the bond energy take their
+    product-state values (D=5, chi=100 shapes)."""
+    v = np.array([np.cos(0.3), np.sin(0.3)])
+    Tn = []
+    for _ in range(4):
+        T = np.zeros((5, 5, 5, 5, 2))
+        T[0, 0, 0, 0] = v
+        Tn.append(T)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=100, Max_CTM_Iteration=3, CTM_Convergence_Epsilon=0.0))
+    ham = models.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    E, mz = observables_gpu(genv, SquareLattice(2, 2), ham)
+    sz = v @ models.Sz @ v
+    sx = v @ models.Sx @ v
+    assert abs(mz[0] - sz) < 1e-13
+    # S.S on |v v>: <Sz>^2 + <Sx>^2 (+ <Sy>^2 = 0 for a real v), 2 bonds/site
+    assert abs(E - 2 * (sz * sz + sx * sx)) < 1e-13
+
+
+def test_random_tensors_spectra(gpu_ctx):
+    """Random unit-norm tensors (D=3, chi=27): a few full sweeps, corner
+    spectra against the oracle."""
+    Tn = random_tensors(3, seed=21)
+    lat = SquareLattice(2, 2)
+    chi = 27
+    p = O.PEPSParameters(CHI=chi, Max_CTM_Iteration=3, CTM_Convergence_Epsilon=0.0)
+    env, _, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=chi, Max_CTM_Iteration=3, CTM_Convergence_Epsilon=0.0))
+    worst = 0.0
+    for i in range(4):
+        for k in ("C1", "C2", "C3", "C4"):
+            ref = O.normalized_spectrum(getattr(env, k)[i])
+            got = genv.corner_spectrum(k, i)
+            worst = max(worst, np.max(np.abs(got - ref)))
+    assert worst < 1e-10

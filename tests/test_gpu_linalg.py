@@ -1,0 +1,112 @@
+"""GPU kernels against textbook references: the gather-GEMM behind every
+tensordot (numpy tensordot/einsum in float64), transposes, and the SVD
+(numpy's LAPACK SVD; reconstruction and orthogonality)."""
+import numpy as np
+import pytest
+
+import tenes_amd as ta
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 4), (1, 1, 1), (5, 7, 3), (64, 64, 64), (100, 37, 250),
+                                   (300, 260, 129), (1024, 1024, 64), (3, 5, 5000), (250000 // 100, 50, 25)])
+def test_matmul_shapes(gpu_ctx, M, N, K):
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C = ta.tensordot(gpu_ctx.tensor(A), gpu_ctx.tensor(B), [1], [0]).numpy()
+    assert C.shape == (M, N)
+    assert _rel(C, A @ B) < RTOL
+
+
+@pytest.mark.parametrize("case", [
+    # (shapeA, shapeB, axA, axB, perm)
+    ((4, 5, 6), (6, 5, 3), [1, 2], [1, 0], None),
+    ((7, 3, 4, 2), (2, 4, 9), [2, 3], [1, 0], [2, 0, 1]),
+    ((10, 10, 5, 5), (10, 5, 5, 11), [1, 2], [0, 1], [3, 1, 0, 2, 4]),
+    ((3, 4), (5, 6), [], [], None),
+    ((20, 20, 4, 4), (20, 4, 4, 20), [0, 2, 3], [0, 1, 2], [1, 0]),
+    ((2, 3, 2, 3, 2), (2, 3, 2, 3, 2), [0, 1, 4], [0, 1, 4], [1, 3, 0, 2]),
+])
+def test_tensordot_permuted(gpu_ctx, case):
+    sa, sb, axa, axb, perm = case
+    rng = np.random.default_rng(len(sa) * 7 + len(sb))
+    A = rng.standard_normal(sa)
+    B = rng.standard_normal(sb)
+    ref = np.tensordot(A, B, axes=(axa, axb))
+    if perm is not None:
+        ref = ref.transpose(perm)
+    got = ta.tensordot(gpu_ctx.tensor(A), gpu_ctx.tensor(B), axa, axb, perm).numpy()
+    assert got.shape == ref.shape
+    assert _rel(got, ref) < RTOL
+
+
+def test_transpose(gpu_ctx):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((3, 4, 5, 6, 2))
+    for axes in [(4, 3, 2, 1, 0), (2, 3, 0, 1, 4), (1, 2, 3, 0, 4), (3, 0, 1, 2, 4)]:
+        got = ta.transpose(gpu_ctx.tensor(A), axes).numpy()
+        assert np.array_equal(got, A.transpose(axes))
+
+
+def _check_svd(A, U, s, V, k, srel):
+    s_ref = np.linalg.svd(A, compute_uv=False)[:k]
+    assert np.max(np.abs(s - s_ref)) / s_ref[0] < srel
+    # orthonormal columns and A V = U S on the kept triplets
+    assert np.max(np.abs(U.T @ U - np.eye(k))) < 1e-10
+    assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-10
+    assert np.max(np.abs(A @ V - U * s)) / s[0] < 1e-10
+
+
+@pytest.mark.parametrize("m,n", [(7, 7), (40, 40), (128, 128), (100, 60), (60, 100), (1, 5)])
+def test_svd_direct_jacobi(gpu_ctx, m, n):
+    rng = np.random.default_rng(m + 3 * n)
+    A = rng.standard_normal((m, n))
+    k = min(m, n)
+    U, s, V = ta.svd_top(gpu_ctx.tensor(A), k)
+    _check_svd(A, U.numpy(), s, V.numpy(), k, 1e-13)
+
+
+@pytest.mark.parametrize("m,n", [(1000, 80), (80, 1000)])
+def test_svd_qr_reduction(gpu_ctx, m, n):
+    rng = np.random.default_rng(11)
+    # graded columns: singular values spanning 12 decades
+    Q1, _ = np.linalg.qr(rng.standard_normal((max(m, n), min(m, n))))
+    Q2, _ = np.linalg.qr(rng.standard_normal((min(m, n), min(m, n))))
+    sv = np.logspace(0, -12, min(m, n))
+    A = (Q1 * sv) @ Q2.T
+    if m < n:
+        A = A.T
+    k = 60
+    U, s, V = ta.svd_top(gpu_ctx.tensor(A), k)
+    _check_svd(A, U.numpy(), s, V.numpy(), k, 1e-13)
+
+
+def test_svd_randomized_decaying(gpu_ctx):
+    rng = np.random.default_rng(5)
+    n = 1500
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, 400)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, 400)))
+    sv = np.exp(-np.arange(400) / 12.0)
+    A = (Q1 * sv) @ Q2.T
+    k = 100
+    U, s, V = ta.svd_top(gpu_ctx.tensor(A), k, power_iters=4)
+    _check_svd(A, U.numpy(), s, V.numpy(), k, 1e-12)
+
+
+def test_svd_rank_deficient(gpu_ctx):
+    rng = np.random.default_rng(9)
+    n = 600
+    X = rng.standard_normal((n, 5))
+    A = X @ rng.standard_normal((5, n))
+    U, s, V = ta.svd_top(gpu_ctx.tensor(A), 20)
+    s_ref = np.linalg.svd(A, compute_uv=False)[:20]
+    assert np.max(np.abs(s[:5] - s_ref[:5])) / s_ref[0] < 1e-12
+    assert np.all(s[5:] < 1e-12 * s[0])

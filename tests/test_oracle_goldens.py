@@ -1,0 +1,142 @@
+"""Pin the oracle to TeNeS's own golden data and known answers (CPU only).
+
+* TE_TFI (test/data/TE_TFI.toml, output_TE_TFI/): complex real-time simple
+  update + CTM (chi=10, <=10 sweeps) + one-/two-site observables, energy and
+  correlation length at t = 0, 0.2, ..., 1.0.
+* simple_update.dat (test/simple_update.cpp): one Simple_update_bond.
+* std::mt19937 / uniform_real_distribution against a g++-compiled probe and
+  the C++ standard's check value.
+"""
+import shutil
+import subprocess
+import textwrap
+
+import numpy as np
+import pytest
+
+import golden_io as G
+from oracle import ctm, itps, simple_update as SU, transfer as TM
+from oracle.lattice import SquareLattice
+from oracle.rng import MT19937, UniformReal
+
+
+def test_mt19937_standard_check_value():
+    # [rand.predef]: the 10000th output of a default-constructed mt19937
+    g = MT19937(5489)
+    assert int(g.raw(10000)[-1]) == 4123659995
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_uniform_real_matches_libstdcxx(tmp_path):
+    src = tmp_path / "probe.cpp"
+    src.write_text(textwrap.dedent("""
+        #include <cstdio>
+        #include <random>
+        int main() {
+          std::mt19937 gen(11);
+          std::uniform_real_distribution<double> dist(-1.0, 1.0);
+          for (int i = 0; i < 64; ++i) std::printf("%.17g\\n", dist(gen));
+          std::mt19937 g2(11 * 11 + 137);
+          for (int i = 0; i < 8; ++i) std::printf("%.17g\\n", dist(g2));
+        }
+    """))
+    exe = tmp_path / "probe"
+    subprocess.run(["g++", "-O1", str(src), "-o", str(exe)], check=True)
+    ref = np.array([float(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                     check=True).stdout.split()])
+    got = np.concatenate([UniformReal()(MT19937(11), 64), UniformReal()(MT19937(11 * 11 + 137), 8)])
+    assert np.array_equal(got, ref)
+
+
+def _te_tfi_setup():
+    fx = G.load("TE_TFI")
+    LX, LY, skew, pdims, vdims, dirs, noises = G.unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=False)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    return fx, lat, pdims, Tn, lam
+
+
+def test_te_tfi_goldens():
+    fx, lat, pdims, Tn, lam = _te_tfi_setup()
+    ev = G.evolution(fx, False)
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, False)
+    p = ctm.PEPSParameters(CHI=fx["parameter"]["ctm"]["dimension"],
+                           Max_CTM_Iteration=fx["parameter"]["ctm"]["iteration_max"])
+    gold = G.density_table(fx["outputs"]["density"], timed=True)
+    interval = fx["parameter"]["general"]["measure_interval"]
+    tau = fx["parameter"]["simple_update"]["tau"]
+    nsteps = fx["parameter"]["simple_update"]["num_step"]
+    names = ["Energy"] + [onames[g] for g in sorted(onames)] + [tnames[g] for g in sorted(tnames)]
+    t = 0.0
+    checked = 0
+    for block in range(nsteps // interval + 1):
+        if block:
+            Tn, lam = SU.simple_update_steps(Tn, lam, lat, ev, interval)
+            t = round(t + interval * tau, 10)
+        env, count, _ = ctm.Calc_CTM_Environment(Tn, p, lat)
+        o, _ = itps.measure_onesite(env, Tn, lat, one)
+        w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+        dens = itps.density(o, w, lat, pdims, onames, tnames)
+        for i, nm in enumerate(names):
+            ref = gold[(t, i)]
+            assert abs(dens[nm] - ref) < 1e-12, (t, nm, dens[nm], ref)
+            checked += 1
+    assert checked == 6 * len(names)
+    # correlation length at the final time (dense transfer matrix, chi^2 = 100)
+    rows = {}
+    for r in fx["outputs"]["correlation_length"]:
+        f = r.split()
+        rows[(round(float(f[0]), 10), int(f[1]), int(f[2]))] = [float(x) for x in f[3:]]
+    for d, fixed, xi, es, _ in itps.correlation_lengths(env, Tn, lat, TM.TMParameters()):
+        ref = rows[(t, d, fixed)]
+        assert abs(xi - ref[0]) < 1e-10 * abs(ref[0])
+        assert np.max(np.abs(np.array(es) - np.array(ref[1:]))) < 1e-9
+
+
+def test_simple_update_bond_golden():
+    """test/simple_update.cpp: all-ones D=2 tensors, gate op(i,i,i,i)=1,
+    connect=2; compared up to a sign per (i,j,k,l) as the reference test does,
+    relative tolerance 1e-8."""
+    D, d = 2, 2
+    T = [np.ones((D, D, D, D, d)) for _ in range(2)]
+    lam = [[np.ones(D) for _ in range(4)] for _ in range(2)]
+    op = np.zeros((d, d, d, d))
+    for i in range(d):
+        op[i, i, i, i] = 1.0
+    T1, T2, lc = SU.Simple_update_bond(T[0], T[1], lam[0], lam[1], op, 2)
+    vals = [float(x) for row in G.load("simple_update") for x in row]
+    ans = [np.array(vals[:32]).reshape(D, D, D, D, d), np.array(vals[32:64]).reshape(D, D, D, D, d)]
+    ans_l = np.array(vals[64:66])
+    for res, a in zip((T1, T2), ans):
+        for idx in np.ndindex(D, D, D, D):
+            r, x = res[idx], a[idx]
+            nz = np.nonzero(r)[0]
+            sign = 1.0 if nz.size == 0 or r[nz[0]] * x[nz[0]] > 0 else -1.0
+            assert np.allclose(sign * r, x, rtol=1e-8, atol=1e-12)
+    assert np.allclose(np.abs(lc), ans_l, rtol=1e-8)
+    assert abs(ans_l[0] - 2 ** -0.25) < 1e-14
+
+
+def test_product_state_ctm_closed_form():
+    """Embedded product state: CTM is exact, <Sz> and the Heisenberg bond
+    energy take their product-state values at any D and chi."""
+    from oracle import models
+
+    lat = SquareLattice(2, 2)
+    v = np.array([np.cos(0.4), np.sin(0.4)])
+    Tn = []
+    for _ in range(4):
+        T = np.zeros((3, 3, 3, 3, 2))
+        T[0, 0, 0, 0] = v
+        Tn.append(T)
+    env, _, _ = ctm.Calc_CTM_Environment(Tn, ctm.PEPSParameters(CHI=12, Max_CTM_Iteration=3), lat)
+    ham = models.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    one = [{"name": "Sz", "group": 0, "site": i, "op": models.Sz} for i in range(4)]
+    two = [{"name": "b", "group": 0, "source": s, "dx": 1 if leg == 2 else 0, "dy": 1 if leg == 1 else 0,
+            "op": ham} for s, leg in models.square_bonds(lat)]
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    sz, sx = v @ models.Sz @ v, v @ models.Sx @ v
+    assert abs(o[0][0] - sz) < 1e-14
+    assert abs(np.real(sum(w[0].values())) / 4 - 2 * (sz * sz + sx * sx)) < 1e-14
