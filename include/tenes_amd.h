@@ -56,6 +56,12 @@ int tenes_ctx_set_seed(tenes_ctx *ctx, uint64_t seed);
 /* Accumulated GEMM work issued through this context (2*M*N*K per GEMM). */
 int tenes_ctx_stats(tenes_ctx *ctx, double *gemm_flops, int64_t *gemm_calls);
 int tenes_ctx_reset_stats(tenes_ctx *ctx);
+/* Per-launch GEMM timing with HIP events on the context stream (on != 0).
+ * tenes_ctx_kernel_report writes one line per kernel instantiation,
+ * "name launches total_ms total_flops", into buf (NUL terminated). */
+int tenes_ctx_set_profiling(tenes_ctx *ctx, int on);
+int tenes_ctx_kernel_report(tenes_ctx *ctx, char *buf, int buflen);
+int tenes_ctx_reset_kernel_report(tenes_ctx *ctx);
 
 /* ---- tensors (mptensor::Tensor replacement) ------------------------------ */
 int tenes_tensor_from_host(tenes_ctx *ctx, int rank, const int64_t *shape,
@@ -184,6 +190,27 @@ int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
 int tenes_create_environment_two_sites(const tenes_tensor *const *T,
                                        tenes_tensor **out);
 /* T[12] = {C1,C2,C3,C4, eT1..eT6, Q1, Q2} */
+
+/* ---- simple update (input preparation for the CTM path) ------------------ */
+/* Simple_update_bond, core/simple_update.cpp:50-198.  lambda1/lambda2: the
+ * four bond-weight vectors of each site concatenated in leg order (lengths
+ * = the site tensor's virtual dimensions).  lambda_c: Tn1.shape[connect1]
+ * values. */
+int tenes_simple_update_bond(const tenes_tensor *Tn1, const tenes_tensor *Tn2,
+                             const double *lambda1, const double *lambda2,
+                             const tenes_tensor *op12, int connect1,
+                             double inverse_lambda_cut, tenes_tensor **Tn1_new,
+                             tenes_tensor **Tn2_new, double *lambda_c);
+/* simple_update.cpp:103-138: nsteps passes over the two-site evolution
+ * operators (source site, source leg, op (d1,d2,d1,d2)) applied to the
+ * environment's site tensors; bond weights start at 1 and persist in env. */
+int tenes_env_simple_update(tenes_env *env, int nops, const int *source_sites,
+                            const int *source_legs,
+                            const tenes_tensor *const *ops, int nsteps,
+                            double inverse_lambda_cut);
+/* Bond weights of (site, leg); *n_out receives the length. */
+int tenes_env_get_lambda(tenes_env *env, int site, int leg, double *out,
+                         int *n_out);
 
 /* ---- transfer matrix (transfer_matrix.hpp:49-139, arnoldi.hpp:32-71) ---- */
 typedef struct tenes_tm_parameters {

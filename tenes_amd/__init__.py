@@ -11,6 +11,7 @@ from .core import (  # noqa: F401
     calc_projector_updown_blocks,
     create_environment_two_sites,
     peps_parameters,
+    simple_update_bond,
     svd_top,
     tensordot,
     tm_parameters,
@@ -20,6 +21,6 @@ from .core import (  # noqa: F401
 __all__ = [
     "Context", "Environment", "Tensor", "TenesError", "calc_next_ctm", "calc_next_et",
     "calc_projector_left_block", "calc_projector_updown_blocks",
-    "create_environment_two_sites", "peps_parameters", "svd_top", "tensordot",
+    "create_environment_two_sites", "peps_parameters", "simple_update_bond", "svd_top", "tensordot",
     "tm_parameters", "transpose",
 ]

@@ -54,6 +54,9 @@ _SIGS = {
     "tenes_ctx_set_seed": (C.c_int, [C.c_void_p, C.c_uint64]),
     "tenes_ctx_stats": (C.c_int, [C.c_void_p, c_dbl_p, c_i64_p]),
     "tenes_ctx_reset_stats": (C.c_int, [C.c_void_p]),
+    "tenes_ctx_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "tenes_ctx_kernel_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "tenes_ctx_reset_kernel_report": (C.c_int, [C.c_void_p]),
     "tenes_tensor_from_host": (C.c_int, [C.c_void_p, C.c_int, c_i64_p, c_dbl_p, c_void_pp]),
     "tenes_tensor_from_device": (C.c_int, [C.c_void_p, C.c_int, c_i64_p, C.c_void_p, c_void_pp]),
     "tenes_tensor_zeros": (C.c_int, [C.c_void_p, C.c_int, c_i64_p, c_void_pp]),
@@ -91,6 +94,12 @@ _SIGS = {
     "tenes_env_contract_cluster": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p]),
     "tenes_create_environment_two_sites": (C.c_int, [C.POINTER(C.c_void_p), c_void_pp]),
+    "tenes_simple_update_bond": (
+        C.c_int, [C.c_void_p, C.c_void_p, c_dbl_p, c_dbl_p, C.c_void_p, C.c_int, C.c_double,
+                  c_void_pp, c_void_pp, c_dbl_p]),
+    "tenes_env_simple_update": (
+        C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.POINTER(C.c_void_p), C.c_int, C.c_double]),
+    "tenes_env_get_lambda": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, c_int_p]),
     "tenes_tm_parameters_default": (None, [C.POINTER(TmParameters)]),
     "tenes_env_transfer_matrix_eigenvalues": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(TmParameters), c_dbl_p, c_int_p]),

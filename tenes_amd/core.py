@@ -50,6 +50,22 @@ class Context:
     def reset_stats(self):
         self.check(self.lib.tenes_ctx_reset_stats(self.h))
 
+    def set_profiling(self, on: bool):
+        self.check(self.lib.tenes_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_report(self) -> dict:
+        """{kernel name: (launches, total_ms, total_flops)} of profiled GEMMs."""
+        buf = C.create_string_buffer(1 << 16)
+        self.check(self.lib.tenes_ctx_kernel_report(self.h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms, fl = line.rsplit(" ", 3)
+            out[name] = (int(n), float(ms), float(fl))
+        return out
+
+    def reset_kernel_report(self):
+        self.check(self.lib.tenes_ctx_reset_kernel_report(self.h))
+
     def tensor(self, a: np.ndarray) -> "Tensor":
         a = np.ascontiguousarray(a, dtype=np.float64)
         shape = np.array(a.shape, dtype=np.int64)
@@ -194,6 +210,20 @@ def calc_next_et(eT8, Tn1, PU, PL):
     return Tensor(ctx, a)
 
 
+def simple_update_bond(Tn1: Tensor, Tn2: Tensor, lambda1, lambda2, op12: Tensor, connect1: int,
+                       inverse_lambda_cut: float = 1e-12):
+    """lambda1/lambda2: lists of the four bond-weight vectors."""
+    ctx = Tn1.ctx
+    l1 = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in lambda1]))
+    l2 = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in lambda2]))
+    a = C.c_void_p()
+    b = C.c_void_p()
+    lc = np.zeros(Tn1.shape[connect1], dtype=np.float64)
+    ctx.check(ctx.lib.tenes_simple_update_bond(Tn1.h, Tn2.h, L.dptr(l1), L.dptr(l2), op12.h, connect1,
+                                               inverse_lambda_cut, C.byref(a), C.byref(b), L.dptr(lc)))
+    return Tensor(ctx, a), Tensor(ctx, b), lc
+
+
 def create_environment_two_sites(tensors: Sequence[Tensor]) -> Tensor:
     ctx = tensors[0].ctx
     arr = (C.c_void_p * 12)(*[t.h for t in tensors])
@@ -285,6 +315,24 @@ class Environment:
             self.h, source, dx, dy, L.dptr(a) if a is not None else None,
             L.dptr(b) if b is not None else None, C.byref(out)))
         return out.value
+
+    def simple_update(self, ops, nsteps: int, inverse_lambda_cut: float = 1e-12):
+        """ops: [(source_site, source_leg, op (d1, d2, d1, d2)), ...]"""
+        src = np.array([o[0] for o in ops], dtype=np.int32)
+        leg = np.array([o[1] for o in ops], dtype=np.int32)
+        tens = [o[2] if isinstance(o[2], Tensor) else self.ctx.tensor(o[2]) for o in ops]
+        arr = (C.c_void_p * max(len(tens), 1))(*[t.h for t in tens])
+        self.ctx.check(self.ctx.lib.tenes_env_simple_update(
+            self.h, len(ops), L.iptr(src), L.iptr(leg), arr, nsteps, inverse_lambda_cut))
+
+    def get_lambda(self, site: int, leg: int) -> np.ndarray:
+        out = np.zeros(4096, dtype=np.float64)
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.tenes_env_get_lambda(self.h, site, leg, L.dptr(out), C.byref(n)))
+        return out[: n.value].copy()
+
+    def site_tensors(self) -> list:
+        return [self.get("Tn", i).numpy() for i in range(self.N)]
 
     def transfer_matrix_eigenvalues(self, direction: int, fixed: int, params=None) -> np.ndarray:
         params = params or tm_parameters()

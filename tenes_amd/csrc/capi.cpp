@@ -1,5 +1,6 @@
 // extern "C" boundary: handles around tn::Context / tn::Tensor and the
 // environment arrays, exception-free, one status code per call.
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -9,6 +10,7 @@
 #include "contract.hpp"
 #include "ctm.hpp"
 #include "linalg.hpp"
+#include "simple_update.hpp"
 #include "transfer.hpp"
 
 struct tenes_ctx {
@@ -26,6 +28,7 @@ struct tenes_env {
   tenes::itps::SquareLattice lat;
   std::vector<tn::Tensor> arr[9];  // C1..C4, eTt, eTr, eTb, eTl, Tn
   std::vector<tn::Tensor> saved[4];
+  std::vector<std::vector<std::vector<double>>> lambda;  // [site][leg]
 };
 
 namespace {
@@ -191,6 +194,30 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx) {
     ctx->c->gemm_flops = 0.0;
     ctx->c->gemm_calls = 0;
   });
+}
+
+int tenes_ctx_set_profiling(tenes_ctx *ctx, int on) {
+  return guard(ctx, [&] { ctx->c->set_profiling(on != 0); });
+}
+
+int tenes_ctx_kernel_report(tenes_ctx *ctx, char *buf, int buflen) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(buf && buflen > 0, "null buffer");
+    std::string out;
+    char line[512];
+    for (auto &kv : ctx->c->kernel_stats()) {
+      std::snprintf(line, sizeof(line), "%s %lld %.9g %.9g\n", kv.first.c_str(),
+                    (long long)kv.second.launches, kv.second.ms,
+                    kv.second.flops);
+      out += line;
+    }
+    TN_REQUIRE((int)out.size() < buflen, "report buffer too small");
+    std::memcpy(buf, out.c_str(), out.size() + 1);
+  });
+}
+
+int tenes_ctx_reset_kernel_report(tenes_ctx *ctx) {
+  return guard(ctx, [&] { ctx->c->reset_kernel_stats(); });
 }
 
 int tenes_tensor_from_host(tenes_ctx *ctx, int rank, const int64_t *shape,
@@ -606,6 +633,83 @@ int tenes_create_environment_two_sites(const tenes_tensor *const *Ts,
                          T(Ts[0]), T(Ts[1]), T(Ts[2]), T(Ts[3]), T(Ts[4]),
                          T(Ts[5]), T(Ts[6]), T(Ts[7]), T(Ts[8]), T(Ts[9]),
                          T(Ts[10]), T(Ts[11])));
+  });
+}
+
+int tenes_simple_update_bond(const tenes_tensor *Tn1, const tenes_tensor *Tn2,
+                             const double *lambda1, const double *lambda2,
+                             const tenes_tensor *op12, int connect1,
+                             double inverse_lambda_cut, tenes_tensor **Tn1_new,
+                             tenes_tensor **Tn2_new, double *lambda_c) {
+  if (!Tn1) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(Tn1);
+  return guard(ctx, [&] {
+    TN_REQUIRE(lambda1 && lambda2 && Tn1_new && Tn2_new && lambda_c,
+               "null pointer");
+    TN_REQUIRE(T(Tn1).rank() == 5 && T(Tn2).rank() == 5,
+               "site tensors must have 5 legs");
+    std::vector<std::vector<double>> l1(4), l2(4);
+    int64_t o1 = 0, o2 = 0;
+    for (int l = 0; l < 4; ++l) {
+      l1[l].assign(lambda1 + o1, lambda1 + o1 + T(Tn1).shape[l]);
+      o1 += T(Tn1).shape[l];
+      l2[l].assign(lambda2 + o2, lambda2 + o2 + T(Tn2).shape[l]);
+      o2 += T(Tn2).shape[l];
+    }
+    tn::Tensor a, b;
+    std::vector<double> lc;
+    tenes::itps::core::Simple_update_bond(T(Tn1), T(Tn2), l1, l2, T(op12),
+                                          connect1, inverse_lambda_cut, a, b,
+                                          lc);
+    *Tn1_new = wrap(ctx, a);
+    *Tn2_new = wrap(ctx, b);
+    std::memcpy(lambda_c, lc.data(), lc.size() * sizeof(double));
+  });
+}
+
+static void ensure_lambda(tenes_env *env) {
+  const int N = env->lat.N_UNIT;
+  if ((int)env->lambda.size() == N) return;
+  env->lambda.assign(N, std::vector<std::vector<double>>(4));
+  for (int i = 0; i < N; ++i) {
+    TN_REQUIRE(env->arr[TENES_TN][i].defined(), "site tensor not set");
+    for (int l = 0; l < 4; ++l)
+      env->lambda[i][l].assign(env->arr[TENES_TN][i].shape[l], 1.0);
+  }
+}
+
+int tenes_env_simple_update(tenes_env *env, int nops, const int *source_sites,
+                            const int *source_legs,
+                            const tenes_tensor *const *ops, int nsteps,
+                            double inverse_lambda_cut) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    TN_REQUIRE(nops >= 0 && (nops == 0 || (source_sites && source_legs && ops)),
+               "null pointer");
+    ensure_lambda(env);
+    std::vector<tenes::itps::core::EvolutionOp> up;
+    for (int k = 0; k < nops; ++k) {
+      TN_REQUIRE(source_sites[k] >= 0 && source_sites[k] < env->lat.N_UNIT,
+                 "bad source site");
+      TN_REQUIRE(source_legs[k] >= 0 && source_legs[k] < 4, "bad source leg");
+      up.push_back({source_sites[k], source_legs[k], T(ops[k])});
+    }
+    tenes::itps::core::Simple_update_steps(env->arr[TENES_TN], env->lambda,
+                                           env->lat, up, nsteps,
+                                           inverse_lambda_cut);
+  });
+}
+
+int tenes_env_get_lambda(tenes_env *env, int site, int leg, double *out,
+                         int *n_out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    TN_REQUIRE(site >= 0 && site < env->lat.N_UNIT && leg >= 0 && leg < 4,
+               "bad site or leg");
+    ensure_lambda(env);
+    const auto &v = env->lambda[site][leg];
+    if (out) std::memcpy(out, v.data(), v.size() * sizeof(double));
+    if (n_out) *n_out = (int)v.size();
   });
 }
 

@@ -39,8 +39,9 @@ struct GemmArgs {
 
 // Scratch needed for split-K on the given shape (bytes, 0 if none).
 size_t gemm_workspace_bytes(int M, int N, int K);
-void gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
-              hipStream_t s);
+// Returns the name of the kernel instantiation launched (static string).
+const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
+                     hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Elementwise / reduction helpers
@@ -86,12 +87,14 @@ void splitk_reduce(const double *P, int splits, int M, int N, double alpha,
 // Counter-based standard normal numbers (SplitMix64 hash + Box-Muller).
 void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
            hipStream_t s);
-// In-place shifted Cholesky of a symmetric p x p matrix G (row-major,
-// p <= 128) using one workgroup: G + shift*I = R^T R, R upper triangular.
-// Also writes Rinv = R^{-1} (upper triangular).  shift is computed on the
-// device as shift_coef * trace(G).
-void chol_inv_upper(const double *G, int p, double shift_coef, double *R,
-                    double *Rinv, hipStream_t s);
+// Shifted Cholesky of a symmetric p x p matrix G (row-major, p <= 128)
+// using one workgroup: G + shift*I = R^T R, R upper triangular, shift =
+// shift_coef * trace(G) (computed on the device).  Also writes Rinv.
+// With rel_floor > 0, a pivot <= rel_floor * max diagonal drops its column
+// (rank-deficient input): that row of R is zero and Rinv has a unit pivot,
+// so Q = Y * Rinv keeps the (negligible) residual of the dropped column.
+void chol_inv_upper(const double *G, int p, double shift_coef,
+                    double rel_floor, double *R, double *Rinv, hipStream_t s);
 // One-sided (Hestenes) Jacobi SVD of a batch of p x p matrices with
 // rows = m (m, p <= 128), held in LDS by one workgroup per matrix.
 // W is column-major per matrix (col j at W + j*m); on exit its columns are

@@ -10,8 +10,9 @@
 // Tiling: 256 threads = 4 waves (2 x 2), block tile BM x BN x 16, each wave
 // owns (BM/2) x (BN/2) = (BM/32) x (BN/32) MFMA 16x16 accumulator tiles.
 // K tiles are staged through LDS with a register prefetch of the next tile.
-// FP64 MFMA on gfx950 issues one 16x16x4 block per 16 cycles per SIMD, so the
-// LDS traffic per MFMA (one double per lane per operand) is far below the LDS
+// An FP64 16x16x4 MFMA does 2048 flop; at the 78.6 TFLOP/s FP64 matrix peak
+// (256 CUs x 4 SIMDs) that is ~64 cycles per instruction per SIMD, so the LDS
+// traffic per MFMA (one double per lane per operand) is far below the LDS
 // bandwidth; the design goal is to keep the MFMA pipe fed, and to have enough
 // workgroups (>= 256 CUs) in flight, splitting K when M x N is small.
 #include "common.hpp"
@@ -157,7 +158,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   }
 
   // ---- epilogue -------------------------------------------------------------
-  // MFMA 16x16 output layout: lane l, register r -> row 4*(l/16)+r, col l%16.
+  // v_mfma_f64_16x16x4 output layout (measured on gfx950): lane l, register
+  // r -> row (l/16) + 4*r, column l%16.
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
       if (n >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wrow + i * 16 + lk * 4 + r;
+        const int m = m0 + wrow + i * 16 + lk + 4 * r;
         if (m >= M) continue;
         const double v = acc[i][j][r];
         if (ka.partial) {
@@ -227,11 +229,15 @@ Plan choose(int M, int N, int K) {
 }
 
 template <int BM, int BN>
-void launch_cfg(const KArgs &ka, dim3 grid, hipStream_t s, bool aff) {
+const char *launch_cfg(const KArgs &ka, dim3 grid, hipStream_t s, bool aff) {
   const bool am = ka.g.a_mfast, bn = ka.g.b_nfast;
 #define TN_LAUNCH(AM, BNF, AF)                                              \
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AM, BNF, AF>), grid, dim3(256), \
-                     0, s, ka)
+  do {                                                                      \
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AM, BNF, AF>), grid,        \
+                       dim3(256), 0, s, ka);                                \
+    return BM == 128 ? "gemm_f64_kernel<128,128," #AM "," #BNF "," #AF ">"   \
+                     : "gemm_f64_kernel<64,64," #AM "," #BNF "," #AF ">";     \
+  } while (0)
   if (aff) {
     if (am && bn) TN_LAUNCH(true, true, true);
     else if (am) TN_LAUNCH(true, false, true);
@@ -254,10 +260,10 @@ size_t gemm_workspace_bytes(int M, int N, int K) {
   return (size_t)p.splits * M * N * sizeof(double);
 }
 
-void gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
-              hipStream_t s) {
+const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
+                     hipStream_t s) {
   TN_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, "gemm: negative size");
-  if (g.M == 0 || g.N == 0) return;
+  if (g.M == 0 || g.N == 0) return "gemm_f64_empty";
   // K == 0 gives C = beta*C through a launch with an empty K range.
   Plan p = choose(g.M, g.N, g.K);
   KArgs ka;
@@ -278,10 +284,8 @@ void gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
   const bool aff = g.a_m_stride || g.a_k_stride || g.b_k_stride ||
                    g.b_n_stride;
   dim3 grid(ceil_div(g.N, p.BN), ceil_div(g.M, p.BM), splits);
-  if (p.BM == 128)
-    launch_cfg<128, 128>(ka, grid, s, aff);
-  else
-    launch_cfg<64, 64>(ka, grid, s, aff);
+  const char *name = (p.BM == 128) ? launch_cfg<128, 128>(ka, grid, s, aff)
+                                   : launch_cfg<64, 64>(ka, grid, s, aff);
   TN_HIP_CHECK(hipGetLastError());
   if (ka.partial) {
     const int64_t MN = (int64_t)g.M * g.N;
@@ -290,6 +294,7 @@ void gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
                        g.beta, g.C, g.c_m, g.c_n);
     TN_HIP_CHECK(hipGetLastError());
   }
+  return name;
 }
 
 void splitk_reduce(const double *P, int splits, int M, int N, double alpha,

@@ -52,58 +52,78 @@ __global__ void randn_kernel(double *p, int64_t n, uint64_t seed,
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void chol_inv_kernel(const double *G, int p,
                                                         double shift_coef,
+                                                        double rel_floor,
                                                         double *R,
                                                         double *Rinv) {
   extern __shared__ double A[];  // p*p, row-major
-  __shared__ double s_shift;
+  __shared__ double s_shift, s_thr;
   __shared__ int s_zero;
+  __shared__ int defic[kMaxDim];
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int i = tid; i < p * p; i += nt) A[i] = G[i];
   __syncthreads();
   if (tid == 0) {
-    double tr = 0.0;
-    for (int i = 0; i < p; ++i) tr += fabs(A[i * p + i]);
+    double tr = 0.0, mx = 0.0;
+    for (int i = 0; i < p; ++i) {
+      tr += fabs(A[i * p + i]);
+      mx = fmax(mx, A[i * p + i]);
+    }
     s_zero = !(tr > 0.0);
     s_shift = shift_coef * tr;
+    // pivots at or below rel_floor * max diagonal mark a column that is
+    // numerically in the span of the previous ones (rank-deficient block)
+    s_thr = rel_floor > 0.0 ? rel_floor * (mx + s_shift) : -1.0;
   }
   __syncthreads();
   if (s_zero) {
-    // Zero (or NaN) Gram matrix: the input block is zero; return R = Rinv = I
-    // so that Q = Y * Rinv stays zero instead of dividing by zero.
+    // Zero (or NaN) Gram matrix: the input block is zero.  R = 0 (the block
+    // has no range), Rinv = I so that Q = Y * Rinv stays zero.
     for (int i = tid; i < p * p; i += nt) {
       const double v = (i / p == i % p) ? 1.0 : 0.0;
-      R[i] = v;
+      R[i] = 0.0;
       Rinv[i] = v;
     }
     return;
   }
-  const double shift = s_shift;
+  const double shift = s_shift, thr = s_thr;
   for (int i = tid; i < p; i += nt) A[i * p + i] += shift;
   __syncthreads();
-  // Floor for pivots that rounding pushed to <= 0 (rank-deficient input).
+  // Floor for pivots that rounding pushed to <= 0 when no deficiency test is
+  // requested (first, shifted pass).
   const double floor_d = shift > 0.0 ? shift : DBL_MIN;
   for (int k = 0; k < p; ++k) {
     if (tid == 0) {
       double d = A[k * p + k];
-      if (!(d > floor_d)) d = floor_d;
-      A[k * p + k] = sqrt(d);
+      if (thr >= 0.0 && !(d > thr)) {
+        // drop column k: unit pivot, zero row, no trailing update
+        defic[k] = 1;
+        A[k * p + k] = 1.0;
+      } else {
+        defic[k] = 0;
+        if (!(d > floor_d)) d = floor_d;
+        A[k * p + k] = sqrt(d);
+      }
     }
     __syncthreads();
     const double rkk = A[k * p + k];
-    for (int j = k + 1 + tid; j < p; j += nt) A[k * p + j] /= rkk;
+    const bool dk = defic[k] != 0;
+    for (int j = k + 1 + tid; j < p; j += nt)
+      A[k * p + j] = dk ? 0.0 : A[k * p + j] / rkk;
     __syncthreads();
-    const int rem = p - k - 1;
-    for (int idx = tid; idx < rem * rem; idx += nt) {
-      const int i = k + 1 + idx / rem;
-      const int j = k + 1 + idx % rem;
-      if (j >= i) A[i * p + j] -= A[k * p + i] * A[k * p + j];
+    if (!dk) {
+      const int rem = p - k - 1;
+      for (int idx = tid; idx < rem * rem; idx += nt) {
+        const int i = k + 1 + idx / rem;
+        const int j = k + 1 + idx % rem;
+        if (j >= i) A[i * p + j] -= A[k * p + i] * A[k * p + j];
+      }
     }
     __syncthreads();
   }
-  // R (upper) to HBM, zeros below the diagonal.
+  // R (upper) to HBM, zeros below the diagonal and in dropped rows.
   for (int i = tid; i < p * p; i += nt) {
     const int r = i / p, c = i % p;
-    R[i] = (c >= r) ? A[i] : 0.0;
+    R[i] = (c >= r && !defic[r]) ? A[i] : 0.0;
   }
   __syncthreads();
   // In-place inversion of the upper factor, column by column (the leading
@@ -326,12 +346,12 @@ void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
   TN_HIP_CHECK(hipGetLastError());
 }
 
-void chol_inv_upper(const double *G, int p, double shift_coef, double *R,
-                    double *Rinv, hipStream_t s) {
+void chol_inv_upper(const double *G, int p, double shift_coef,
+                    double rel_floor, double *R, double *Rinv, hipStream_t s) {
   TN_REQUIRE(p >= 1 && p <= kMaxDim, "chol_inv_upper: p out of range");
   const size_t lds = (size_t)p * p * sizeof(double);
   hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(1024), lds, s, G, p,
-                     shift_coef, R, Rinv);
+                     shift_coef, rel_floor, R, Rinv);
   TN_HIP_CHECK(hipGetLastError());
 }
 

@@ -76,10 +76,14 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out) {
     Tensor G = tensordot(Y, Y, Axes{0}, Axes{0});  // p x p
     Tensor R = empty(ctx, Shape{p, p});
     Tensor Ri = empty(ctx, Shape{p, p});
+    // shifted CholeskyQR3: the first pass is shifted (stable for any
+    // condition number of Y), the next two are plain CholeskyQR with a
+    // relative pivot floor that drops numerically dependent columns.
     const double coef =
         pass == 0 ? 11.0 * (double)(m * p + p * (p + 1)) * u : 0.0;
-    kern::chol_inv_upper(G.data(), (int)p, coef, R.data(), Ri.data(),
-                         ctx.stream());
+    const double rel_floor = pass == 0 ? 0.0 : 1e-13;
+    kern::chol_inv_upper(G.data(), (int)p, coef, rel_floor, R.data(),
+                         Ri.data(), ctx.stream());
     Y = tensordot(Y, Ri, Axes{1}, Axes{0});
     if (R_out) Rtot = Rtot.defined() ? tensordot(R, Rtot, Axes{1}, Axes{0}) : R;
   }

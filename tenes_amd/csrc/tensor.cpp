@@ -93,6 +93,11 @@ Context::Context(int device, hipStream_t stream)
 
 Context::~Context() {
   (void)hipStreamSynchronize(stream_);
+  for (auto &p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : event_pool_) (void)hipEventDestroy(e);
   for (auto &kv : plans_) (void)hipFree(kv.second.d_tab);
   if (ws_) alloc_.free(ws_, ws_bytes_);
   if (scal_) (void)hipFree(scal_);
@@ -113,6 +118,60 @@ double *Context::scalars(int n) {
 }
 
 void Context::sync() { TN_HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+hipEvent_t Context::get_event() {
+  if (!event_pool_.empty()) {
+    hipEvent_t e = event_pool_.back();
+    event_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  TN_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void Context::set_profiling(bool on) {
+  if (!on) flush_pending();
+  profiling_ = on;
+}
+
+void Context::profile_begin(hipEvent_t *a) {
+  *a = get_event();
+  TN_HIP_CHECK(hipEventRecord(*a, stream_));
+}
+
+void Context::profile_end(hipEvent_t a, const char *name, double flops) {
+  hipEvent_t b = get_event();
+  TN_HIP_CHECK(hipEventRecord(b, stream_));
+  pending_.push_back({a, b, name, flops});
+  if (pending_.size() >= 4096) flush_pending();
+}
+
+void Context::flush_pending() {
+  if (pending_.empty()) return;
+  TN_HIP_CHECK(hipEventSynchronize(pending_.back().b));
+  for (auto &p : pending_) {
+    float ms = 0.f;
+    TN_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    auto &st = kstats_[p.name];
+    st.launches++;
+    st.ms += ms;
+    st.flops += p.flops;
+    event_pool_.push_back(p.a);
+    event_pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+std::map<std::string, Context::KernelStat> Context::kernel_stats() {
+  flush_pending();
+  return kstats_;
+}
+
+void Context::reset_kernel_stats() {
+  flush_pending();
+  kstats_.clear();
+}
 
 std::string shape_str(const Shape &s) {
   std::ostringstream os;
@@ -426,8 +485,16 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
   g.b_n_stride = p.b_n_stride;
   const size_t ws = kern::gemm_workspace_bytes(p.M, p.N, p.K);
   void *w = ws ? ctx.workspace(ws) : nullptr;
-  kern::gemm_f64(g, w, ws, ctx.stream());
-  ctx.gemm_flops += 2.0 * p.M * (double)p.N * p.K;
+  const double flops = 2.0 * p.M * (double)p.N * p.K;
+  if (ctx.profiling()) {
+    hipEvent_t a;
+    ctx.profile_begin(&a);
+    const char *name = kern::gemm_f64(g, w, ws, ctx.stream());
+    ctx.profile_end(a, name, flops);
+  } else {
+    kern::gemm_f64(g, w, ws, ctx.stream());
+  }
+  ctx.gemm_flops += flops;
   ctx.gemm_calls++;
 }
 

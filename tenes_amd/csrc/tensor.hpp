@@ -119,6 +119,21 @@ class Context {
   double gemm_flops = 0.0;
   int64_t gemm_calls = 0;
 
+  // Optional per-launch timing of the GEMM kernels with HIP events recorded
+  // on this context's stream: per kernel instantiation, launch count, summed
+  // device time and summed 2*M*N*K flops.
+  struct KernelStat {
+    int64_t launches = 0;
+    double ms = 0.0;
+    double flops = 0.0;
+  };
+  void set_profiling(bool on);
+  bool profiling() const { return profiling_; }
+  void profile_begin(hipEvent_t *a);
+  void profile_end(hipEvent_t a, const char *name, double flops);
+  std::map<std::string, KernelStat> kernel_stats();  // flushes pending
+  void reset_kernel_stats();
+
  private:
   int device_;
   hipStream_t stream_;
@@ -129,6 +144,17 @@ class Context {
   double *scal_ = nullptr;
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
+  bool profiling_ = false;
+  struct Pending {
+    hipEvent_t a, b;
+    std::string name;
+    double flops;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> event_pool_;
+  std::map<std::string, KernelStat> kstats_;
+  hipEvent_t get_event();
+  void flush_pending();
 };
 
 // ---- construction / transfer ----------------------------------------------

@@ -98,8 +98,13 @@ def test_projector_left_block(gpu_ctx, model, D, chi):
 
 
 def test_projector_updown_blocks(gpu_ctx):
+    """The kept spectrum of this case reaches s/s0 ~ 3e-12, where the singular
+    vectors of any FP64 SVD carry ~eps*s0/s errors that the s^-1/2 scaling
+    amplifies in PU PL^T; compare with a cutoff that keeps the well-conditioned
+    part (s/s0 > 1e-6).  Whole-run parity of this projector: see
+    test_ctm_environment_updown."""
     Tn, lat, env, _ = oracle_env("heisenberg", 2, 8, 40, 2)
-    p = O.PEPSParameters(CHI=8, CTM_Projector_corner=False)
+    p = O.PEPSParameters(CHI=8, CTM_Projector_corner=False, Inverse_projector_cut=1e-6)
     i = 0
     j = lat.right(i)
     k = lat.bottom(j)
@@ -108,7 +113,8 @@ def test_projector_updown_blocks(gpu_ctx):
             env.eTb[k], env.eTb[l], env.eTl[l], env.eTl[i], Tn[i], Tn[j], Tn[k], Tn[l])
     PU, PL = O.Calc_projector_updown_blocks(*args, p)
     gPU, gPL = ta.calc_projector_updown_blocks([gpu_ctx.tensor(a) for a in args],
-                                               ta.peps_parameters(CHI=8, CTM_Projector_corner=0))
+                                               ta.peps_parameters(CHI=8, CTM_Projector_corner=0,
+                                                                  Inverse_projector_cut=1e-6))
     P = np.tensordot(PU, PL, ([3], [3]))
     gP = np.tensordot(gPU.numpy(), gPL.numpy(), ([3], [3]))
     assert rel(gP, P) < 1e-10
@@ -240,3 +246,70 @@ def test_random_tensors_spectra(gpu_ctx):
             got = genv.corner_spectrum(k, i)
             worst = max(worst, np.max(np.abs(got - ref)))
     assert worst < 1e-10
+
+
+def test_ctm_environment_updown(gpu_ctx):
+    """Calc_CTM_Environment with the up-down (4-block) projector."""
+    Tn, ham = su_tensors("heisenberg", 2, 60)
+    Tn = list(Tn)
+    lat = SquareLattice(2, 2)
+    p = O.PEPSParameters(CHI=10, Max_CTM_Iteration=5, CTM_Convergence_Epsilon=0.0, CTM_Projector_corner=False)
+    env, _, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=10, Max_CTM_Iteration=5, CTM_Convergence_Epsilon=0.0,
+                                                 CTM_Projector_corner=0))
+    E, mz = observables_oracle(env, Tn, lat, ham)
+    gE, gmz = observables_gpu(genv, lat, ham)
+    assert abs(gE - E) < 1e-10 * abs(E)
+    assert np.max(np.abs(gmz - mz)) < 1e-10
+
+
+def test_simple_update_bond_gpu(gpu_ctx):
+    """Simple_update_bond: bond weights (gauge invariant) and the two-site
+    wavefunction contracted over the updated bond against the oracle."""
+    from oracle import simple_update as SU
+
+    rng = np.random.default_rng(4)
+    D, d = 3, 2
+    T1 = rng.standard_normal((D, D, D, D, d))
+    T2 = rng.standard_normal((D, D, D, D, d))
+    lam1 = [rng.uniform(0.2, 1.0, D) for _ in range(4)]
+    lam2 = [rng.uniform(0.2, 1.0, D) for _ in range(4)]
+    gate = models.evolution_twosite(models.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0), 0.05)
+    for leg in range(4):
+        r1, r2, rl = SU.Simple_update_bond(T1, T2, lam1, lam2, gate, leg)
+        g1, g2, gl = ta.simple_update_bond(gpu_ctx.tensor(T1), gpu_ctx.tensor(T2), lam1, lam2,
+                                           gpu_ctx.tensor(gate), leg)
+        assert np.max(np.abs(gl - rl)) < 1e-13
+        l2 = (leg + 2) % 4
+        ref = np.tensordot(r1, r2, ([leg], [l2]))
+        got = np.tensordot(g1.numpy(), g2.numpy(), ([leg], [l2]))
+        assert rel(got, ref) < 1e-12
+
+
+def test_simple_update_steps_energy(gpu_ctx):
+    """20 simple-update steps on the GPU from the same initial tensors as the
+    oracle, then CTM on both: energies agree (bond gauges may differ)."""
+    from oracle import simple_update as SU
+
+    lat = SquareLattice(2, 2)
+    D = 2
+    Tn0 = SU.initial_tensors(lat, [2] * 4, [[D] * 4] * 4, models.neel_dirs(lat), [0.01] * 4, seed=11)
+    ham = models.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    gate = models.evolution_twosite(ham, 0.01)
+    ops = [(s, leg, gate) for s, leg in models.square_bonds(lat)]
+    lam = [[np.ones(D) for _ in range(4)] for _ in range(4)]
+    Tn, lam = SU.simple_update_steps([t.copy() for t in Tn0], lam, lat, ops, 20)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn0)
+    genv.simple_update(ops, 20)
+    for s in range(4):
+        for leg in range(4):
+            assert np.max(np.abs(genv.get_lambda(s, leg) - lam[s][leg])) < 1e-12
+    p = O.PEPSParameters(CHI=8, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0)
+    env, _, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=8, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0))
+    E, _ = observables_oracle(env, Tn, lat, ham)
+    gE, _ = observables_gpu(genv, lat, ham)
+    assert abs(gE - E) < 1e-10 * abs(E)

@@ -30,7 +30,7 @@ def test_matmul_shapes(gpu_ctx, M, N, K):
     # (shapeA, shapeB, axA, axB, perm)
     ((4, 5, 6), (6, 5, 3), [1, 2], [1, 0], None),
     ((7, 3, 4, 2), (2, 4, 9), [2, 3], [1, 0], [2, 0, 1]),
-    ((10, 10, 5, 5), (10, 5, 5, 11), [1, 2], [0, 1], [3, 1, 0, 2, 4]),
+    ((10, 10, 5, 5), (10, 5, 5, 11), [1, 2], [0, 1], [3, 1, 0, 2]),
     ((3, 4), (5, 6), [], [], None),
     ((20, 20, 4, 4), (20, 4, 4, 20), [0, 2, 3], [0, 1, 2], [1, 0]),
     ((2, 3, 2, 3, 2), (2, 3, 2, 3, 2), [0, 1, 4], [0, 1, 4], [1, 3, 0, 2]),
