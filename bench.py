@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""CTMRG sweeps/sec at D=5, chi=100 on one MI355X (FP64 path).
+
+Workload (the metric's configuration): Heisenberg antiferromagnet
+(Jx = Jy = Jz = 1) on the 2x2 square unit cell, D = 5, d = 2, chi = 100.
+Site tensors: 500 simple-update steps (tau = 0.01) on the GPU from the Neel
+state with 0.01 noise (std::mt19937 seed 11, TeNeS's fill order).  The
+environment is initialised from the site tensors and `--warmup` sweeps are
+run untimed (chi reaches 100 in the first move); then `--steps` sweeps are
+timed, each = 2 left + 2 right + 2 top + 2 bottom moves (16 projectors with
+a truncated FP64 SVD of a 2500 x 2500 matrix, 16 Calc_Next_CTM, 32
+Calc_Next_eT) + the convergence check (epsilon = 0: no early exit).
+
+Reported: sweeps/s; the dominant kernel's achieved FP64 MFMA TFLOP/s from
+HIP events recorded around every launch on the library stream during the
+timed region (roofline); the CPU baseline = the numpy oracle (same
+contraction order, LAPACK SVD) timing one Left_move on the same environment
+on this host, scaled to sweeps/s (a sweep = 8 moves of equal cost).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CTMRG sweeps/sec at D=5 chi=100 (MFMA TFLOP/s) + rel energy err vs FP64"
+FP64_MFMA_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_inputs(ta, ctx, D, su_steps, tau, seed=11, noise=0.01):
+    from tenes_amd import inputs
+
+    LX = LY = 2
+    env = ta.Environment(ctx, LX, LY)
+    Tn = inputs.initial_tensors(LX * LY, D, 2, inputs.neel_dirs(LX, LY), noise, seed)
+    env.set_site_tensors(Tn)
+    H = inputs.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    g = inputs.gate(H, tau)
+    ops = [(s, leg, g) for s, leg in inputs.square_bonds(LX, LY)]
+    t0 = time.perf_counter()
+    env.simple_update(ops, su_steps)
+    ctx.synchronize()
+    log(f"simple update: {su_steps} steps D={D} in {time.perf_counter() - t0:.2f} s")
+    return env, H
+
+
+def energy(env, H):
+    from tenes_amd import inputs
+
+    E = 0.0
+    for s, leg in inputs.square_bonds(2, 2):
+        dx, dy = (1, 0) if leg == 2 else (0, 1)
+        E += env.contract_two_sites_op12(s, dx, dy, H) / env.contract_cluster(s, dx, dy)
+    return E / 4.0
+
+
+def cpu_baseline(env, chi, n_moves):
+    """Oracle (numpy/LAPACK, FP64) Left_move on the GPU's current
+    environment; returns (sweeps/s, seconds, threads)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import ctm as O
+    from oracle.lattice import SquareLattice
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # pragma: no cover
+        threads = os.cpu_count() or 1
+    lat = SquareLattice(2, 2)
+    oenv = O.Env(4)
+    for k in ("C1", "C2", "C3", "C4", "eTt", "eTr", "eTb", "eTl"):
+        for i in range(4):
+            getattr(oenv, k)[i] = env.get(k, i).numpy()
+    Tn = [env.get("Tn", i).numpy() for i in range(4)]
+    p = O.PEPSParameters(CHI=chi)
+    t0 = time.perf_counter()
+    for m in range(n_moves):
+        O.Left_move(oenv, Tn, m % 2, p, lat)
+    dt = time.perf_counter() - t0
+    per_move = dt / n_moves
+    return 1.0 / (8.0 * per_move), dt, threads
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed CTM sweeps")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed CTM sweeps")
+    ap.add_argument("--D", type=int, default=5)
+    ap.add_argument("--chi", type=int, default=100)
+    ap.add_argument("--su-steps", type=int, default=500)
+    ap.add_argument("--cpu-moves", type=int, default=1, help="oracle moves timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--power-iters", type=int, default=None)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (the CTMRG hot path of one problem runs on a single "
+              f"MI355X here); --gpus {args.gpus} / WORLD_SIZE {world} is not supported.", flush=True)
+        sys.exit(2)
+
+    import tenes_amd as ta
+
+    ctx = ta.Context(0, seed=2024)
+    env, H = build_inputs(ta, ctx, args.D, args.su_steps, 0.01)
+    kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)
+    if args.power_iters is not None:
+        kw["svd_power_iters"] = args.power_iters
+    t0 = time.perf_counter()
+    env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.warmup, **kw), initialize=True)
+    ctx.synchronize()
+    log(f"warmup: {args.warmup} sweeps in {time.perf_counter() - t0:.2f} s")
+
+    ctx.reset_stats()
+    ctx.reset_kernel_report()
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    n, _ = env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.steps, **kw), initialize=False)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    assert n == args.steps
+    flops, ncalls = ctx.stats()
+    rep = ctx.kernel_report()
+    sweeps_per_s = args.steps / dt
+    log(f"timed: {args.steps} sweeps in {dt:.3f} s -> {sweeps_per_s:.3f} sweeps/s; "
+        f"{ncalls} GEMMs, {flops / dt / 1e12:.2f} TFLOP/s overall")
+    dom_name, (dom_n, dom_ms, dom_fl) = max(rep.items(), key=lambda kv: kv[1][1])
+    gemm_ms = sum(v[1] for v in rep.values())
+    for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1]):
+        log(f"  {k}: {v[0]} launches, {v[1]:.2f} ms, {v[2] / (v[1] * 1e-3) / 1e12 if v[1] else 0:.2f} TFLOP/s")
+    achieved = dom_fl / (dom_ms * 1e-3) / 1e12
+    E = energy(env, H)
+    log(f"energy per site (FP64): {E:.15f}; GEMM share of timed region {gemm_ms / (dt * 1e3):.1%}")
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cps, cdt, threads = cpu_baseline(env, args.chi, args.cpu_moves)
+        log(f"cpu baseline: {args.cpu_moves} Left_move in {cdt:.2f} s on {threads} threads -> {cps:.5f} sweeps/s")
+        cpu = {"value": cps, "unit": "sweeps/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_moves} Left_move (2 projectors with a 2500x2500 LAPACK SVD, 2 Calc_Next_CTM, "
+                         f"4 Calc_Next_eT) of the numpy oracle on the same environment; sweeps/s = 1/(8 t_move)"}
+
+    out = {
+        "metric": METRIC,
+        "value": sweeps_per_s,
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: simple-update Heisenberg site tensors generated on the GPU (seeded)",
+        "config": {"workload": "CTMRG sweep, Heisenberg AFM 2x2 unit cell, D=5, d=2, chi=100, FP64",
+                   "D": args.D, "chi": args.chi, "unit_cell": [2, 2], "su_steps": args.su_steps,
+                   "projector": "corner", "svd": "GPU truncated FP64 SVD (randomized range finder + Jacobi)"},
+        "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "launches": dom_n, "avg_launch_ms": dom_ms / dom_n},
+        "cpu_baseline": cpu,
+        "energy_per_site_fp64": E,
+        "energy_rel_err_vs_fp64": 0.0,
+        "gemm_tflops_overall": flops / dt / 1e12,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -88,3 +88,22 @@ def density_table(rows, timed: bool):
             v = r.split("=")[1].split()
             out[name] = float(v[0]) + 1j * float(v[1])
     return out
+
+
+def afh_real_site_tensors():
+    """AntiferroHeisenberg_real: initial tensors (seed 11), 100 simple-update
+    steps and 1 full-update step (fast full update) through the oracle, as the
+    TeNeS test run does before measuring.  Returns (fixture, lattice, Tn)."""
+    from oracle import ctm, full_update as FU, simple_update as SU
+    from oracle.lattice import SquareLattice
+
+    fx = load("AntiferroHeisenberg_real")
+    LX, LY, skew, pdims, vdims, dirs, noises = unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=True)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    Tn, lam = SU.simple_update_steps(Tn, lam, lat, evolution(fx, True), fx["parameter"]["simple_update"]["num_step"])
+    p = ctm.PEPSParameters(CHI=fx["parameter"]["ctm"]["dimension"])
+    Tn = FU.full_update(Tn, lat, evolution(fx, True, "evolution_full"), fx["parameter"]["full_update"]["num_step"],
+                        p, FU.FUParameters())
+    return fx, lat, pdims, Tn

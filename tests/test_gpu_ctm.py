@@ -313,3 +313,46 @@ def test_simple_update_steps_energy(gpu_ctx):
     E, _ = observables_oracle(env, Tn, lat, ham)
     gE, _ = observables_gpu(genv, lat, ham)
     assert abs(gE - E) < 1e-10 * abs(E)
+
+
+def test_afh_real_goldens_on_gpu(gpu_ctx):
+    """GPU CTM + GPU observables on the AntiferroHeisenberg_real site tensors
+    (SU 100 + FU 1 prepared by the oracle) against TeNeS's own output:
+    energy, <Sz>, <Sx>, SzSz/SxSx/SySy bond averages and correlation lengths."""
+    import golden_io as G
+
+    fx, lat, pdims, Tn = G.afh_real_site_tensors()
+    chi = fx["parameter"]["ctm"]["dimension"]
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=chi))
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, True)
+    gold = G.density_table(fx["outputs"]["density"], timed=False)
+    site_op = {(o["site"], o["group"]): o for o in one}
+    vals = {}
+    for g, name in onames.items():
+        vals[name] = sum(o["coeff"] * genv.contract_one_site(o["site"], o["op"]) / genv.contract_one_site(o["site"])
+                         for o in one if o["group"] == g) / 4
+    for g, name in tnames.items():
+        tot = 0.0
+        for op in (t for t in two if t["group"] == g):
+            s, dx, dy = op["source"], op["dx"], op["dy"]
+            norm = genv.contract_cluster(s, dx, dy)
+            if "op" in op:
+                v = genv.contract_two_sites_op12(s, dx, dy, op["op"])
+            else:
+                tgt = lat.other(s, dx, dy)
+                v = genv.contract_cluster(s, dx, dy, site_op[(s, op["ops"][0])]["op"],
+                                          site_op[(tgt, op["ops"][1])]["op"])
+            tot += op["coeff"] * v / norm
+        vals[name] = tot / 4
+    for name, v in vals.items():
+        assert abs(v - gold[name].real) < 1e-10, (name, v, gold[name])
+    assert abs(vals["bond_hamiltonian"] - gold["Energy"].real) < 1e-10
+    for r in fx["outputs"]["correlation_length"]:
+        ref = [float(x) for x in r.split()]
+        d, f = int(ref[0]), int(ref[1])
+        ev = genv.transfer_matrix_eigenvalues(d, f)
+        L = 2
+        xi = -L / np.log(abs(ev[1]) / abs(ev[0]))
+        assert abs(xi - ref[2]) < 1e-10

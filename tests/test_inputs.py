@@ -1,0 +1,25 @@
+"""Host-side input preparation of the package against the oracle."""
+import numpy as np
+
+from oracle import models, simple_update as SU
+from oracle.lattice import SquareLattice
+from tenes_amd import inputs
+
+
+def test_initial_tensors_match_tenes_rng_order():
+    lat = SquareLattice(2, 2)
+    for D in (2, 5):
+        ref = SU.initial_tensors(lat, [2] * 4, [[D] * 4] * 4, models.neel_dirs(lat), [0.01] * 4, seed=11)
+        got = inputs.initial_tensors(4, D, 2, inputs.neel_dirs(2, 2), 0.01, seed=11)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+
+
+def test_hamiltonian_and_gate():
+    for kw in ({"Jz": 1.0, "Jx": 1.0, "Jy": 1.0}, {"Jz": -1.0, "hx": 1.0}):
+        H = inputs.bond_hamiltonian(**kw)
+        assert np.array_equal(H, np.real(models.bond_hamiltonian(**kw)))
+        assert np.allclose(inputs.gate(H, 0.01), models.evolution_twosite(H, 0.01), atol=1e-15)
+    # Heisenberg bond: singlet energy -3/4, triplet 1/4
+    w = np.linalg.eigvalsh(inputs.bond_hamiltonian(Jz=1, Jx=1, Jy=1).reshape(4, 4))
+    assert np.allclose(w, [-0.75, 0.25, 0.25, 0.25])

@@ -140,3 +140,58 @@ def test_product_state_ctm_closed_form():
     sz, sx = v @ models.Sz @ v, v @ models.Sx @ v
     assert abs(o[0][0] - sz) < 1e-14
     assert abs(np.real(sum(w[0].values())) / 4 - 2 * (sz * sz + sx * sx)) < 1e-14
+
+
+def test_full_update_bond_golden():
+    """test/full_update.cpp: D=2, chi=4 environment (corners: ones on the
+    leading D x D block, edges all ones), all-ones site tensors, gate
+    op(i,i,i,i)=1, connect=2; compared up to a sign per (i,j,k,l) block at
+    relative tolerance 1e-8 as the reference test does."""
+    from oracle import full_update as FU
+
+    D, d, chi = 2, 2, 4
+    T = [np.ones((D, D, D, D, d)) for _ in range(2)]
+    C = []
+    for _ in range(4):
+        c = np.zeros((chi, chi))
+        c[:D, :D] = 1.0
+        C.append(c)
+    E = [np.ones((chi, chi, D, D)) for _ in range(6)]
+    op = np.zeros((d, d, d, d))
+    for i in range(d):
+        op[i, i, i, i] = 1.0
+    n1, n2 = FU.Full_update_bond(*C, *E, T[0], T[1], op, 2, FU.FUParameters())
+    vals = [float(x) for row in G.load("full_update") for x in row]
+    ans = [np.array(vals[:32]).reshape(D, D, D, D, d), np.array(vals[32:64]).reshape(D, D, D, D, d)]
+    for res, a in zip((n1, n2), ans):
+        for idx in np.ndindex(D, D, D, D):
+            r, x = res[idx], a[idx]
+            nz = np.nonzero(np.abs(r) > 1e-14)[0]
+            sign = 1.0 if nz.size == 0 or r[nz[0]] * x[nz[0]] > 0 else -1.0
+            assert np.allclose(sign * r, x, rtol=1e-8, atol=1e-12), (idx, r, x)
+
+
+def test_afh_real_end_to_end_goldens():
+    """AntiferroHeisenberg_real (real FP64, 2x2, D=2, chi=5): SU 100 + FU 1
+    (fast full update) + CTM to convergence + observables and correlation
+    lengths against output_AntiferroHeisenberg_real/ (agreement ~1e-14)."""
+    fx, lat, pdims, Tn = G.afh_real_site_tensors()
+    p = ctm.PEPSParameters(CHI=fx["parameter"]["ctm"]["dimension"])
+    env, count, _ = ctm.Calc_CTM_Environment(Tn, p, lat)
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, True)
+    o, norms = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    dens = itps.density(o, w, lat, pdims, onames, tnames)
+    gold = G.density_table(fx["outputs"]["density"], timed=False)
+    n = 0
+    for k, v in gold.items():
+        if k in dens:
+            assert abs(dens[k] - v) < 1e-12, (k, dens[k], v)
+            n += 1
+    assert n >= 9
+    for r, (d, f, xi, es, ev) in zip(fx["outputs"]["correlation_length"],
+                                     itps.correlation_lengths(env, Tn, lat, TM.TMParameters())):
+        ref = [float(x) for x in r.split()]
+        assert (ref[0], ref[1]) == (d, f)
+        assert abs(xi - ref[2]) < 1e-12
+        assert np.max(np.abs(np.array(es) - np.array(ref[3:]))) < 1e-11
