@@ -136,10 +136,12 @@ def main():
     ctx.set_profiling(False)
     assert n == args.steps
     flops, ncalls = ctx.stats()
+    nsvd, nrounds = ctx.svd_stats()
     rep = ctx.kernel_report()
     sweeps_per_s = args.steps / dt
     log(f"timed: {args.steps} sweeps in {dt:.3f} s -> {sweeps_per_s:.3f} sweeps/s; "
-        f"{ncalls} GEMMs, {flops / dt / 1e12:.2f} TFLOP/s overall")
+        f"{ncalls} GEMMs, {flops / dt / 1e12:.2f} TFLOP/s overall; {nsvd} truncated SVDs, "
+        f"{nrounds / max(nsvd, 1):.0f} Jacobi rounds each")
     dom_name, (dom_n, dom_ms, dom_fl) = max(rep.items(), key=lambda kv: kv[1][1])
     gemm_ms = sum(v[1] for v in rep.values())
     for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1]):

@@ -56,6 +56,11 @@ int tenes_ctx_set_seed(tenes_ctx *ctx, uint64_t seed);
 /* Accumulated GEMM work issued through this context (2*M*N*K per GEMM). */
 int tenes_ctx_stats(tenes_ctx *ctx, double *gemm_flops, int64_t *gemm_calls);
 int tenes_ctx_reset_stats(tenes_ctx *ctx);
+/* Truncated SVDs (tenes_svd_top and the projectors) since the last reset and
+ * the Jacobi rounds they took (a round orthogonalises p/2 disjoint column
+ * pairs of the p x p projected matrix; a sweep is p - 1 rounds). */
+int tenes_ctx_svd_stats(tenes_ctx *ctx, int64_t *svd_calls,
+                        int64_t *jacobi_rounds);
 /* Per-launch GEMM timing with HIP events on the context stream (on != 0).
  * tenes_ctx_kernel_report writes one line per kernel instantiation,
  * "name launches total_ms total_flops", into buf (NUL terminated). */

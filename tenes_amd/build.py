@@ -37,13 +37,21 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found: the ROCm toolchain is required to build tenes_amd")
 
 
+# Per-source extra flags.  The GEMM keeps its MFMA accumulators in ordinary
+# VGPRs: with the accumulation-register form the compiler copies all 128
+# accumulator registers out and back around every K tile of the main loop.
+EXTRA_FLAGS = {
+    "kernels_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
 
 def _compile(src: Path, hipcc: str) -> Path:
     obj = BUILD / (src.name + ".o")
-    cmd = [hipcc, *FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc, *FLAGS, *EXTRA_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass warns that it does not know the packed-fp32 feature
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)

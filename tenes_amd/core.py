@@ -47,6 +47,13 @@ class Context:
         self.check(self.lib.tenes_ctx_stats(self.h, C.byref(f), C.byref(n)))
         return f.value, n.value
 
+    def svd_stats(self):
+        """(truncated SVDs, Jacobi rounds) since the last reset_stats()."""
+        a = C.c_int64()
+        b = C.c_int64()
+        self.check(self.lib.tenes_ctx_svd_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def reset_stats(self):
         self.check(self.lib.tenes_ctx_reset_stats(self.h))
 

@@ -193,6 +193,16 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx) {
   return guard(ctx, [&] {
     ctx->c->gemm_flops = 0.0;
     ctx->c->gemm_calls = 0;
+    ctx->c->svd_calls = 0;
+    ctx->c->jacobi_rounds = 0;
+  });
+}
+
+int tenes_ctx_svd_stats(tenes_ctx *ctx, int64_t *svd_calls,
+                        int64_t *jacobi_rounds) {
+  return guard(ctx, [&] {
+    if (svd_calls) *svd_calls = ctx->c->svd_calls;
+    if (jacobi_rounds) *jacobi_rounds = ctx->c->jacobi_rounds;
   });
 }
 

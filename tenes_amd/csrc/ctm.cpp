@@ -133,26 +133,30 @@ tensor identity_projector(tn::Context &ctx, int64_t e78, int64_t t41) {
 }
 
 // Enlarged corner of ctm.cpp:207-212 (and its rotated uses):
-// LT = Tn1*(((C1*eT1)*eT8)*Tn1c), legs (t12, t41, e12, e78, tc12, tc41).
+// LT = Tn1*(((C1*eT1)*eT8)*Tn1c), reference legs (t12, t41, e12, e78, tc12,
+// tc41).  Stored with the legs grouped as a matrix,
+// (t12, e12, tc12 | t41, e78, tc41) = reference transpose (0,2,4,1,3,5),
+// fused into the last contraction's store: the projector GEMMs that consume
+// it then read plain row-major operands.
 tensor enlarged_LT(const tensor &C1, const tensor &eT1, const tensor &eT8,
                    const tensor &Tn1) {
-  return tn::tensordot(
+  return tn::tensordot_t(
       Tn1,
       tn::tensordot(tn::tensordot(tn::tensordot(C1, eT1, Axes{1}, Axes{0}),
                                   eT8, Axes{0}, Axes{1}),
                     tn::conj(Tn1), Axes{2, 5}, Axes{1, 0}),
-      Axes{0, 1, 4}, Axes{3, 1, 6});
+      Axes{0, 1, 4}, Axes{3, 1, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
-// ctm.cpp:222-227: LB = Tn4*(((C4*eT7)*eT6)*Tn4c),
-// legs (t41, t34, e78, e56, tc41, tc34).
+// ctm.cpp:222-227: LB = Tn4*(((C4*eT7)*eT6)*Tn4c), reference legs (t41,
+// t34, e78, e56, tc41, tc34); stored as (t41, e78, tc41 | t34, e56, tc34).
 tensor enlarged_LB(const tensor &C4, const tensor &eT7, const tensor &eT6,
                    const tensor &Tn4) {
-  return tn::tensordot(
+  return tn::tensordot_t(
       Tn4,
       tn::tensordot(tn::tensordot(tn::tensordot(C4, eT7, Axes{1}, Axes{0}),
                                   eT6, Axes{0}, Axes{1}),
                     tn::conj(Tn4), Axes{2, 5}, Axes{0, 3}),
-      Axes{0, 3, 4}, Axes{1, 3, 6});
+      Axes{0, 3, 4}, Axes{1, 3, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
 
 // From the SVD of the (rows x cols) matrix `Rm`, build the oblique projector
@@ -215,9 +219,9 @@ void Calc_projector_left_block(const tensor &C1, const tensor &C4,
                           e12 * t12 * t12),
         e56 * t34 * t34);
   // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273)
-  tensor R = tn::tensordot(LT, LB, Axes{1, 3, 5}, Axes{0, 2, 4});
-  projectors_from_matrix(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 2, 4},
-                         LB, Axes{1, 3, 5}, pp, rsvd, PU, PL);
+  tensor R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
+  projectors_from_matrix(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 1, 2},
+                         LB, Axes{3, 4, 5}, pp, rsvd, PU, PL);
 }
 
 // ctm.cpp:320-491
@@ -257,8 +261,8 @@ void Calc_projector_updown_blocks(
   int cap = pp.CHI;
   if (rsvd) cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * t23);
   // R1 = RT*LT, R2 = RB*LB, legs (t23, e34, tc23, t41, e78, tc41)
-  tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 2, 4});
-  tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{1, 3, 5});
+  tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 1, 2});
+  tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{3, 4, 5});
   tensor M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
   projectors_from_matrix(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2},
                          R2, Axes{0, 1, 2}, pp, rsvd, PU, PL);

@@ -32,20 +32,20 @@ struct KArgs {
   double *partial;  // split-K partials [z][M][N] or nullptr
 };
 
-template <bool FAST_STRIDE>
-__device__ __forceinline__ int off_of(const int *tab, int stride, int i) {
-  return FAST_STRIDE ? i * stride : tab[i];
-}
-
-template <int BM, int BN, bool A_MFAST, bool B_NFAST, bool AFF>
+template <int BM, int BN, bool A_MFAST, bool B_NFAST, bool KA, bool KB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
+  // KA / KB: the contracted-index offsets of A / B form an arithmetic
+  // progression (computed), otherwise they come from the table, prefetched
+  // one K tile ahead so that the tile loads never wait on a table read.
   const GemmArgs &g = ka.g;
   constexpr int TM = BM / 32;
   constexpr int TN = BN / 32;
   constexpr int A_PER = BM * BK / 256;
   constexpr int B_PER = BN * BK / 256;
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
+  constexpr int A_NK = A_MFAST ? A_PER : 1;  // distinct k per thread and tile
+  constexpr int B_NK = B_NFAST ? B_PER : 1;
+  __shared__ double As[2][BK][BM + 1];  // double-buffered K tiles
+  __shared__ double Bs[2][BK][BN + 1];
 
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -75,8 +75,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   for (int p = 0; p < (A_MFAST ? 1 : A_PER); ++p) {
     const int m = m0 + a_ml + p * A_MSTEP;
     a_mok[p] = m < M;
-    a_moff[p] = a_mok[p] ? (AFF && g.a_m_stride ? m * g.a_m_stride : g.a_m[m])
-                         : 0;
+    a_moff[p] = a_mok[p] ? (g.a_m_stride ? m * g.a_m_stride : g.a_m[m]) : 0;
   }
   int b_noff[B_NFAST ? 1 : B_PER];
   bool b_nok[B_NFAST ? 1 : B_PER];
@@ -84,43 +83,65 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   for (int p = 0; p < (B_NFAST ? 1 : B_PER); ++p) {
     const int n = n0 + b_nl + p * B_NSTEP;
     b_nok[p] = n < N;
-    b_noff[p] = b_nok[p] ? (AFF && g.b_n_stride ? n * g.b_n_stride : g.b_n[n])
-                         : 0;
+    b_noff[p] = b_nok[p] ? (g.b_n_stride ? n * g.b_n_stride : g.b_n[n]) : 0;
   }
 
+  // k of load p of this thread within the tile starting at k0
+  auto a_k_of = [&](int k0, int p) { return k0 + a_kl + (A_MFAST ? p * A_KSTEP : 0); };
+  auto b_k_of = [&](int k0, int p) { return k0 + b_kl + (B_NFAST ? p * B_KSTEP : 0); };
+  int ka_off[A_NK], kb_off[B_NK];  // table offsets of the next tile (!KA / !KB)
+  auto fetch_koff = [&](int k0) {
+    if (!KA) {
+#pragma unroll
+      for (int p = 0; p < A_NK; ++p) {
+        const int kk = a_k_of(k0, p);
+        ka_off[p] = g.a_k[kk < kend ? kk : kbeg];
+      }
+    }
+    if (!KB) {
+#pragma unroll
+      for (int p = 0; p < B_NK; ++p) {
+        const int kk = b_k_of(k0, p);
+        kb_off[p] = g.b_k[kk < kend ? kk : kbeg];
+      }
+    }
+  };
+
+  // Branch-free tile loads: out-of-range elements read offset 0 (a valid
+  // address) and are replaced by zero only when the tile is stored to LDS,
+  // so all loads of a tile stay in flight during the MFMA work.
   double ra[A_PER], rb[B_PER];
+  bool oka[A_PER], okb[B_PER];
   auto load_tile = [&](int k0) {
 #pragma unroll
     for (int p = 0; p < A_PER; ++p) {
-      int m_i = A_MFAST ? 0 : p;
-      int kk = k0 + a_kl + (A_MFAST ? p * A_KSTEP : 0);
-      bool ok = a_mok[m_i] && kk < kend;
-      int koff = 0;
-      if (ok) koff = (AFF && g.a_k_stride) ? kk * g.a_k_stride : g.a_k[kk];
-      ra[p] = ok ? g.A[a_moff[m_i] + koff] : 0.0;
+      const int m_i = A_MFAST ? 0 : p;
+      const int kk = a_k_of(k0, p);
+      oka[p] = a_mok[m_i] && kk < kend;
+      const int koff = KA ? kk * g.a_k_stride : ka_off[A_MFAST ? p : 0];
+      ra[p] = g.A[oka[p] ? a_moff[m_i] + koff : 0];
     }
 #pragma unroll
     for (int p = 0; p < B_PER; ++p) {
-      int n_i = B_NFAST ? 0 : p;
-      int kk = k0 + b_kl + (B_NFAST ? p * B_KSTEP : 0);
-      bool ok = b_nok[n_i] && kk < kend;
-      int koff = 0;
-      if (ok) koff = (AFF && g.b_k_stride) ? kk * g.b_k_stride : g.b_k[kk];
-      rb[p] = ok ? g.B[b_noff[n_i] + koff] : 0.0;
+      const int n_i = B_NFAST ? 0 : p;
+      const int kk = b_k_of(k0, p);
+      okb[p] = b_nok[n_i] && kk < kend;
+      const int koff = KB ? kk * g.b_k_stride : kb_off[B_NFAST ? p : 0];
+      rb[p] = g.B[okb[p] ? b_noff[n_i] + koff : 0];
     }
   };
-  auto store_tile = [&]() {
+  auto store_tile = [&](int buf) {
 #pragma unroll
     for (int p = 0; p < A_PER; ++p) {
-      int ml = a_ml + (A_MFAST ? 0 : p * A_MSTEP);
-      int kl = a_kl + (A_MFAST ? p * A_KSTEP : 0);
-      As[kl][ml] = ra[p];
+      const int ml = a_ml + (A_MFAST ? 0 : p * A_MSTEP);
+      const int kl = a_kl + (A_MFAST ? p * A_KSTEP : 0);
+      As[buf][kl][ml] = oka[p] ? ra[p] : 0.0;
     }
 #pragma unroll
     for (int p = 0; p < B_PER; ++p) {
-      int nl = b_nl + (B_NFAST ? 0 : p * B_NSTEP);
-      int kl = b_kl + (B_NFAST ? p * B_KSTEP : 0);
-      Bs[kl][nl] = rb[p];
+      const int nl = b_nl + (B_NFAST ? 0 : p * B_NSTEP);
+      const int kl = b_kl + (B_NFAST ? p * B_KSTEP : 0);
+      Bs[buf][kl][nl] = okb[p] ? rb[p] : 0.0;
     }
   };
 
@@ -135,19 +156,30 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   const int wrow = wm * (BM / 2);
   const int wcol = wn * (BN / 2);
 
-  if (kbeg < kend) load_tile(kbeg);
+  // Pipeline: tile k+1 is loaded into registers while tile k is multiplied
+  // from LDS buffer k&1, then stored into the other buffer; one barrier per
+  // K tile.
+  if (kbeg < kend) {
+    fetch_koff(kbeg);
+    load_tile(kbeg);
+    store_tile(0);
+    fetch_koff(kbeg + BK);
+  }
+  __syncthreads();
+  int buf = 0;
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    __syncthreads();
-    store_tile();
-    __syncthreads();
-    if (k0 + BK < kend) load_tile(k0 + BK);
+    const bool next = k0 + BK < kend;
+    if (next) {
+      load_tile(k0 + BK);
+      fetch_koff(k0 + 2 * BK);
+    }
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       double a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[ks * 4 + lk][wrow + i * 16 + li];
+      for (int i = 0; i < TM; ++i) a[i] = As[buf][ks * 4 + lk][wrow + i * 16 + li];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = Bs[ks * 4 + lk][wcol + j * 16 + li];
+      for (int j = 0; j < TN; ++j) b[j] = Bs[buf][ks * 4 + lk][wcol + j * 16 + li];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -155,6 +187,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j],
                                                            acc[i][j], 0, 0, 0);
     }
+    if (next) store_tile(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   // ---- epilogue -------------------------------------------------------------
@@ -228,28 +263,42 @@ Plan choose(int M, int N, int K) {
   return p;
 }
 
-template <int BM, int BN>
-const char *launch_cfg(const KArgs &ka, dim3 grid, hipStream_t s, bool aff) {
-  const bool am = ka.g.a_mfast, bn = ka.g.b_nfast;
-#define TN_LAUNCH(AM, BNF, AF)                                              \
-  do {                                                                      \
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AM, BNF, AF>), grid,        \
-                       dim3(256), 0, s, ka);                                \
-    return BM == 128 ? "gemm_f64_kernel<128,128," #AM "," #BNF "," #AF ">"   \
-                     : "gemm_f64_kernel<64,64," #AM "," #BNF "," #AF ">";     \
+// Static name of an instantiation: "gemm_f64_kernel<BM,BN,A_MFAST,B_NFAST,
+// KA,KB>" (used by the per-launch timing report).
+template <int BM, bool AM, bool BNF, bool KAA, bool KBA>
+const char *kernel_name() {
+  static const std::string n = std::string("gemm_f64_kernel<") +
+                               std::to_string(BM) + "," + std::to_string(BM) +
+                               (AM ? ",true" : ",false") +
+                               (BNF ? ",true" : ",false") +
+                               (KAA ? ",true" : ",false") +
+                               (KBA ? ",true>" : ",false>");
+  return n.c_str();
+}
+
+template <int BM, int BN, bool AM, bool BNF>
+const char *launch_k(const KArgs &ka, dim3 grid, hipStream_t s) {
+  const bool ka_aff = ka.g.a_k_stride != 0, kb_aff = ka.g.b_k_stride != 0;
+#define TN_LAUNCH(KAA, KBA)                                                  \
+  do {                                                                       \
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AM, BNF, KAA, KBA>), grid,   \
+                       dim3(256), 0, s, ka);                                 \
+    return kernel_name<BM, AM, BNF, KAA, KBA>();                             \
   } while (0)
-  if (aff) {
-    if (am && bn) TN_LAUNCH(true, true, true);
-    else if (am) TN_LAUNCH(true, false, true);
-    else if (bn) TN_LAUNCH(false, true, true);
-    else TN_LAUNCH(false, false, true);
-  } else {
-    if (am && bn) TN_LAUNCH(true, true, false);
-    else if (am) TN_LAUNCH(true, false, false);
-    else if (bn) TN_LAUNCH(false, true, false);
-    else TN_LAUNCH(false, false, false);
-  }
+  if (ka_aff && kb_aff) TN_LAUNCH(true, true);
+  if (ka_aff) TN_LAUNCH(true, false);
+  if (kb_aff) TN_LAUNCH(false, true);
+  TN_LAUNCH(false, false);
 #undef TN_LAUNCH
+}
+
+template <int BM, int BN>
+const char *launch_cfg(const KArgs &ka, dim3 grid, hipStream_t s) {
+  const bool am = ka.g.a_mfast, bn = ka.g.b_nfast;
+  if (am && bn) return launch_k<BM, BN, true, true>(ka, grid, s);
+  if (am) return launch_k<BM, BN, true, false>(ka, grid, s);
+  if (bn) return launch_k<BM, BN, false, true>(ka, grid, s);
+  return launch_k<BM, BN, false, false>(ka, grid, s);
 }
 
 }  // namespace
@@ -281,11 +330,9 @@ const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
     }
   }
   if (g.K == 0) ka.k_per_split = 0;
-  const bool aff = g.a_m_stride || g.a_k_stride || g.b_k_stride ||
-                   g.b_n_stride;
   dim3 grid(ceil_div(g.N, p.BN), ceil_div(g.M, p.BM), splits);
-  const char *name = (p.BM == 128) ? launch_cfg<128, 128>(ka, grid, s, aff)
-                                   : launch_cfg<64, 64>(ka, grid, s, aff);
+  const char *name = (p.BM == 128) ? launch_cfg<128, 128>(ka, grid, s)
+                                   : launch_cfg<64, 64>(ka, grid, s);
   TN_HIP_CHECK(hipGetLastError());
   if (ka.partial) {
     const int64_t MN = (int64_t)g.M * g.N;

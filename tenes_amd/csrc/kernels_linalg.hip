@@ -23,6 +23,7 @@ namespace kern {
 namespace {
 
 constexpr int kMaxDim = 128;
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
@@ -48,122 +49,278 @@ __global__ void randn_kernel(double *p, int64_t n, uint64_t seed,
 }
 
 // ---------------------------------------------------------------------------
-// Shifted Cholesky + inverse of the upper factor, one workgroup (1024 thr).
+// Shifted Cholesky + inverse of the upper factor: blocked, one workgroup of
+// 4 waves, the padded matrix resident in LDS.
+//
+// Blocks of 16: the 16 x 16 diagonal block is factored (and inverted) by one
+// wave in registers, lane l / register t holding element (l/16 + 4t, l%16) --
+// the output layout of v_mfma_f64_16x16x4, so rows are broadcast with lane
+// shuffles and no LDS round trip.  The block row of R is one triangular
+// multiply, the trailing update and the block back substitution for R^{-1}
+// are 16 x 16 x 16 products on the FP64 MFMA, whose output registers are
+// directly the B operand of the next product.  Barriers: 3 per block column
+// plus 1 per block row of the inverse (31 for p = 128, against 2p for an
+// unblocked factorisation).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void chol_inv_kernel(const double *G, int p,
-                                                        double shift_coef,
-                                                        double rel_floor,
-                                                        double *R,
-                                                        double *Rinv) {
-  extern __shared__ double A[];  // p*p, row-major
+constexpr int kCholThreads = 256;
+constexpr int kCholXdLd = 17;  // padded leading dimension of the diagonal inverses
+
+// Value of v in lane `src` (a compile-time or wave-uniform lane index).
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+size_t chol_lds_bytes(int p) {
+  const int pp = (p + 15) / 16 * 16;
+  return ((size_t)pp * (pp + 1) + (size_t)(pp / 16) * 16 * kCholXdLd) *
+         sizeof(double);
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
+    const double *G, int p, double shift_coef, double rel_floor, double *R,
+    double *Rinv) {
+  extern __shared__ double lds[];
+  __shared__ int defic[kMaxDim];
   __shared__ double s_shift, s_thr;
   __shared__ int s_zero;
-  __shared__ int defic[kMaxDim];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < p * p; i += nt) A[i] = G[i];
+  const int pp = (p + 15) / 16 * 16;  // padded size (identity padding)
+  const int nb = pp / 16;
+  const int ld = pp + 1;
+  double *A = lds;                    // pp x ld, upper triangle: G, then F
+  double *Xd = lds + (size_t)pp * ld; // nb blocks of 16 x kCholXdLd: F_kk^-1
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int lr = lane >> 4, lc = lane & 15;  // (row in 4-group, column)
+
+  // Load: thread -> columns (tid & 63) and (tid & 63) + 64, rows
+  // (tid >> 6) + 4 i; batches of 8 rows keep 16 global loads in flight.
+  const int c0 = tid & 63, r0 = tid >> 6;
+  for (int rb = 0; rb < pp; rb += 32) {
+    double v[8][2];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = rb + r0 + 4 * u, c = c0 + 64 * h;
+        v[u][h] = (r < p && c < p) ? G[r * p + c] : (r == c ? 1.0 : 0.0);
+      }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = rb + r0 + 4 * u, c = c0 + 64 * h;
+        if (r < pp && c < pp) A[r * ld + c] = v[u][h];
+      }
+  }
   __syncthreads();
-  if (tid == 0) {
+  if (w == 0) {
     double tr = 0.0, mx = 0.0;
-    for (int i = 0; i < p; ++i) {
-      tr += fabs(A[i * p + i]);
-      mx = fmax(mx, A[i * p + i]);
+    for (int i = lane; i < p; i += 64) {
+      const double g = A[i * ld + i];
+      tr += fabs(g);
+      mx = fmax(mx, g);
     }
-    s_zero = !(tr > 0.0);
-    s_shift = shift_coef * tr;
-    // pivots at or below rel_floor * max diagonal mark a column that is
-    // numerically in the span of the previous ones (rank-deficient block)
-    s_thr = rel_floor > 0.0 ? rel_floor * (mx + s_shift) : -1.0;
+    for (int o = 32; o > 0; o >>= 1) {
+      tr += __shfl_xor(tr, o, 64);
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) {
+      s_zero = !(tr > 0.0);
+      s_shift = shift_coef * tr;
+      // pivots at or below rel_floor * max diagonal mark a column that is
+      // numerically in the span of the previous ones (rank-deficient block)
+      s_thr = rel_floor > 0.0 ? rel_floor * (mx + s_shift) : -1.0;
+    }
   }
   __syncthreads();
   if (s_zero) {
     // Zero (or NaN) Gram matrix: the input block is zero.  R = 0 (the block
     // has no range), Rinv = I so that Q = Y * Rinv stays zero.
-    for (int i = tid; i < p * p; i += nt) {
-      const double v = (i / p == i % p) ? 1.0 : 0.0;
-      R[i] = 0.0;
-      Rinv[i] = v;
-    }
+    for (int r = r0; r < p; r += 4)
+      for (int c = c0; c < p; c += 64) {
+        R[r * p + c] = 0.0;
+        Rinv[r * p + c] = (r == c) ? 1.0 : 0.0;
+      }
     return;
   }
   const double shift = s_shift, thr = s_thr;
-  for (int i = tid; i < p; i += nt) A[i * p + i] += shift;
+  for (int i = tid; i < p; i += kCholThreads) A[i * ld + i] += shift;
   __syncthreads();
   // Floor for pivots that rounding pushed to <= 0 when no deficiency test is
   // requested (first, shifted pass).
   const double floor_d = shift > 0.0 ? shift : DBL_MIN;
-  for (int k = 0; k < p; ++k) {
-    if (tid == 0) {
-      double d = A[k * p + k];
-      if (thr >= 0.0 && !(d > thr)) {
-        // drop column k: unit pivot, zero row, no trailing update
-        defic[k] = 1;
-        A[k * p + k] = 1.0;
-      } else {
-        defic[k] = 0;
-        if (!(d > floor_d)) d = floor_d;
-        A[k * p + k] = sqrt(d);
+
+  for (int kb = 0; kb < nb; ++kb) {
+    const int K0 = kb * 16;
+    // ---- (1) diagonal block: Cholesky and inverse, wave 0, in registers --
+    // Lane c (c = lane % 16; lanes 16..63 duplicate 0..15) holds column c of
+    // the block; every cross-lane value an elimination step needs is a row
+    // element D[k][i] = column i's register k, read with v_readlane (uniform
+    // source lane, SGPR result) instead of an LDS shuffle.
+    if (w == 0) {
+      const int c = lane & 15;
+      double d[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) d[r] = A[(K0 + r) * ld + K0 + c];
+      double piv[16];
+      bool dfc[16];
+      // right-looking, rows kept unscaled: D[i][c] -= D[k][i] D[k][c] / d_k
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        double pk = readlane_f64(d[k], k);
+        const bool dk = thr >= 0.0 && !(pk > thr);
+        if (!dk && !(pk > floor_d)) pk = floor_d;
+        const double inv = dk ? 0.0 : 1.0 / pk;
+        const double rkc = d[k] * inv;
+#pragma unroll
+        for (int i = k + 1; i < 16; ++i) {
+          const double rki = readlane_f64(d[k], i);
+          if (c >= i) d[i] -= rki * rkc;
+        }
+        piv[k] = pk;
+        dfc[k] = dk;
+      }
+      // F: scaled rows, unit pivot and zero row for dropped columns
+      double f[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double sd = sqrt(piv[r]);
+        f[r] = c == r ? (dfc[r] ? 1.0 : sd)
+                      : (c > r ? (dfc[r] ? 0.0 : d[r] / sd) : 0.0);
+      }
+      // X = F^{-1}, column c of X in lane c, rows from the bottom:
+      //   X[i][c] = (delta_ic - sum_{k>i} F[i][k] X[k][c]) / F[i][i]
+      double x[16];
+#pragma unroll
+      for (int i = 15; i >= 0; --i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 16; ++k) acc += readlane_f64(f[i], k) * x[k];
+        x[i] = ((c == i ? 1.0 : 0.0) - acc) / readlane_f64(f[i], i);
+      }
+      if (lane < 16) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          if (c >= r) A[(K0 + r) * ld + K0 + c] = f[r];
+          Xd[(kb * 16 + r) * kCholXdLd + c] = x[r];
+        }
+        if (lane == 0)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) defic[K0 + r] = dfc[r] ? 1 : 0;
       }
     }
     __syncthreads();
-    const double rkk = A[k * p + k];
-    const bool dk = defic[k] != 0;
-    for (int j = k + 1 + tid; j < p; j += nt)
-      A[k * p + j] = dk ? 0.0 : A[k * p + j] / rkk;
-    __syncthreads();
-    if (!dk) {
-      const int rem = p - k - 1;
-      for (int idx = tid; idx < rem * rem; idx += nt) {
-        const int i = k + 1 + idx / rem;
-        const int j = k + 1 + idx % rem;
-        if (j >= i) A[i * p + j] -= A[k * p + i] * A[k * p + j];
+    // ---- (2) block row of F: P = F_kk^{-T} A[K0:K0+16, K0+16:], masked ---
+    const int ncol = pp - K0 - 16;
+    if (tid < ncol) {
+      const int col = K0 + 16 + tid;
+      double v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) v[t] = A[(K0 + t) * ld + col];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t <= r; ++t) acc += Xd[(kb * 16 + t) * kCholXdLd + r] * v[t];
+        A[(K0 + r) * ld + col] = defic[K0 + r] ? 0.0 : acc;
       }
     }
     __syncthreads();
-  }
-  // R (upper) to HBM, zeros below the diagonal and in dropped rows.
-  for (int i = tid; i < p * p; i += nt) {
-    const int r = i / p, c = i % p;
-    R[i] = (c >= r && !defic[r]) ? A[i] : 0.0;
-  }
-  __syncthreads();
-  // In-place inversion of the upper factor, column by column (the leading
-  // j x j block already holds its inverse):
-  //   Rinv[0:j, j] = -(Rinv[0:j,0:j] * R[0:j, j]) / R[j][j],
-  //   Rinv[j][j]   = 1 / R[j][j].
-  // Eight lanes share one row i of the triangular mat-vec.
-  __shared__ double y[kMaxDim];
-  const int i_row = tid >> 3, part = tid & 7;
-  for (int j = 0; j < p; ++j) {
-    double acc = 0.0;
-    if (i_row < j) {
-      for (int k = i_row + part; k < j; k += 8)
-        acc += A[i_row * p + k] * A[k * p + j];
+    // ---- (3) trailing update A[bi][bj] -= P[:,bi]^T P[:,bj], bi <= bj ------
+    const int n = nb - kb - 1;
+    const int nblk = n * (n + 1) / 2;
+    for (int b = w; b < nblk; b += kCholThreads / 64) {
+      int bi = 0, rem = b;
+      while (rem >= n - bi) {  // row-wise enumeration of the upper blocks
+        rem -= n - bi;
+        ++bi;
+      }
+      const int I0 = (kb + 1 + bi) * 16, J0 = (kb + 1 + bi + rem) * 16;
+      d4 acc;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = A[(I0 + lr + 4 * q) * ld + J0 + lc];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double av = -A[(K0 + 4 * s + lr) * ld + I0 + lc];
+        const double bv = A[(K0 + 4 * s + lr) * ld + J0 + lc];
+        acc = mfma16(av, bv, acc);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) A[(I0 + lr + 4 * q) * ld + J0 + lc] = acc[q];
     }
-    acc += __shfl_xor(acc, 4, 64);
-    acc += __shfl_xor(acc, 2, 64);
-    acc += __shfl_xor(acc, 1, 64);
-    if (i_row < j && part == 0) y[i_row] = acc;
-    __syncthreads();
-    const double inv_d = 1.0 / A[j * p + j];
-    if (tid < j) A[tid * p + j] = -y[tid] * inv_d;
-    __syncthreads();
-    if (tid == 0) A[j * p + j] = inv_d;
     __syncthreads();
   }
-  for (int i = tid; i < p * p; i += nt) {
-    const int r = i / p, c = i % p;
-    Rinv[i] = (c >= r) ? A[i] : 0.0;
+
+  // ---- X = F^{-1}: block back substitution, block rows from the bottom ----
+  // X[bi][bj] = -F_bi^{-1} sum_{bi<k<=bj} F[bi][k] X[k][bj]; off-diagonal
+  // blocks of X are stored transposed in the (unused) lower triangle of A.
+  for (int bi = nb - 2; bi >= 0; --bi) {
+    const int n = nb - 1 - bi;
+    for (int idx = w; idx < n; idx += kCholThreads / 64) {
+      const int bj = bi + 1 + idx;
+      d4 T = {0.0, 0.0, 0.0, 0.0};
+      for (int k = bi + 1; k <= bj; ++k) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int kk = k * 16 + 4 * s + lr;
+          const double av = A[(bi * 16 + lc) * ld + kk];
+          const double bv = (k == bj) ? Xd[(bj * 16 + 4 * s + lr) * kCholXdLd + lc]
+                                      : A[(bj * 16 + lc) * ld + kk];
+          T = mfma16(av, bv, T);
+        }
+      }
+      d4 Xo = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        Xo = mfma16(Xd[(bi * 16 + lc) * kCholXdLd + 4 * s + lr], T[s], Xo);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) A[(bj * 16 + lc) * ld + bi * 16 + lr + 4 * q] = -Xo[q];
+    }
+    __syncthreads();
   }
+
+  for (int r = r0; r < p; r += 4)
+    for (int c = c0; c < p; c += 64) {
+    double rv = 0.0, xv = 0.0;
+    if (c >= r) {
+      rv = defic[r] ? 0.0 : A[r * ld + c];
+      // a dropped column gets a zero column in R^{-1}: Q = Y R^{-1} then has
+      // an exact zero there instead of an unnormalised residual (the other
+      // columns of Q do not depend on it: row c of F is e_c^T)
+      xv = defic[c] ? 0.0
+                    : ((r >> 4) == (c >> 4) ? Xd[r * kCholXdLd + (c & 15)]
+                                            : A[c * ld + r]);
+    }
+    R[r * p + c] = rv;
+    Rinv[r * p + c] = xv;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // One-sided Jacobi SVD.
 // ---------------------------------------------------------------------------
+// Cross-lane moves inside a 16-lane row with DPP (register-to-register, no
+// LDS round trip): quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror.  Each pairs every lane with one of the other half of the group
+// being combined, so four steps give every lane the sum over its 16 lanes.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double sum16(double v) {
-  v += __shfl_xor(v, 8, 64);
-  v += __shfl_xor(v, 4, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 1, 64);
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
   return v;
 }
 
@@ -190,13 +347,42 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
     double *rot, int *nsteps) {
   extern __shared__ double W[];  // pe columns of length m (col-major)
   __shared__ double s_off[64];
+  __shared__ double s_tiny;
   const int b = blockIdx.x;
   double *Wb = Wg + (int64_t)b * m * p;
   const int pe = p + (p & 1);  // even number of columns (pad one zero col)
   const int tid = threadIdx.x;
-  for (int i = tid; i < m * pe; i += blockDim.x)
-    W[i] = (i < m * p) ? Wb[i] : 0.0;
+  double fro = 0.0;
+  const int nt = blockDim.x;
+  for (int i0 = 0; i0 < m * pe; i0 += 8 * nt) {  // 8 loads in flight
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * nt + tid;
+      v[u] = (i < m * p) ? Wb[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * nt + tid;
+      if (i < m * pe) W[i] = v[u];
+      fro += v[u] * v[u];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) fro += __shfl_xor(fro, o, 64);
+  if ((tid & 63) == 0) s_off[tid >> 6] = fro;
   __syncthreads();
+  if (tid == 0) {
+    double f = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) f += s_off[w];
+    // columns whose norm is below 1e-30 of the matrix norm are treated as
+    // zero: neither rotated nor counted for convergence.  The bound is far
+    // below rounding level on purpose: skipping a column of norm t leaves a
+    // kept column of norm s unrotated by an angle of up to t / s, and kept
+    // singular values go down to 1e-12 of the largest.
+    s_tiny = f * 1e-60;
+  }
+  __syncthreads();
+  const double tiny = s_tiny;
 
   const int grp = tid >> 4;  // pair slot 0..63
   const int gl = tid & 15;   // lane in the 16-lane group
@@ -212,19 +398,25 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
         rr_pair(s, k, pe, i, j);
         double *wi = W + i * m;
         double *wj = W + j * m;
+        // the pair's two columns stay in registers between the inner
+        // products and the rotation (m <= 128: 8 rows per lane)
+        double xr[8], yr[8];
         double a = 0.0, bb = 0.0, g = 0.0;
-        for (int r = gl; r < m; r += 16) {
-          const double x = wi[r], y = wj[r];
-          a += x * x;
-          bb += y * y;
-          g += x * y;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int r = gl + 16 * u;
+          xr[u] = r < m ? wi[r] : 0.0;
+          yr[u] = r < m ? wj[r] : 0.0;
+          a += xr[u] * xr[u];
+          bb += yr[u] * yr[u];
+          g += xr[u] * yr[u];
         }
         a = sum16(a);
         bb = sum16(bb);
         g = sum16(g);
         double c = 1.0, sn = 0.0;
         const double den = sqrt(a) * sqrt(bb);
-        if (g != 0.0 && den > 0.0) {
+        if (g != 0.0 && a > tiny && bb > tiny) {
           const double rel = fabs(g) / den;
           off_local = fmax(off_local, rel);
           if (rel > tol) {
@@ -233,10 +425,13 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
                              (fabs(zeta) + hypot(1.0, zeta));
             c = 1.0 / sqrt(1.0 + t * t);
             sn = c * t;
-            for (int r = gl; r < m; r += 16) {
-              const double x = wi[r], y = wj[r];
-              wi[r] = c * x - sn * y;
-              wj[r] = sn * x + c * y;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int r = gl + 16 * u;
+              if (r < m) {
+                wi[r] = c * xr[u] - sn * yr[u];
+                wj[r] = sn * xr[u] + c * yr[u];
+              }
             }
           }
         }
@@ -270,12 +465,18 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
 }
 
 // J = I * (product of recorded rotations).  Row-parallel: one 64-lane
-// workgroup owns ROWS rows of J in LDS and replays the rotation stream.
+// workgroup owns ROWS rows of J in LDS and replays the rotation stream.  The
+// stream is read in chunks of kAccChunk rounds: the next chunk is loaded
+// into registers (independent loads, all in flight) while the current one is
+// replayed from LDS, so the replay does not wait on memory every round.
+constexpr int kAccChunk = 16;
+
 template <int ROWS>
 __global__ __launch_bounds__(64) void jacobi_accum_kernel(const double *rot,
                                                           const int *nsteps,
                                                           int p, double *J) {
   __shared__ double row[ROWS][kMaxDim + 2];
+  __shared__ double cs[kAccChunk * (kMaxDim / 2) * 2];
   const int pe = p + (p & 1);
   const int npairs = pe / 2;
   const int r0 = blockIdx.x * ROWS;
@@ -283,24 +484,51 @@ __global__ __launch_bounds__(64) void jacobi_accum_kernel(const double *rot,
   for (int q = 0; q < ROWS; ++q)
     for (int c = lane; c < pe; c += 64)
       row[q][c] = (r0 + q < p && c == r0 + q) ? 1.0 : 0.0;
-  __syncthreads();
   const int ns = nsteps[0];
-  for (int st = 0; st < ns; ++st) {
-    const int s = st % (pe - 1);
-    for (int k = lane; k < npairs; k += 64) {
-      int i, j;
-      rr_pair(s, k, pe, i, j);
-      const double c = rot[((int64_t)st * npairs + k) * 2];
-      const double sn = rot[((int64_t)st * npairs + k) * 2 + 1];
+  const int per = kAccChunk * npairs * 2;  // doubles per chunk (<= 2048)
+  const int64_t total = (int64_t)ns * npairs * 2;
+  const int nchunks = (ns + kAccChunk - 1) / kAccChunk;
+  constexpr int kPerLane = kAccChunk * (kMaxDim / 2) * 2 / 64;
+  double buf[kPerLane];
+  auto fetch = [&](int ch) {
+    const int64_t base = (int64_t)ch * per;
 #pragma unroll
-      for (int q = 0; q < ROWS; ++q) {
-        const double x = row[q][i], y = row[q][j];
-        row[q][i] = c * x - sn * y;
-        row[q][j] = sn * x + c * y;
-      }
+    for (int u = 0; u < kPerLane; ++u) {
+      const int idx = u * 64 + lane;
+      buf[u] = (idx < per && base + idx < total) ? rot[base + idx] : 0.0;
+    }
+  };
+  if (nchunks > 0) fetch(0);
+  for (int ch = 0; ch < nchunks; ++ch) {
+    __syncthreads();  // the previous chunk has been replayed
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+      const int idx = u * 64 + lane;
+      if (idx < per) cs[idx] = buf[u];
     }
     __syncthreads();
+    if (ch + 1 < nchunks) fetch(ch + 1);
+    const int st0 = ch * kAccChunk;
+    const int st1 = min(ns, st0 + kAccChunk);
+    for (int st = st0; st < st1; ++st) {
+      const int s = st % (pe - 1);
+      const double *cst = cs + (st - st0) * npairs * 2;
+      for (int k = lane; k < npairs; k += 64) {
+        int i, j;
+        rr_pair(s, k, pe, i, j);
+        const double c = cst[2 * k];
+        const double sn = cst[2 * k + 1];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          const double x = row[q][i], y = row[q][j];
+          row[q][i] = c * x - sn * y;
+          row[q][j] = sn * x + c * y;
+        }
+      }
+      __syncthreads();
+    }
   }
+  __syncthreads();
   for (int q = 0; q < ROWS; ++q)
     if (r0 + q < p)
       for (int c = lane; c < p; c += 64) J[(int64_t)(r0 + q) * p + c] = row[q][c];
@@ -330,7 +558,8 @@ void linalg_init_device() {
   const int lds = kMaxDim * kMaxDim * (int)sizeof(double);
   TN_HIP_CHECK(hipFuncSetAttribute(
       reinterpret_cast<const void *>(&chol_inv_kernel),
-      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)chol_lds_bytes(kMaxDim)));
   TN_HIP_CHECK(hipFuncSetAttribute(
       reinterpret_cast<const void *>(&jacobi_kernel),
       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
@@ -349,8 +578,8 @@ void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
 void chol_inv_upper(const double *G, int p, double shift_coef,
                     double rel_floor, double *R, double *Rinv, hipStream_t s) {
   TN_REQUIRE(p >= 1 && p <= kMaxDim, "chol_inv_upper: p out of range");
-  const size_t lds = (size_t)p * p * sizeof(double);
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(1024), lds, s, G, p,
+  const size_t lds = chol_lds_bytes(p);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(kCholThreads), lds, s, G, p,
                      shift_coef, rel_floor, R, Rinv);
   TN_HIP_CHECK(hipGetLastError());
 }

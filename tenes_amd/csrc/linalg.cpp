@@ -24,9 +24,11 @@ struct IntBuf {
   }
 };
 
+// Relative orthogonality |x.y| / (|x||y|) below which a column pair counts
+// as converged: sqrt(m) * eps, the tolerance of LAPACK's dgesvj.
 double jacobi_tol_for(const SvdOptions &opt, int m) {
   if (opt.jacobi_tol > 0.0) return opt.jacobi_tol;
-  return std::max(1.0, std::sqrt((double)m)) * 2.220446049250313e-16;
+  return std::sqrt(std::max(1.0, (double)m)) * 2.220446049250313e-16;
 }
 
 // Jacobi SVD of the rows x cols matrix held column-major in W (consumed).
@@ -35,9 +37,13 @@ double jacobi_tol_for(const SvdOptions &opt, int m) {
 struct JacobiOut {
   Tensor sigma_sorted;
   IntBuf *perm = nullptr;
+  IntBuf *nsteps = nullptr;  // rounds taken (device int)
   Tensor J;
   Tensor W;
-  ~JacobiOut() { delete perm; }
+  ~JacobiOut() {
+    delete perm;
+    delete nsteps;
+  }
 };
 
 void run_jacobi(Context &ctx, Tensor &W, int rows, int cols, bool want_J,
@@ -45,34 +51,37 @@ void run_jacobi(Context &ctx, Tensor &W, int rows, int cols, bool want_J,
   hipStream_t s = ctx.stream();
   const int pe = cols + (cols & 1);
   Tensor sigma = empty(ctx, Shape{cols});
-  IntBuf nsteps(ctx, 1);
+  out.nsteps = new IntBuf(ctx, 1);
+  int *nsteps = out.nsteps->ptr;
   Tensor rot;
   if (want_J)
     rot = empty(ctx, Shape{(int64_t)opt.max_sweeps * (pe - 1) * (pe / 2) * 2});
   kern::jacobi_svd(W.data(), rows, cols, 1, jacobi_tol_for(opt, rows),
                    opt.max_sweeps, sigma.data(), want_J ? rot.data() : nullptr,
-                   nsteps.ptr, s);
+                   nsteps, s);
   out.sigma_sorted = empty(ctx, Shape{cols});
   out.perm = new IntBuf(ctx, cols);
   kern::sort_desc(sigma.data(), cols, 1, out.sigma_sorted.data(),
                   out.perm->ptr, s);
   if (want_J) {
     out.J = empty(ctx, Shape{cols, cols});
-    kern::jacobi_accumulate(rot.data(), nsteps.ptr, cols, out.J.data(), s);
+    kern::jacobi_accumulate(rot.data(), nsteps, cols, out.J.data(), s);
   }
   out.W = W;
 }
 
 }  // namespace
 
-Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out) {
+Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
+                      int passes) {
   TN_REQUIRE(Y_in.rank() == 2, "orthonormalize: 2-D input expected");
+  TN_REQUIRE(passes >= 1 && passes <= 3, "orthonormalize: 1 to 3 passes");
   const int64_t m = Y_in.shape[0], p = Y_in.shape[1];
   TN_REQUIRE(p >= 1 && p <= kJac, "orthonormalize: block width > 128");
   Tensor Y = Y_in;
   Tensor Rtot;
   const double u = 1.1102230246251565e-16;
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     Tensor G = tensordot(Y, Y, Axes{0}, Axes{0});  // p x p
     Tensor R = empty(ctx, Shape{p, p});
     Tensor Ri = empty(ctx, Shape{p, p});
@@ -122,10 +131,19 @@ SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
     Tensor Om = empty(ctx, Shape{n, p});
     kern::randn(Om.data(), Om.size(), ctx.seed,
                 ctx.next_rng_counter((uint64_t)Om.size()), s);
-    Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr);
-    for (int it = 0; it < opt.power_iters; ++it) {
-      Tensor Z = orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}), nullptr);
-      Q = orthonormalize(ctx, tensordot(A, Z, Axes{1}, Axes{0}), nullptr);
+    // subspace iteration: the intermediate blocks only need every direction
+    // normalised to a well-conditioned basis (shifted + one plain CholeskyQR
+    // pass: cond ~ 1 + O(1e-2)); a single shifted pass is not enough, it
+    // damps the directions below sqrt(shift) instead of normalising them and
+    // the small kept singular directions die out over the iterations.  The
+    // final range basis gets the full three passes.
+    const int q = std::max(0, opt.power_iters);
+    Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr,
+                              q == 0 ? 3 : 2);
+    for (int it = 0; it < q; ++it) {
+      Tensor Z = orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}), nullptr, 2);
+      Q = orthonormalize(ctx, tensordot(A, Z, Axes{1}, Axes{0}), nullptr,
+                         it + 1 == q ? 3 : 2);
     }
     Tensor R2;
     Tensor Bt = tensordot(A, Q, Axes{0}, Axes{0});  // n x p = A^T Q
@@ -156,7 +174,12 @@ SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
   r.s.resize(k);
   TN_HIP_CHECK(hipMemcpyAsync(r.s.data(), r.s_dev.data(), k * sizeof(double),
                               hipMemcpyDeviceToHost, s));
+  int rounds = 0;
+  TN_HIP_CHECK(hipMemcpyAsync(&rounds, jo.nsteps->ptr, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
   ctx.sync();
+  ctx.svd_calls++;
+  ctx.jacobi_rounds += rounds;
   return r;
 }
 

@@ -118,6 +118,10 @@ class Context {
   // accumulated statistics (for benchmarks)
   double gemm_flops = 0.0;
   int64_t gemm_calls = 0;
+  // truncated SVDs through the Jacobi kernel and their round-robin rounds
+  // (one round = one pass over p/2 disjoint column pairs)
+  int64_t svd_calls = 0;
+  int64_t jacobi_rounds = 0;
 
   // Optional per-launch timing of the GEMM kernels with HIP events recorded
   // on this context's stream: per kernel instantiation, launch count, summed

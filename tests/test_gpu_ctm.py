@@ -81,14 +81,21 @@ def upload_env(ctx, Tn, env):
     return genv
 
 
-@pytest.mark.parametrize("model,D,chi", [("heisenberg", 2, 10), ("tfi", 3, 20)])
-def test_projector_left_block(gpu_ctx, model, D, chi):
+@pytest.mark.parametrize("model,D,chi,cut", [("heisenberg", 2, 10, 1e-12), ("tfi", 3, 20, 1e-10)])
+def test_projector_left_block(gpu_ctx, model, D, chi, cut):
+    """PU PL^T against the oracle.  The projector scales the kept singular
+    vectors by s^-1/2, so any two FP64 SVDs (backward error ~eps*s0) give
+    projectors that differ by up to ~eps*s0/s_min.  The tfi case has a
+    singular value at s/s0 = 6.7e-12, just above the default 1e-12 cut,
+    where that bound is 1.6e-5; it is compared with the cut at 1e-10 (smallest
+    kept s/s0 = 3.7e-10).  Whole-run parity at the default cut:
+    test_ctm_environment_observables."""
     Tn, lat, env, _ = oracle_env(model, D, chi, 40, 2)
-    p = O.PEPSParameters(CHI=chi)
+    p = O.PEPSParameters(CHI=chi, Inverse_projector_cut=cut)
     i, l = 0, lat.bottom(0)
     args = (env.C1[i], env.C4[l], env.eTt[i], env.eTb[l], env.eTl[l], env.eTl[i], Tn[i], Tn[l])
     PU, PL = O.Calc_projector_left_block(*args, p)
-    gp = ta.peps_parameters(CHI=chi)
+    gp = ta.peps_parameters(CHI=chi, Inverse_projector_cut=cut)
     gPU, gPL = ta.calc_projector_left_block(*[gpu_ctx.tensor(a) for a in args], gp)
     assert gPU.shape == PU.shape and gPL.shape == PL.shape
     # the SVD gauge cancels in the oblique projector PU PL^T
