@@ -3,19 +3,25 @@
 
 Workload (the metric's configuration): Heisenberg antiferromagnet
 (Jx = Jy = Jz = 1) on the 2x2 square unit cell, D = 5, d = 2, chi = 100.
-Site tensors: 500 simple-update steps (tau = 0.01) on the GPU from the Neel
-state with 0.01 noise (std::mt19937 seed 11, TeNeS's fill order).  The
-environment is initialised from the site tensors and `--warmup` sweeps are
-run untimed (chi reaches 100 in the first move); then `--steps` sweeps are
-timed, each = 2 left + 2 right + 2 top + 2 bottom moves (16 projectors with
-a truncated FP64 SVD of a 2500 x 2500 matrix, 16 Calc_Next_CTM, 32
-Calc_Next_eT) + the convergence check (epsilon = 0: no early exit).
+Site tensors: 500 simple-update steps (tau = 0.01) from the Neel state with
+0.01 noise (std::mt19937 seed 11, TeNeS's fill order) -- the oracle's
+tensors stored in tests/golden/heisenberg_d5_chi100.json when present
+(else the same recipe run on the GPU).  The environment is initialised from
+the site tensors and `--warmup` sweeps are run untimed (chi reaches 100 in
+the first move); then `--steps` sweeps are timed, each = 2 left + 2 right +
+2 top + 2 bottom moves (16 projectors with a truncated FP64 SVD of a
+2500 x 2500 matrix, 16 Calc_Next_CTM, 32 Calc_Next_eT) + the convergence
+check (epsilon = 0: no early exit).
 
-Reported: sweeps/s; the dominant kernel's achieved FP64 MFMA TFLOP/s from
-HIP events recorded around every launch on the library stream during the
-timed region (roofline); the CPU baseline = the numpy oracle (same
-contraction order, LAPACK SVD) timing one Left_move on the same environment
-on this host, scaled to sweeps/s (a sweep = 8 moves of equal cost).
+Reported: sweeps/s; the dominant kernel (the GEMM instantiation and shape
+carrying the most flops: R = LT*LB, 2500^3) with its achieved FP64 MFMA
+TFLOP/s from HIP events recorded around every launch during the timed region
+(roofline); its HBM traffic per launch from the committed rocprofv3 PMC
+summary (profiles/); the relative energy error against the oracle's FP64
+(LAPACK SVD) run of the same problem and sweep count, from a separate untimed
+run; the CPU baseline = the numpy oracle (same contraction order, LAPACK
+SVD) timing one Left_move on the same environment on this host, scaled to
+sweeps/s (a sweep = 8 moves of equal cost).
 """
 from __future__ import annotations
 
@@ -39,21 +45,53 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_inputs(ta, ctx, D, su_steps, tau, seed=11, noise=0.01):
+FIXTURE = os.path.join(ROOT, "tests", "golden", "heisenberg_d5_chi100.json")
+
+
+def load_fixture(D, chi, su_steps):
+    """The oracle's run of this problem, if stored and of the same shape."""
+    if not os.path.exists(FIXTURE):
+        return None
+    fx = json.load(open(FIXTURE))
+    if (fx["D"], fx["chi"], fx["su_steps"]) != (D, chi, su_steps):
+        return None
+    return fx
+
+
+def build_inputs(ta, ctx, D, su_steps, tau, fx, seed=11, noise=0.01):
     from tenes_amd import inputs
 
     LX = LY = 2
     env = ta.Environment(ctx, LX, LY)
+    H = inputs.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    if fx is not None:
+        Tn = [np.array(t).reshape(D, D, D, D, 2) for t in fx["site_tensors"]]
+        env.set_site_tensors(Tn)
+        log(f"site tensors: oracle simple update ({su_steps} steps) from {os.path.relpath(FIXTURE, ROOT)}")
+        return env, H, Tn
     Tn = inputs.initial_tensors(LX * LY, D, 2, inputs.neel_dirs(LX, LY), noise, seed)
     env.set_site_tensors(Tn)
-    H = inputs.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
     g = inputs.gate(H, tau)
     ops = [(s, leg, g) for s, leg in inputs.square_bonds(LX, LY)]
     t0 = time.perf_counter()
     env.simple_update(ops, su_steps)
     ctx.synchronize()
-    log(f"simple update: {su_steps} steps D={D} in {time.perf_counter() - t0:.2f} s")
-    return env, H
+    log(f"simple update: {su_steps} steps D={D} on the GPU in {time.perf_counter() - t0:.2f} s")
+    return env, H, [env.get("Tn", i).numpy() for i in range(4)]
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` (name[MxNxK]) from the committed PMC
+    summary profiles/gemm_traffic.txt: lines "kernel fetch_bytes write_bytes"
+    already scaled by the copy-kernel calibration; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "gemm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == kernel:
+            return float(f[1]) + float(f[2])
+    return None
 
 
 def energy(env, H):
@@ -116,7 +154,8 @@ def main():
     import tenes_amd as ta
 
     ctx = ta.Context(0, seed=2024)
-    env, H = build_inputs(ta, ctx, args.D, args.su_steps, 0.01)
+    fx = load_fixture(args.D, args.chi, args.su_steps)
+    env, H, Tn = build_inputs(ta, ctx, args.D, args.su_steps, 0.01, fx)
     kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)
     if args.power_iters is not None:
         kw["svd_power_iters"] = args.power_iters
@@ -142,13 +181,27 @@ def main():
     log(f"timed: {args.steps} sweeps in {dt:.3f} s -> {sweeps_per_s:.3f} sweeps/s; "
         f"{ncalls} GEMMs, {flops / dt / 1e12:.2f} TFLOP/s overall; {nsvd} truncated SVDs, "
         f"{nrounds / max(nsvd, 1):.0f} Jacobi rounds each")
-    dom_name, (dom_n, dom_ms, dom_fl) = max(rep.items(), key=lambda kv: kv[1][1])
+    # dominant kernel: the GEMM (instantiation and shape) carrying the most
+    # algorithmic work of the timed region
+    dom_name, (dom_n, dom_ms, dom_fl) = max(rep.items(), key=lambda kv: kv[1][2])
     gemm_ms = sum(v[1] for v in rep.values())
-    for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1]):
+    for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])[:12]:
         log(f"  {k}: {v[0]} launches, {v[1]:.2f} ms, {v[2] / (v[1] * 1e-3) / 1e12 if v[1] else 0:.2f} TFLOP/s")
     achieved = dom_fl / (dom_ms * 1e-3) / 1e12
     E = energy(env, H)
     log(f"energy per site (FP64): {E:.15f}; GEMM share of timed region {gemm_ms / (dt * 1e3):.1%}")
+    # accuracy against the oracle's FP64 (LAPACK SVD) run of the same problem:
+    # a fresh environment, the oracle's sweep count, untimed
+    e_err = None
+    if fx is not None:
+        chk = ta.Environment(ctx, 2, 2)
+        chk.set_site_tensors(Tn)
+        chk.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=fx["sweeps"], **kw), initialize=True)
+        E_chk = energy(chk, H)
+        e_err = abs(E_chk - fx["energy_per_site"]) / abs(fx["energy_per_site"])
+        log(f"energy after {fx['sweeps']} sweeps: GPU {E_chk:.15f}, oracle {fx['energy_per_site']:.15f}, "
+            f"rel err {e_err:.2e}")
+    traffic = load_traffic(dom_name)
 
     cpu = None
     if not args.no_cpu:
@@ -176,11 +229,11 @@ def main():
                    "D": args.D, "chi": args.chi, "unit_cell": [2, 2], "su_steps": args.su_steps,
                    "projector": "corner", "svd": "GPU truncated FP64 SVD (randomized range finder + Jacobi)"},
         "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic,
                      "launches": dom_n, "avg_launch_ms": dom_ms / dom_n},
         "cpu_baseline": cpu,
         "energy_per_site_fp64": E,
-        "energy_rel_err_vs_fp64": 0.0,
+        "energy_rel_err_vs_fp64": e_err,
         "gemm_tflops_overall": flops / dt / 1e12,
     }
     print(json.dumps(out), flush=True)

@@ -56,14 +56,21 @@ int tenes_ctx_set_seed(tenes_ctx *ctx, uint64_t seed);
 /* Accumulated GEMM work issued through this context (2*M*N*K per GEMM). */
 int tenes_ctx_stats(tenes_ctx *ctx, double *gemm_flops, int64_t *gemm_calls);
 int tenes_ctx_reset_stats(tenes_ctx *ctx);
+/* The L projector computations of a CTM move (L = rows or columns of the
+ * unit cell) and its renormalisation steps are independent; they run
+ * overlapped on up to n side streams (default 8; n = 1: all work in issue
+ * order on the context stream).  No counterpart in the reference, whose
+ * moves loop over the rows (src/iTPS/core/ctm.cpp:577-615). */
+int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n);
 /* Truncated SVDs (tenes_svd_top and the projectors) since the last reset and
  * the Jacobi rounds they took (a round orthogonalises p/2 disjoint column
  * pairs of the p x p projected matrix; a sweep is p - 1 rounds). */
 int tenes_ctx_svd_stats(tenes_ctx *ctx, int64_t *svd_calls,
                         int64_t *jacobi_rounds);
 /* Per-launch GEMM timing with HIP events on the context stream (on != 0).
- * tenes_ctx_kernel_report writes one line per kernel instantiation,
- * "name launches total_ms total_flops", into buf (NUL terminated). */
+ * tenes_ctx_kernel_report writes one line per kernel instantiation and
+ * problem shape, "name[MxNxK] launches total_ms total_flops", into buf
+ * (NUL terminated). */
 int tenes_ctx_set_profiling(tenes_ctx *ctx, int on);
 int tenes_ctx_kernel_report(tenes_ctx *ctx, char *buf, int buflen);
 int tenes_ctx_reset_kernel_report(tenes_ctx *ctx);

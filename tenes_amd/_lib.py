@@ -55,6 +55,7 @@ _SIGS = {
     "tenes_ctx_stats": (C.c_int, [C.c_void_p, c_dbl_p, c_i64_p]),
     "tenes_ctx_reset_stats": (C.c_int, [C.c_void_p]),
     "tenes_ctx_svd_stats": (C.c_int, [C.c_void_p, c_i64_p, c_i64_p]),
+    "tenes_ctx_set_max_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_kernel_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "tenes_ctx_reset_kernel_report": (C.c_int, [C.c_void_p]),

@@ -47,6 +47,10 @@ class Context:
         self.check(self.lib.tenes_ctx_stats(self.h, C.byref(f), C.byref(n)))
         return f.value, n.value
 
+    def set_max_lanes(self, n: int):
+        """Side streams a CTM move may overlap its rows on (1: none)."""
+        self.check(self.lib.tenes_ctx_set_max_lanes(self.h, int(n)))
+
     def svd_stats(self):
         """(truncated SVDs, Jacobi rounds) since the last reset_stats()."""
         a = C.c_int64()
@@ -62,7 +66,7 @@ class Context:
 
     def kernel_report(self) -> dict:
         """{kernel name: (launches, total_ms, total_flops)} of profiled GEMMs."""
-        buf = C.create_string_buffer(1 << 16)
+        buf = C.create_string_buffer(1 << 20)
         self.check(self.lib.tenes_ctx_kernel_report(self.h, buf, len(buf)))
         out = {}
         for line in buf.value.decode().splitlines():

@@ -198,6 +198,13 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx) {
   });
 }
 
+int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(n >= 1, "max lanes must be >= 1");
+    ctx->c->set_max_lanes(n);
+  });
+}
+
 int tenes_ctx_svd_stats(tenes_ctx *ctx, int64_t *svd_calls,
                         int64_t *jacobi_rounds) {
   return guard(ctx, [&] {

@@ -159,22 +159,51 @@ tensor enlarged_LB(const tensor &C4, const tensor &eT7, const tensor &eT6,
       Axes{0, 3, 4}, Axes{1, 3, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
 
+// One projector computation, split at its only host synchronisation (the
+// singular values decide the cut): *_begin issues the enlarged corners, R
+// and the truncated SVD on the current stream, projectors_finish waits for
+// the singular values and issues the projector GEMMs.
+struct ProjectorJob {
+  bool done = false;  // identity projector (t41 == 1): PU, PL already set
+  tensor PU, PL;
+  tensor Ablk, Bblk;
+  Axes axA, axB;
+  int64_t r0 = 0, r1 = 0, c0 = 0, c1 = 0;
+  int kmax = 0;
+  tn::SvdPending sv;
+};
+
+constexpr int kSyncSlot = 61;  // pinned slot of the synchronous entry points
+
 // From the SVD of the (rows x cols) matrix `Rm`, build the oblique projector
 // pair: PU = Bblk * (VT_c^dagger s^-1/2), PL = Ablk * (U_c^* s^-1/2), both
 // transposed to (e_in, t, tc, cut) (ctm.cpp:290-301 / 466-475).
-void projectors_from_matrix(tn::Context &ctx, const tensor &Rm, int64_t r0,
-                            int64_t r1, int64_t c0, int64_t c1, int chi_cap,
-                            const tensor &Ablk, const Axes &axA,
-                            const tensor &Bblk, const Axes &axB,
-                            const PEPS_Parameters &pp, bool rsvd, tensor &PU,
-                            tensor &PL) {
+void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
+                      int64_t r1, int64_t c0, int64_t c1, int chi_cap,
+                      const tensor &Ablk, const Axes &axA, const tensor &Bblk,
+                      const Axes &axB, const PEPS_Parameters &pp, bool rsvd,
+                      int slot, ProjectorJob &job) {
   const int64_t nr = r0 * r1 * r0, nc = c0 * c1 * c0;
-  const int kmax = (int)std::min<int64_t>(std::min<int64_t>(nr, nc), chi_cap);
-  tn::SvdResult sv =
-      tn::svd_top(ctx, tn::reshape(Rm, Shape{nr, nc}), kmax, svd_opts(pp, rsvd));
+  job.kmax = (int)std::min<int64_t>(std::min<int64_t>(nr, nc), chi_cap);
+  job.sv = tn::svd_top_begin(ctx, tn::reshape(Rm, Shape{nr, nc}), job.kmax,
+                             svd_opts(pp, rsvd), slot);
+  job.Ablk = Ablk;
+  job.Bblk = Bblk;
+  job.axA = axA;
+  job.axB = axB;
+  job.r0 = r0;
+  job.r1 = r1;
+  job.c0 = c0;
+  job.c1 = c1;
+}
+
+void projectors_finish(tn::Context &ctx, ProjectorJob &job,
+                       const PEPS_Parameters &pp) {
+  if (job.done) return;
+  tn::SvdResult sv = tn::svd_top_end(job.sv);
   std::vector<double> s_c;
   const int cut =
-      truncate_spectrum(sv.s, std::min<int>(kmax, (int)sv.s.size()),
+      truncate_spectrum(sv.s, std::min<int>(job.kmax, (int)sv.s.size()),
                         pp.Inverse_projector_cut, s_c);
   TN_REQUIRE(cut >= 1, "projector: no singular value above the cutoff");
   tensor sc = tn::from_host(ctx, Shape{cut}, s_c.data());
@@ -182,21 +211,20 @@ void projectors_from_matrix(tn::Context &ctx, const tensor &Rm, int64_t r0,
   tensor Vc = tn::slice(sv.V, 1, 0, cut);
   tn::multiply_vector_dev(Uc, sc.data(), 1);
   tn::multiply_vector_dev(Vc, sc.data(), 1);
-  Uc = tn::reshape(Uc, Shape{r0, r1, r0, cut});
-  Vc = tn::reshape(Vc, Shape{c0, c1, c0, cut});
-  PU = tn::tensordot_t(Bblk, Vc, axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
-  PL = tn::tensordot_t(Ablk, Uc, axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+  Uc = tn::reshape(Uc, Shape{job.r0, job.r1, job.r0, cut});
+  Vc = tn::reshape(Vc, Shape{job.c0, job.c1, job.c0, cut});
+  job.PU = tn::tensordot_t(job.Bblk, Vc, job.axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+  job.PL = tn::tensordot_t(job.Ablk, Uc, job.axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+  job.Ablk = tensor();
+  job.Bblk = tensor();
+  job.done = true;
 }
 
-}  // namespace
-
-// ctm.cpp:174-318
-void Calc_projector_left_block(const tensor &C1, const tensor &C4,
-                               const tensor &eT1, const tensor &eT6,
-                               const tensor &eT7, const tensor &eT8,
-                               const tensor &Tn1, const tensor &Tn4,
-                               const PEPS_Parameters &pp, tensor &PU,
-                               tensor &PL) {
+// ctm.cpp:174-318 up to the SVD
+void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
+                      const tensor &eT6, const tensor &eT7, const tensor &eT8,
+                      const tensor &Tn1, const tensor &Tn4,
+                      const PEPS_Parameters &pp, int slot, ProjectorJob &job) {
   tn::Context &ctx = *C1.ctx();
   const int64_t e12 = eT1.shape[1];
   const int64_t e56 = eT6.shape[0];
@@ -205,8 +233,9 @@ void Calc_projector_left_block(const tensor &C1, const tensor &C4,
   const int64_t t41 = Tn1.shape[3];
   const int64_t t34 = Tn4.shape[2];
   if (t41 == 1) {
-    PU = identity_projector(ctx, e78, t41);
-    PL = PU;
+    job.PU = identity_projector(ctx, e78, t41);
+    job.PL = job.PU;
+    job.done = true;
     return;
   }
   tensor LT = enlarged_LT(C1, eT1, eT8, Tn1);
@@ -220,25 +249,26 @@ void Calc_projector_left_block(const tensor &C1, const tensor &C4,
         e56 * t34 * t34);
   // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273)
   tensor R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
-  projectors_from_matrix(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 1, 2},
-                         LB, Axes{3, 4, 5}, pp, rsvd, PU, PL);
+  projectors_begin(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 1, 2}, LB,
+                   Axes{3, 4, 5}, pp, rsvd, slot, job);
 }
 
-// ctm.cpp:320-491
-void Calc_projector_updown_blocks(
+// ctm.cpp:320-491 up to the SVD
+void updown_blocks_begin(
     const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
     const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
     const tensor &eT5, const tensor &eT6, const tensor &eT7, const tensor &eT8,
     const tensor &Tn1, const tensor &Tn2, const tensor &Tn3, const tensor &Tn4,
-    const PEPS_Parameters &pp, tensor &PU, tensor &PL) {
+    const PEPS_Parameters &pp, int slot, ProjectorJob &job) {
   tn::Context &ctx = *C1.ctx();
   const int64_t e34 = eT3.shape[1];
   const int64_t e78 = eT7.shape[1];
   const int64_t t23 = Tn2.shape[3];
   const int64_t t41 = Tn4.shape[1];
   if (t41 == 1) {
-    PU = identity_projector(ctx, e78, t41);
-    PL = PU;
+    job.PU = identity_projector(ctx, e78, t41);
+    job.PL = job.PU;
+    job.done = true;
     return;
   }
   tensor LT = enlarged_LT(C1, eT1, eT8, Tn1);
@@ -264,8 +294,78 @@ void Calc_projector_updown_blocks(
   tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 1, 2});
   tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{3, 4, 5});
   tensor M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
-  projectors_from_matrix(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2},
-                         R2, Axes{0, 1, 2}, pp, rsvd, PU, PL);
+  projectors_begin(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2}, R2,
+                   Axes{0, 1, 2}, pp, rsvd, slot, job);
+}
+
+// The L projectors of a move are independent chains of small kernels, each
+// with one host synchronisation; with L > 1 they run overlapped on the lanes
+// of the forked context.  begin(i, slot, job) issues projector i.
+template <class Begin>
+void run_projectors(tn::Context &ctx, int L, Begin begin,
+                    const PEPS_Parameters &pp, std::vector<tensor> &PUs,
+                    std::vector<tensor> &PLs) {
+  std::vector<ProjectorJob> jobs(L);
+  const int nl = std::min(L, ctx.max_lanes());
+  const bool fork = nl > 1 && L <= kSyncSlot && !ctx.forked();
+  if (fork) ctx.fork(nl);
+  for (int i = 0; i < L; ++i) {
+    if (fork) ctx.use_lane(i % nl);
+    begin(i, fork ? i : kSyncSlot, jobs[i]);
+    if (!fork) projectors_finish(ctx, jobs[i], pp);
+  }
+  for (int i = 0; i < L; ++i) {
+    if (fork) ctx.use_lane(i % nl);
+    projectors_finish(ctx, jobs[i], pp);
+    PUs[i] = jobs[i].PU;
+    PLs[i] = jobs[i].PL;
+  }
+  if (fork) ctx.join();
+}
+
+// Independent per-row work of a move (the renormalisation), overlapped on
+// the lanes of the forked context when L > 1.
+template <class Body>
+void run_lanes(tn::Context &ctx, int L, Body body) {
+  const int nl = std::min(L, ctx.max_lanes());
+  const bool fork = nl > 1 && !ctx.forked();
+  if (fork) ctx.fork(nl);
+  for (int i = 0; i < L; ++i) {
+    if (fork) ctx.use_lane(i % nl);
+    body(i);
+  }
+  if (fork) ctx.join();
+}
+
+}  // namespace
+
+// ctm.cpp:174-318
+void Calc_projector_left_block(const tensor &C1, const tensor &C4,
+                               const tensor &eT1, const tensor &eT6,
+                               const tensor &eT7, const tensor &eT8,
+                               const tensor &Tn1, const tensor &Tn4,
+                               const PEPS_Parameters &pp, tensor &PU,
+                               tensor &PL) {
+  ProjectorJob job;
+  left_block_begin(C1, C4, eT1, eT6, eT7, eT8, Tn1, Tn4, pp, kSyncSlot, job);
+  projectors_finish(*C1.ctx(), job, pp);
+  PU = job.PU;
+  PL = job.PL;
+}
+
+// ctm.cpp:320-491
+void Calc_projector_updown_blocks(
+    const tensor &C1, const tensor &C2, const tensor &C3, const tensor &C4,
+    const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
+    const tensor &eT5, const tensor &eT6, const tensor &eT7, const tensor &eT8,
+    const tensor &Tn1, const tensor &Tn2, const tensor &Tn3, const tensor &Tn4,
+    const PEPS_Parameters &pp, tensor &PU, tensor &PL) {
+  ProjectorJob job;
+  updown_blocks_begin(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, eT7, eT8,
+                      Tn1, Tn2, Tn3, Tn4, pp, kSyncSlot, job);
+  projectors_finish(*C1.ctx(), job, pp);
+  PU = job.PU;
+  PL = job.PL;
 }
 
 // ctm.cpp:493-518
@@ -300,23 +400,26 @@ void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
                const std::vector<tensor> &Tn, int ix,
                const PEPS_Parameters &pp, const SquareLattice &lat) {
   const int L = lat.LY_noskew;
+  tn::Context &ctx = *Tn[0].ctx();
   std::vector<tensor> PUs(L), PLs(L);
-  for (int iy = 0; iy < L; ++iy) {
-    const int i = lat.index(ix, iy);
-    const int j = lat.right(i);
-    const int k = lat.bottom(j);
-    const int l = lat.left(k);
-    if (pp.CTM_Projector_corner)
-      Calc_projector_left_block(C1[i], C4[l], eTt[i], eTb[l], eTl[l], eTl[i],
-                                Tn[i], Tn[l], pp, PUs[iy], PLs[iy]);
-    else
-      Calc_projector_updown_blocks(C1[i], C2[j], C3[k], C4[l], eTt[i], eTt[j],
-                                   eTr[j], eTr[k], eTb[k], eTb[l], eTl[l],
-                                   eTl[i], Tn[i], Tn[j], Tn[k], Tn[l], pp,
-                                   PUs[iy], PLs[iy]);
-  }
+  run_projectors(
+      ctx, L,
+      [&](int iy, int slot, ProjectorJob &job) {
+        const int i = lat.index(ix, iy);
+        const int j = lat.right(i);
+        const int k = lat.bottom(j);
+        const int l = lat.left(k);
+        if (pp.CTM_Projector_corner)
+          left_block_begin(C1[i], C4[l], eTt[i], eTb[l], eTl[l], eTl[i], Tn[i],
+                           Tn[l], pp, slot, job);
+        else
+          updown_blocks_begin(C1[i], C2[j], C3[k], C4[l], eTt[i], eTt[j],
+                              eTr[j], eTr[k], eTb[k], eTb[l], eTl[l], eTl[i],
+                              Tn[i], Tn[j], Tn[k], Tn[l], pp, slot, job);
+      },
+      pp, PUs, PLs);
   const std::vector<tensor> C1_bak = C1, C4_bak = C4, eTl_bak = eTl;
-  for (int iy = 0; iy < L; ++iy) {
+  run_lanes(ctx, L, [&](int iy) {
     const int i = lat.index(ix, iy);
     const int j = lat.right(i);
     const int k = lat.bottom(j);
@@ -327,7 +430,7 @@ void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
                   PLs[iy_down], C1[j], C4[k]);
     Calc_Next_eT(eTl_bak[i], Tn[i], PUs[iy], PLs[iy_up], eTl[j]);
     Calc_Next_eT(eTl_bak[l], Tn[l], PUs[iy_down], PLs[iy], eTl[k]);
-  }
+  });
 }
 
 // ctm.cpp:618-681
@@ -339,26 +442,29 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
                 const PEPS_Parameters &pp, const SquareLattice &lat) {
   const int L = lat.LY_noskew;
   const Axes rot{2, 3, 0, 1, 4};
+  tn::Context &ctx = *Tn[0].ctx();
   std::vector<tensor> PUs(L), PLs(L);
-  for (int iy = 0; iy < L; ++iy) {
-    const int k = lat.index(ix, iy);
-    const int l = lat.left(k);
-    const int i = lat.top(l);
-    const int j = lat.right(i);
-    if (pp.CTM_Projector_corner)
-      Calc_projector_left_block(C3[k], C2[j], eTb[k], eTt[j], eTr[j], eTr[k],
-                                tn::transpose(Tn[k], rot),
-                                tn::transpose(Tn[j], rot), pp, PUs[iy],
-                                PLs[iy]);
-    else
-      Calc_projector_updown_blocks(
-          C3[k], C4[l], C1[i], C2[j], eTb[k], eTb[l], eTl[l], eTl[i], eTt[i],
-          eTt[j], eTr[j], eTr[k], tn::transpose(Tn[k], rot),
-          tn::transpose(Tn[l], rot), tn::transpose(Tn[i], rot),
-          tn::transpose(Tn[j], rot), pp, PUs[iy], PLs[iy]);
-  }
+  run_projectors(
+      ctx, L,
+      [&](int iy, int slot, ProjectorJob &job) {
+        const int k = lat.index(ix, iy);
+        const int l = lat.left(k);
+        const int i = lat.top(l);
+        const int j = lat.right(i);
+        if (pp.CTM_Projector_corner)
+          left_block_begin(C3[k], C2[j], eTb[k], eTt[j], eTr[j], eTr[k],
+                           tn::transpose(Tn[k], rot), tn::transpose(Tn[j], rot),
+                           pp, slot, job);
+        else
+          updown_blocks_begin(
+              C3[k], C4[l], C1[i], C2[j], eTb[k], eTb[l], eTl[l], eTl[i],
+              eTt[i], eTt[j], eTr[j], eTr[k], tn::transpose(Tn[k], rot),
+              tn::transpose(Tn[l], rot), tn::transpose(Tn[i], rot),
+              tn::transpose(Tn[j], rot), pp, slot, job);
+      },
+      pp, PUs, PLs);
   const std::vector<tensor> C2_bak = C2, C3_bak = C3, eTr_bak = eTr;
-  for (int iy = 0; iy < L; ++iy) {
+  run_lanes(ctx, L, [&](int iy) {
     const int k = lat.index(ix, iy);
     const int l = lat.left(k);
     const int i = lat.top(l);
@@ -371,7 +477,7 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
                  PLs[iy_down], eTr[l]);
     Calc_Next_eT(eTr_bak[j], tn::transpose(Tn[j], rot), PUs[iy_up], PLs[iy],
                  eTr[i]);
-  }
+  });
 }
 
 // ctm.cpp:683-746
@@ -383,26 +489,29 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
               const SquareLattice &lat) {
   const int L = lat.LX_noskew;
   const Axes rot{1, 2, 3, 0, 4};
+  tn::Context &ctx = *Tn[0].ctx();
   std::vector<tensor> PUs(L), PLs(L);
-  for (int ix = 0; ix < L; ++ix) {
-    const int j = lat.index(ix, iy);
-    const int k = lat.bottom(j);
-    const int l = lat.left(k);
-    const int i = lat.top(l);
-    if (pp.CTM_Projector_corner)
-      Calc_projector_left_block(C2[j], C1[i], eTr[j], eTl[i], eTt[i], eTt[j],
-                                tn::transpose(Tn[j], rot),
-                                tn::transpose(Tn[i], rot), pp, PUs[ix],
-                                PLs[ix]);
-    else
-      Calc_projector_updown_blocks(
-          C2[j], C3[k], C4[l], C1[i], eTr[j], eTr[k], eTb[k], eTb[l], eTl[l],
-          eTl[i], eTt[i], eTt[j], tn::transpose(Tn[j], rot),
-          tn::transpose(Tn[k], rot), tn::transpose(Tn[l], rot),
-          tn::transpose(Tn[i], rot), pp, PUs[ix], PLs[ix]);
-  }
+  run_projectors(
+      ctx, L,
+      [&](int ix, int slot, ProjectorJob &job) {
+        const int j = lat.index(ix, iy);
+        const int k = lat.bottom(j);
+        const int l = lat.left(k);
+        const int i = lat.top(l);
+        if (pp.CTM_Projector_corner)
+          left_block_begin(C2[j], C1[i], eTr[j], eTl[i], eTt[i], eTt[j],
+                           tn::transpose(Tn[j], rot), tn::transpose(Tn[i], rot),
+                           pp, slot, job);
+        else
+          updown_blocks_begin(
+              C2[j], C3[k], C4[l], C1[i], eTr[j], eTr[k], eTb[k], eTb[l],
+              eTl[l], eTl[i], eTt[i], eTt[j], tn::transpose(Tn[j], rot),
+              tn::transpose(Tn[k], rot), tn::transpose(Tn[l], rot),
+              tn::transpose(Tn[i], rot), pp, slot, job);
+      },
+      pp, PUs, PLs);
   const std::vector<tensor> C1_bak = C1, C2_bak = C2, eTt_bak = eTt;
-  for (int ix = 0; ix < L; ++ix) {
+  run_lanes(ctx, L, [&](int ix) {
     const int j = lat.index(ix, iy);
     const int k = lat.bottom(j);
     const int l = lat.left(k);
@@ -415,7 +524,7 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
                  PLs[ix_right], eTt[k]);
     Calc_Next_eT(eTt_bak[i], tn::transpose(Tn[i], rot), PUs[ix_left], PLs[ix],
                  eTt[l]);
-  }
+  });
 }
 
 // ctm.cpp:748-813
@@ -427,26 +536,29 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
                  int iy, const PEPS_Parameters &pp, const SquareLattice &lat) {
   const int L = lat.LX_noskew;
   const Axes rot{3, 0, 1, 2, 4};
+  tn::Context &ctx = *Tn[0].ctx();
   std::vector<tensor> PUs(L), PLs(L);
-  for (int ix = 0; ix < L; ++ix) {
-    const int l = lat.index(ix, iy);
-    const int i = lat.top(l);
-    const int j = lat.right(i);
-    const int k = lat.bottom(j);
-    if (pp.CTM_Projector_corner)
-      Calc_projector_left_block(C4[l], C3[k], eTl[l], eTr[k], eTb[k], eTb[l],
-                                tn::transpose(Tn[l], rot),
-                                tn::transpose(Tn[k], rot), pp, PUs[ix],
-                                PLs[ix]);
-    else
-      Calc_projector_updown_blocks(
-          C4[l], C1[i], C2[j], C3[k], eTl[l], eTl[i], eTt[i], eTt[j], eTr[j],
-          eTr[k], eTb[k], eTb[l], tn::transpose(Tn[l], rot),
-          tn::transpose(Tn[i], rot), tn::transpose(Tn[j], rot),
-          tn::transpose(Tn[k], rot), pp, PUs[ix], PLs[ix]);
-  }
+  run_projectors(
+      ctx, L,
+      [&](int ix, int slot, ProjectorJob &job) {
+        const int l = lat.index(ix, iy);
+        const int i = lat.top(l);
+        const int j = lat.right(i);
+        const int k = lat.bottom(j);
+        if (pp.CTM_Projector_corner)
+          left_block_begin(C4[l], C3[k], eTl[l], eTr[k], eTb[k], eTb[l],
+                           tn::transpose(Tn[l], rot), tn::transpose(Tn[k], rot),
+                           pp, slot, job);
+        else
+          updown_blocks_begin(
+              C4[l], C1[i], C2[j], C3[k], eTl[l], eTl[i], eTt[i], eTt[j],
+              eTr[j], eTr[k], eTb[k], eTb[l], tn::transpose(Tn[l], rot),
+              tn::transpose(Tn[i], rot), tn::transpose(Tn[j], rot),
+              tn::transpose(Tn[k], rot), pp, slot, job);
+      },
+      pp, PUs, PLs);
   const std::vector<tensor> C3_bak = C3, C4_bak = C4, eTb_bak = eTb;
-  for (int ix = 0; ix < L; ++ix) {
+  run_lanes(ctx, L, [&](int ix) {
     const int l = lat.index(ix, iy);
     const int i = lat.top(l);
     const int j = lat.right(i);
@@ -459,7 +571,7 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
                  eTb[i]);
     Calc_Next_eT(eTb_bak[k], tn::transpose(Tn[k], rot), PUs[ix_right],
                  PLs[ix], eTb[j]);
-  }
+  });
 }
 
 namespace {

@@ -32,6 +32,28 @@ struct KArgs {
   double *partial;  // split-K partials [z][M][N] or nullptr
 };
 
+// XCD-aware tile order.  Workgroups are dealt to the 8 XCDs round-robin by
+// linear id, and each XCD has its own L2; so the ids of XCD x (x, x+8, ...)
+// are given a contiguous range of the logical tile order, and that order
+// walks the output in groups of kGroupM tile rows (column by column inside
+// a group).  The tiles running at the same time on one XCD then share a few
+// A row panels and B column panels instead of touching all of them.
+constexpr int kXcds = 8;
+constexpr int kGroupM = 8;
+__device__ __forceinline__ void tile_of(int id, int nbm, int nbn, int &bm, int &bn) {
+  const int ntiles = nbm * nbn;
+  const int xcd = id % kXcds, slot = id / kXcds;
+  const int base = ntiles / kXcds, extra = ntiles % kXcds;
+  const int t = xcd * base + min(xcd, extra) + slot;  // bijection on [0, ntiles)
+  const int per_group = kGroupM * nbn;
+  const int g = t / per_group;
+  const int first_m = g * kGroupM;
+  const int gsize = min(nbm - first_m, kGroupM);
+  const int r = t - g * per_group;
+  bm = first_m + r % gsize;
+  bn = r / gsize;
+}
+
 template <int BM, int BN, bool A_MFAST, bool B_NFAST, bool KA, bool KB>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   // KA / KB: the contracted-index offsets of A / B form an arithmetic
@@ -51,8 +73,10 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   const int lane = t & 63;
   const int w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * BM;
-  const int n0 = blockIdx.x * BN;
+  int bm, bn;
+  tile_of(blockIdx.y * gridDim.x + blockIdx.x, gridDim.y, gridDim.x, bm, bn);
+  const int m0 = bm * BM;
+  const int n0 = bn * BN;
   const int kbeg = blockIdx.z * ka.k_per_split;
   const int kend = min(g.K, kbeg + ka.k_per_split);
   const int M = g.M, N = g.N;

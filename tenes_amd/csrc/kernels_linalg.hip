@@ -52,33 +52,32 @@ __global__ void randn_kernel(double *p, int64_t n, uint64_t seed,
 // Shifted Cholesky + inverse of the upper factor: blocked, one workgroup of
 // 4 waves, the padded matrix resident in LDS.
 //
-// Blocks of 16: the 16 x 16 diagonal block is factored (and inverted) by one
-// wave in registers, lane l / register t holding element (l/16 + 4t, l%16) --
-// the output layout of v_mfma_f64_16x16x4, so rows are broadcast with lane
-// shuffles and no LDS round trip.  The block row of R is one triangular
-// multiply, the trailing update and the block back substitution for R^{-1}
-// are 16 x 16 x 16 products on the FP64 MFMA, whose output registers are
-// directly the B operand of the next product.  Barriers: 3 per block column
-// plus 1 per block row of the inverse (31 for p = 128, against 2p for an
-// unblocked factorisation).
+// Blocks of 16.  Per block column: wave 0 factors the 16 x 16 diagonal block
+// (lane l / register t holds element (l/16 + 4t, l%16); the pivot row is
+// broadcast through a 16-entry LDS row, wave-synchronous, no barrier), the
+// block row of R is a forward substitution with one thread per column, and
+// the trailing update runs on the FP64 MFMA.  After the sweep the diagonal
+// blocks are inverted in parallel (one 16-lane group per block) and the
+// off-diagonal blocks of R^{-1} follow by block back substitution on the
+// MFMA, whose output registers are directly the B operand of the next
+// product.  Barriers: 3 per block column + 1 per block row of the inverse.
 // ---------------------------------------------------------------------------
 constexpr int kCholThreads = 256;
 constexpr int kCholXdLd = 17;  // padded leading dimension of the diagonal inverses
-
-// Value of v in lane `src` (a compile-time or wave-uniform lane index).
-__device__ __forceinline__ double readlane_f64(double v, int src) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-  return __hiloint2double(hi, lo);
-}
 
 __device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Orders one wave's LDS accesses around an exchange between its own lanes:
+// the stores before the fence are complete before the loads after it.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
 size_t chol_lds_bytes(int p) {
   const int pp = (p + 15) / 16 * 16;
-  return ((size_t)pp * (pp + 1) + (size_t)(pp / 16) * 16 * kCholXdLd) *
+  return ((size_t)pp * (pp + 1) + (size_t)(pp / 16) * 16 * kCholXdLd + 16) *
          sizeof(double);
 }
 
@@ -92,29 +91,32 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
   const int pp = (p + 15) / 16 * 16;  // padded size (identity padding)
   const int nb = pp / 16;
   const int ld = pp + 1;
-  double *A = lds;                    // pp x ld, upper triangle: G, then F
-  double *Xd = lds + (size_t)pp * ld; // nb blocks of 16 x kCholXdLd: F_kk^-1
+  double *A = lds;                     // pp x ld, upper triangle: G, then F
+  double *Xd = lds + (size_t)pp * ld;  // nb blocks of 16 x kCholXdLd: F_kk^-1
+  double *rowbuf = Xd + (size_t)nb * 16 * kCholXdLd;  // 16: pivot row
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int lr = lane >> 4, lc = lane & 15;  // (row in 4-group, column)
+  const int c0 = tid & 63, r0 = tid >> 6;    // load/store mapping
 
-  // Load: thread -> columns (tid & 63) and (tid & 63) + 64, rows
-  // (tid >> 6) + 4 i; batches of 8 rows keep 16 global loads in flight.
-  const int c0 = tid & 63, r0 = tid >> 6;
-  for (int rb = 0; rb < pp; rb += 32) {
-    double v[8][2];
+  // Load: thread -> columns c0, c0 + 64 and rows r0 + 4u; all 64 loads of a
+  // thread are issued before the first LDS store (one memory round trip).
+  {
+    double v[32][2];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < 32; ++u)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int r = rb + r0 + 4 * u, c = c0 + 64 * h;
-        v[u][h] = (r < p && c < p) ? G[r * p + c] : (r == c ? 1.0 : 0.0);
+        const int r = r0 + 4 * u, c = c0 + 64 * h;
+        const bool in = r < p && c < p;
+        const double g = G[in ? r * p + c : 0];
+        v[u][h] = in ? g : (r == c ? 1.0 : 0.0);
       }
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < 32; ++u)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int r = rb + r0 + 4 * u, c = c0 + 64 * h;
+        const int r = r0 + 4 * u, c = c0 + 64 * h;
         if (r < pp && c < pp) A[r * ld + c] = v[u][h];
       }
   }
@@ -158,78 +160,66 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
 
   for (int kb = 0; kb < nb; ++kb) {
     const int K0 = kb * 16;
-    // ---- (1) diagonal block: Cholesky and inverse, wave 0, in registers --
-    // Lane c (c = lane % 16; lanes 16..63 duplicate 0..15) holds column c of
-    // the block; every cross-lane value an elimination step needs is a row
-    // element D[k][i] = column i's register k, read with v_readlane (uniform
-    // source lane, SGPR result) instead of an LDS shuffle.
+    // ---- (1) diagonal block: Cholesky, wave 0 --------------------------
     if (w == 0) {
-      const int c = lane & 15;
-      double d[16];
+      double d[4], pv[4];
+      bool dfc[4];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) d[r] = A[(K0 + r) * ld + K0 + c];
-      double piv[16];
-      bool dfc[16];
-      // right-looking, rows kept unscaled: D[i][c] -= D[k][i] D[k][c] / d_k
+      for (int t = 0; t < 4; ++t) {
+        d[t] = A[(K0 + lr + 4 * t) * ld + K0 + lc];
+        pv[t] = 1.0;
+        dfc[t] = false;
+      }
+      // right-looking, rows kept unscaled: D[i][j] -= D[k][i] D[k][j] / d_k
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        double pk = readlane_f64(d[k], k);
-        const bool dk = thr >= 0.0 && !(pk > thr);
-        if (!dk && !(pk > floor_d)) pk = floor_d;
-        const double inv = dk ? 0.0 : 1.0 / pk;
-        const double rkc = d[k] * inv;
+        if (lr == (k & 3)) rowbuf[lc] = d[k >> 2];  // row k of the block
+        wave_lds_fence();
+        double piv = rowbuf[k];
+        const bool dk = thr >= 0.0 && !(piv > thr);
+        if (!dk && !(piv > floor_d)) piv = floor_d;
+        const double rkc = dk ? 0.0 : rowbuf[lc] / piv;
 #pragma unroll
-        for (int i = k + 1; i < 16; ++i) {
-          const double rki = readlane_f64(d[k], i);
-          if (c >= i) d[i] -= rki * rkc;
+        for (int t = 0; t < 4; ++t) {
+          const int r = lr + 4 * t;
+          if (r > k && lc >= r) d[t] -= rowbuf[r] * rkc;
+          if (r == k) {
+            pv[t] = piv;
+            dfc[t] = dk;
+          }
         }
-        piv[k] = pk;
-        dfc[k] = dk;
+        wave_lds_fence();  // all reads of this row before the next store
       }
       // F: scaled rows, unit pivot and zero row for dropped columns
-      double f[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const double sd = sqrt(piv[r]);
-        f[r] = c == r ? (dfc[r] ? 1.0 : sd)
-                      : (c > r ? (dfc[r] ? 0.0 : d[r] / sd) : 0.0);
-      }
-      // X = F^{-1}, column c of X in lane c, rows from the bottom:
-      //   X[i][c] = (delta_ic - sum_{k>i} F[i][k] X[k][c]) / F[i][i]
-      double x[16];
-#pragma unroll
-      for (int i = 15; i >= 0; --i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = i + 1; k < 16; ++k) acc += readlane_f64(f[i], k) * x[k];
-        x[i] = ((c == i ? 1.0 : 0.0) - acc) / readlane_f64(f[i], i);
-      }
-      if (lane < 16) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          if (c >= r) A[(K0 + r) * ld + K0 + c] = f[r];
-          Xd[(kb * 16 + r) * kCholXdLd + c] = x[r];
+      for (int t = 0; t < 4; ++t) {
+        const int r = lr + 4 * t;
+        if (lc >= r) {
+          const double sd = sqrt(pv[t]);
+          A[(K0 + r) * ld + K0 + lc] =
+              lc == r ? (dfc[t] ? 1.0 : sd) : (dfc[t] ? 0.0 : d[t] / sd);
         }
-        if (lane == 0)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) defic[K0 + r] = dfc[r] ? 1 : 0;
+        if (lc == r) defic[K0 + r] = dfc[t] ? 1 : 0;
       }
     }
     __syncthreads();
-    // ---- (2) block row of F: P = F_kk^{-T} A[K0:K0+16, K0+16:], masked ---
+    // ---- (2) block row of F by forward substitution, one thread per column:
+    //   F_kk^T P = A[K0:K0+16, col], rows of dropped columns set to zero
     const int ncol = pp - K0 - 16;
     if (tid < ncol) {
       const int col = K0 + 16 + tid;
       double v[16];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) v[t] = A[(K0 + t) * ld + col];
+      for (int r = 0; r < 16; ++r) v[r] = A[(K0 + r) * ld + col];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        double acc = 0.0;
+        double acc = v[r];
 #pragma unroll
-        for (int t = 0; t <= r; ++t) acc += Xd[(kb * 16 + t) * kCholXdLd + r] * v[t];
-        A[(K0 + r) * ld + col] = defic[K0 + r] ? 0.0 : acc;
+        for (int t = 0; t < r; ++t) acc -= A[(K0 + t) * ld + K0 + r] * v[t];
+        v[r] = defic[K0 + r] ? 0.0 : acc / A[(K0 + r) * ld + K0 + r];
       }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) A[(K0 + r) * ld + col] = v[r];
     }
     __syncthreads();
     // ---- (3) trailing update A[bi][bj] -= P[:,bi]^T P[:,bj], bi <= bj ------
@@ -257,15 +247,37 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
     __syncthreads();
   }
 
+  // ---- diagonal blocks of X = F^{-1}: 16-lane group g of wave w inverts
+  // block 4w + g, lane lc holding column lc, rows from the bottom:
+  //   X[i][c] = (delta_ic - sum_{k>i} F[i][k] X[k][c]) / F[i][i]
+  {
+    const int b = w * 4 + lr;
+    if (b < nb) {
+      const double *Fb = A + (size_t)b * 16 * ld + b * 16;
+      double x[16];
+#pragma unroll
+      for (int i = 15; i >= 0; --i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 16; ++k) acc += Fb[i * ld + k] * x[k];
+        x[i] = ((lc == i ? 1.0 : 0.0) - acc) / Fb[i * ld + i];
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Xd[(b * 16 + r) * kCholXdLd + lc] = x[r];
+    }
+  }
+  __syncthreads();
+
   // ---- X = F^{-1}: block back substitution, block rows from the bottom ----
   // X[bi][bj] = -F_bi^{-1} sum_{bi<k<=bj} F[bi][k] X[k][bj]; off-diagonal
   // blocks of X are stored transposed in the (unused) lower triangle of A.
+  // The k-sum uses two accumulators to halve the dependent MFMA chain.
   for (int bi = nb - 2; bi >= 0; --bi) {
     const int n = nb - 1 - bi;
     for (int idx = w; idx < n; idx += kCholThreads / 64) {
       const int bj = bi + 1 + idx;
-      d4 T = {0.0, 0.0, 0.0, 0.0};
-      for (int k = bi + 1; k <= bj; ++k) {
+      // T += F[bi][k] X[k][bj] for one block k
+      auto accum = [&](d4 T, int k) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int kk = k * 16 + 4 * s + lr;
@@ -274,7 +286,16 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
                                       : A[(bj * 16 + lc) * ld + kk];
           T = mfma16(av, bv, T);
         }
+        return T;
+      };
+      d4 T0 = {0.0, 0.0, 0.0, 0.0}, T1 = {0.0, 0.0, 0.0, 0.0};
+      int k = bi + 1;
+      for (; k + 1 <= bj; k += 2) {
+        T0 = accum(T0, k);
+        T1 = accum(T1, k + 1);
       }
+      if (k <= bj) T0 = accum(T0, k);
+      const d4 T = T0 + T1;
       d4 Xo = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int s = 0; s < 4; ++s)
@@ -285,20 +306,25 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
     __syncthreads();
   }
 
-  for (int r = r0; r < p; r += 4)
-    for (int c = c0; c < p; c += 64) {
-    double rv = 0.0, xv = 0.0;
-    if (c >= r) {
-      rv = defic[r] ? 0.0 : A[r * ld + c];
-      // a dropped column gets a zero column in R^{-1}: Q = Y R^{-1} then has
-      // an exact zero there instead of an unnormalised residual (the other
-      // columns of Q do not depend on it: row c of F is e_c^T)
-      xv = defic[c] ? 0.0
-                    : ((r >> 4) == (c >> 4) ? Xd[r * kCholXdLd + (c & 15)]
-                                            : A[c * ld + r]);
-    }
-    R[r * p + c] = rv;
-    Rinv[r * p + c] = xv;
+#pragma unroll 8
+  for (int u = 0; u < 32; ++u)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = r0 + 4 * u, c = c0 + 64 * h;
+      if (r < p && c < p) {
+        double rv = 0.0, xv = 0.0;
+        if (c >= r) {
+          rv = defic[r] ? 0.0 : A[r * ld + c];
+          // a dropped column gets a zero column in R^{-1}: Q = Y R^{-1} then
+          // has an exact zero there instead of an unnormalised residual (the
+          // other columns of Q do not depend on it: row c of F is e_c^T)
+          xv = defic[c] ? 0.0
+                        : ((r >> 4) == (c >> 4) ? Xd[r * kCholXdLd + (c & 15)]
+                                                : A[c * ld + r]);
+        }
+        R[r * p + c] = rv;
+        Rinv[r * p + c] = xv;
+      }
     }
 }
 
@@ -383,6 +409,7 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
   }
   __syncthreads();
   const double tiny = s_tiny;
+  const double tol2 = tol * tol;
 
   const int grp = tid >> 4;  // pair slot 0..63
   const int gl = tid & 15;   // lane in the 16-lane group
@@ -415,14 +442,16 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
         bb = sum16(bb);
         g = sum16(g);
         double c = 1.0, sn = 0.0;
-        const double den = sqrt(a) * sqrt(bb);
-        if (g != 0.0 && a > tiny && bb > tiny) {
-          const double rel = fabs(g) / den;
-          off_local = fmax(off_local, rel);
-          if (rel > tol) {
-            const double zeta = (bb - a) / (2.0 * g);
-            const double t = copysign(1.0, zeta) /
-                             (fabs(zeta) + hypot(1.0, zeta));
+        // rotate when |g| / sqrt(a bb) > tol, tested without a division
+        const double g2 = g * g;
+        if (g != 0.0 && a > tiny && bb > tiny && g2 > tol2 * (a * bb)) {
+          {
+            off_local = 1.0;
+            // t = tan(theta) = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with
+            // zeta = (bb - a) / (2 g), multiplied through by |2 g|
+            const double dd = bb - a;
+            const double t = (dd < 0.0 ? -2.0 * g : 2.0 * g) /
+                             (fabs(dd) + sqrt(dd * dd + 4.0 * g2));
             c = 1.0 / sqrt(1.0 + t * t);
             sn = c * t;
 #pragma unroll
@@ -444,7 +473,7 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
       ++steps;
       __syncthreads();
     }
-    // workgroup max of the largest relative off-diagonal seen in this sweep
+    // did any pair of this sweep exceed the tolerance (off_local = 1)?
     double v = off_local;
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     if ((tid & 63) == 0) s_off[tid >> 6] = v;
@@ -452,7 +481,7 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
     double off = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) off = fmax(off, s_off[w]);
     __syncthreads();
-    if (!(off > tol)) break;
+    if (!(off > 0.0)) break;
   }
   for (int jc = tid; jc < p; jc += blockDim.x) {
     double acc = 0.0;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "kernels.hpp"
 
@@ -100,8 +101,9 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
   return Y;
 }
 
-SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
-                  const SvdOptions &opt) {
+SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
+                         const SvdOptions &opt, int slot) {
+  TN_REQUIRE(slot >= 0 && slot < 63, "svd_top_begin: pinned slot 0..62");
   TN_REQUIRE(A.rank() == 2, "svd_top: 2-D input expected");
   const int m = (int)A.shape[0], n = (int)A.shape[1];
   TN_REQUIRE(m >= 1 && n >= 1, "svd_top: empty matrix");
@@ -167,20 +169,41 @@ SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
   Tensor Jp = empty(ctx, Shape{cols, k});
   kern::gather_cols(jo.J.data(), cols, 1, jo.perm->ptr, nullptr, cols, k,
                     Jp.data(), s);
-  SvdResult r;
+  SvdPending pd;
+  SvdResult &r = pd.r;
   r.U = have_L ? tensordot(Lq, What, Axes{1}, Axes{0}) : What;
   r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
   r.s_dev = slice(jo.sigma_sorted, 0, 0, k);
-  r.s.resize(k);
-  TN_HIP_CHECK(hipMemcpyAsync(r.s.data(), r.s_dev.data(), k * sizeof(double),
+  // pinned slot: k <= 128 singular values, then the Jacobi round count
+  double *host = ctx.pinned(slot);
+  TN_HIP_CHECK(hipMemcpyAsync(host, r.s_dev.data(), k * sizeof(double),
                               hipMemcpyDeviceToHost, s));
+  TN_HIP_CHECK(hipMemcpyAsync(host + 511, jo.nsteps->ptr, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+  pd.ctx = &ctx;
+  pd.stream = s;
+  pd.slot = slot;
+  pd.k = k;
+  return pd;
+}
+
+SvdResult svd_top_end(SvdPending &pd) {
+  TN_REQUIRE(pd.ctx != nullptr, "svd_top_end: no pending SVD");
+  TN_HIP_CHECK(hipStreamSynchronize(pd.stream));
+  const double *host = pd.ctx->pinned(pd.slot);
+  pd.r.s.assign(host, host + pd.k);
   int rounds = 0;
-  TN_HIP_CHECK(hipMemcpyAsync(&rounds, jo.nsteps->ptr, sizeof(int),
-                              hipMemcpyDeviceToHost, s));
-  ctx.sync();
-  ctx.svd_calls++;
-  ctx.jacobi_rounds += rounds;
-  return r;
+  std::memcpy(&rounds, host + 511, sizeof(int));
+  pd.ctx->svd_calls++;
+  pd.ctx->jacobi_rounds += rounds;
+  pd.ctx = nullptr;
+  return std::move(pd.r);
+}
+
+SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
+                  const SvdOptions &opt) {
+  SvdPending pd = svd_top_begin(ctx, A, k_target, opt, 62);  // slot of synchronous calls
+  return svd_top_end(pd);
 }
 
 Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats) {

@@ -28,6 +28,22 @@ struct SvdResult {
 SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
                   const SvdOptions &opt);
 
+// The same in two halves, for overlapping independent SVDs on the lanes of a
+// forked Context: svd_top_begin issues all device work on the current stream
+// (U, V and s_dev included) and an asynchronous copy of s into pinned host
+// slot `slot` (0..62, distinct for SVDs in flight at the same time);
+// svd_top_end waits for that stream and fills s.
+struct SvdPending {
+  SvdResult r;
+  Context *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  int slot = 0;
+  int k = 0;
+};
+SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
+                         const SvdOptions &opt, int slot);
+SvdResult svd_top_end(SvdPending &pending);
+
 // Singular values (descending, all min(m,n) of them) of a batch of equal
 // shaped matrices, all of them <= 128 x 128.  Result: device tensor
 // (batch, min(m,n)).

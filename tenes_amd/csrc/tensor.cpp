@@ -59,18 +59,32 @@ void Allocator::free(void *p, size_t bytes) {
   const size_t b = round(bytes);
   in_use_ -= b;
   cached_ += b;
-  free_.emplace(b, p);
+  if (defer_)
+    deferred_.emplace_back(b, p);
+  else
+    free_.emplace(b, p);
+}
+
+void Allocator::set_deferred(bool on) {
+  std::lock_guard<std::mutex> lk(mu_);
+  defer_ = on;
+  if (!on) {
+    for (auto &kv : deferred_) free_.emplace(kv.first, kv.second);
+    deferred_.clear();
+  }
 }
 
 void Allocator::release_cached() {
   std::lock_guard<std::mutex> lk(mu_);
   for (auto &kv : free_) (void)hipFree(kv.second);
-  free_.clear();
   cached_ = 0;
+  for (auto &kv : deferred_) cached_ += kv.first;
+  free_.clear();
 }
 
 Allocator::~Allocator() {
   for (auto &kv : free_) (void)hipFree(kv.second);
+  for (auto &kv : deferred_) (void)hipFree(kv.second);
 }
 
 Storage::~Storage() {
@@ -80,44 +94,112 @@ Storage::~Storage() {
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
-Context::Context(int device, hipStream_t stream)
-    : device_(device), stream_(stream), alloc_(device) {
+Context::Context(int device, hipStream_t stream) : device_(device), alloc_(device) {
   TN_HIP_CHECK(hipSetDevice(device));
-  if (stream_ == nullptr) {
-    TN_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  Lane main;
+  main.s = stream;
+  if (main.s == nullptr) {
+    TN_HIP_CHECK(hipStreamCreateWithFlags(&main.s, hipStreamNonBlocking));
     own_stream_ = true;
   }
   kern::linalg_init_device();
-  TN_HIP_CHECK(hipMalloc(&scal_, 4096 * sizeof(double)));
+  TN_HIP_CHECK(hipMalloc(&main.scal, 4096 * sizeof(double)));
+  lanes_.push_back(main);
+  TN_HIP_CHECK(hipEventCreateWithFlags(&fork_event_, hipEventDisableTiming));
 }
 
 Context::~Context() {
-  (void)hipStreamSynchronize(stream_);
+  for (auto &l : lanes_) (void)hipStreamSynchronize(l.s);
   for (auto &p : pending_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
   for (auto e : event_pool_) (void)hipEventDestroy(e);
   for (auto &kv : plans_) (void)hipFree(kv.second.d_tab);
-  if (ws_) alloc_.free(ws_, ws_bytes_);
-  if (scal_) (void)hipFree(scal_);
-  if (own_stream_) (void)hipStreamDestroy(stream_);
+  for (size_t i = 0; i < lanes_.size(); ++i) {
+    if (lanes_[i].ws) alloc_.free(lanes_[i].ws, lanes_[i].ws_bytes);
+    if (lanes_[i].scal) (void)hipFree(lanes_[i].scal);
+    if (i > 0 || own_stream_) (void)hipStreamDestroy(lanes_[i].s);
+  }
+  if (fork_event_) (void)hipEventDestroy(fork_event_);
+  if (pinned_) (void)hipHostFree(pinned_);
+}
+
+void Context::fork(int n) {
+  TN_REQUIRE(!forked_, "Context::fork: already forked");
+  TN_REQUIRE(n >= 1 && n <= 16, "Context::fork: 1 to 16 lanes");
+  while ((int)lanes_.size() < n + 1) {
+    Lane l;
+    // default priority: every stream of the library runs at one priority
+    TN_HIP_CHECK(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));
+    TN_HIP_CHECK(hipMalloc(&l.scal, 4096 * sizeof(double)));
+    lanes_.push_back(l);
+  }
+  TN_HIP_CHECK(hipEventRecord(fork_event_, lanes_[0].s));
+  for (int i = 1; i <= n; ++i)
+    TN_HIP_CHECK(hipStreamWaitEvent(lanes_[i].s, fork_event_, 0));
+  alloc_.set_deferred(true);
+  nforked_ = n;
+  forked_ = true;
+  cur_ = 0;
+}
+
+void Context::use_lane(int i) {
+  TN_REQUIRE(forked_ || i < 0, "Context::use_lane outside a fork");
+  TN_REQUIRE(i < nforked_, "Context::use_lane: no such lane");
+  cur_ = i < 0 ? 0 : i + 1;
+}
+
+void Context::join() {
+  TN_REQUIRE(forked_, "Context::join without fork");
+  for (int i = 1; i <= nforked_; ++i) {
+    hipEvent_t e = get_event();
+    TN_HIP_CHECK(hipEventRecord(e, lanes_[i].s));
+    TN_HIP_CHECK(hipStreamWaitEvent(lanes_[0].s, e, 0));
+    // the wait is enqueued: the event may be recycled once it has completed,
+    // so keep it until the next synchronisation point of the main stream
+    joined_events_.push_back(e);
+  }
+  if (joined_events_.size() > 256) {
+    for (auto ev : joined_events_) {
+      TN_HIP_CHECK(hipEventSynchronize(ev));
+      event_pool_.push_back(ev);
+    }
+    joined_events_.clear();
+  }
+  cur_ = 0;
+  forked_ = false;
+  nforked_ = 0;
+  alloc_.set_deferred(false);
 }
 
 void *Context::workspace(size_t bytes) {
-  if (bytes <= ws_bytes_) return ws_;
-  if (ws_) alloc_.free(ws_, ws_bytes_);
-  ws_bytes_ = std::max(bytes, ws_bytes_ * 2);
-  ws_ = alloc_.alloc(ws_bytes_);
-  return ws_;
+  Lane &l = lanes_[cur_];
+  if (bytes <= l.ws_bytes) return l.ws;
+  if (l.ws) alloc_.free(l.ws, l.ws_bytes);
+  l.ws_bytes = std::max(bytes, l.ws_bytes * 2);
+  l.ws = alloc_.alloc(l.ws_bytes);
+  return l.ws;
 }
 
 double *Context::scalars(int n) {
   TN_REQUIRE(n <= 4096, "scalar area exhausted");
-  return scal_;
+  return lanes_[cur_].scal;
 }
 
-void Context::sync() { TN_HIP_CHECK(hipStreamSynchronize(stream_)); }
+double *Context::pinned(int slot) {
+  TN_REQUIRE(slot >= 0 && slot < 64, "pinned slot out of range");
+  if (!pinned_) TN_HIP_CHECK(hipHostMalloc(&pinned_, 64 * 512 * sizeof(double)));
+  return pinned_ + (size_t)slot * 512;
+}
+
+void Context::sync() {
+  TN_HIP_CHECK(hipStreamSynchronize(stream()));
+  if (cur_ == 0 && !joined_events_.empty()) {
+    for (auto e : joined_events_) event_pool_.push_back(e);
+    joined_events_.clear();
+  }
+}
 
 hipEvent_t Context::get_event() {
   if (!event_pool_.empty()) {
@@ -137,19 +219,19 @@ void Context::set_profiling(bool on) {
 
 void Context::profile_begin(hipEvent_t *a) {
   *a = get_event();
-  TN_HIP_CHECK(hipEventRecord(*a, stream_));
+  TN_HIP_CHECK(hipEventRecord(*a, stream()));
 }
 
 void Context::profile_end(hipEvent_t a, const char *name, double flops) {
   hipEvent_t b = get_event();
-  TN_HIP_CHECK(hipEventRecord(b, stream_));
+  TN_HIP_CHECK(hipEventRecord(b, stream()));
   pending_.push_back({a, b, name, flops});
   if (pending_.size() >= 4096) flush_pending();
 }
 
 void Context::flush_pending() {
   if (pending_.empty()) return;
-  TN_HIP_CHECK(hipEventSynchronize(pending_.back().b));
+  for (auto &p : pending_) TN_HIP_CHECK(hipEventSynchronize(p.b));
   for (auto &p : pending_) {
     float ms = 0.f;
     TN_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
@@ -490,7 +572,11 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
     hipEvent_t a;
     ctx.profile_begin(&a);
     const char *name = kern::gemm_f64(g, w, ws, ctx.stream());
-    ctx.profile_end(a, name, flops);
+    // one report line per instantiation and problem shape
+    const std::string key = std::string(name) + "[" + std::to_string(p.M) +
+                            "x" + std::to_string(p.N) + "x" +
+                            std::to_string(p.K) + "]";
+    ctx.profile_end(a, key.c_str(), flops);
   } else {
     kern::gemm_f64(g, w, ws, ctx.stream());
   }

@@ -31,9 +31,12 @@ using Axes = std::vector<int>;
 
 class Context;
 
-// Size-bucketed caching allocator.  All work of a Context runs on one HIP
-// stream, so a freed block may be handed to the next allocation without a
-// device synchronisation: the stream orders the reuse after the last use.
+// Size-bucketed caching allocator.  Outside a fork all work of a Context
+// runs on its main HIP stream, so a freed block may be handed to the next
+// allocation without a device synchronisation: the stream orders the reuse
+// after the last use.  While the context is forked over several streams,
+// freed blocks are held back (deferred) until the join has ordered every
+// stream's work before the main stream's.
 class Allocator {
  public:
   explicit Allocator(int device) : device_(device) {}
@@ -41,6 +44,7 @@ class Allocator {
   void *alloc(size_t bytes);
   void free(void *p, size_t bytes);
   void release_cached();
+  void set_deferred(bool on);
   size_t bytes_in_use() const { return in_use_; }
   size_t bytes_cached() const { return cached_; }
 
@@ -48,6 +52,8 @@ class Allocator {
   static size_t round(size_t bytes);
   int device_;
   std::multimap<size_t, void *> free_;
+  std::vector<std::pair<size_t, void *>> deferred_;
+  bool defer_ = false;
   size_t in_use_ = 0, cached_ = 0;
   std::mutex mu_;
 };
@@ -95,13 +101,32 @@ class Context {
   Context &operator=(const Context &) = delete;
 
   int device() const { return device_; }
-  hipStream_t stream() const { return stream_; }
-  void set_stream(hipStream_t s) { stream_ = s; }
+  // The stream all operations issue to: the main stream, or while forked the
+  // side stream selected with use_lane().
+  hipStream_t stream() const { return lanes_[cur_].s; }
   Allocator &allocator() { return alloc_; }
 
-  // Scratch (grows on demand, stream ordered).
+  // Fork/join over side streams.  fork(n): n side streams ("lanes") start
+  // after all work issued so far on the main stream; use_lane(i) selects
+  // lane i (i < 0: the main stream) for the operations that follow; join():
+  // the main stream continues after all lanes' work and becomes current.
+  // Each lane has its own scratch areas.  Calls that return host values
+  // synchronise only the current stream.
+  void fork(int n);
+  void use_lane(int i);
+  void join();
+  bool forked() const { return forked_; }
+  // Upper bound on the lanes the CTM moves fork into (1: no overlap, every
+  // kernel on the main stream in issue order).
+  int max_lanes() const { return max_lanes_; }
+  void set_max_lanes(int n) { max_lanes_ = n < 1 ? 1 : (n > 16 ? 16 : n); }
+
+  // Scratch of the current stream (grows on demand, stream ordered).
   void *workspace(size_t bytes);
   double *scalars(int n);  // small device scalar area
+  // Pinned host buffer for asynchronous device-to-host copies of small
+  // results (slot < 64, at most 512 doubles each).
+  double *pinned(int slot);
 
   const ContractPlan &plan(const Shape &sa, const Shape &sb, const Axes &axa,
                            const Axes &axb, const Axes &perm);
@@ -139,13 +164,22 @@ class Context {
   void reset_kernel_stats();
 
  private:
+  struct Lane {
+    hipStream_t s = nullptr;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    double *scal = nullptr;
+  };
   int device_;
-  hipStream_t stream_;
   bool own_stream_ = false;
   Allocator alloc_;
-  void *ws_ = nullptr;
-  size_t ws_bytes_ = 0;
-  double *scal_ = nullptr;
+  std::vector<Lane> lanes_;  // [0] = main stream, then the side streams
+  int cur_ = 0;
+  int nforked_ = 0;
+  int max_lanes_ = 8;
+  bool forked_ = false;
+  hipEvent_t fork_event_ = nullptr;
+  double *pinned_ = nullptr;
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
   bool profiling_ = false;
@@ -156,6 +190,7 @@ class Context {
   };
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> event_pool_;
+  std::vector<hipEvent_t> joined_events_;
   std::map<std::string, KernelStat> kstats_;
   hipEvent_t get_event();
   void flush_pending();

@@ -42,6 +42,18 @@ def initial_tensors(n_sites: int, D: int, d: int, dirs, noise: float, seed: int 
     return out
 
 
+def random_tensors(n_sites: int, D: int, d: int = 2, seed: int = 11):
+    """Real site tensors (D, D, D, D, d) with i.i.d. U(-1, 1) entries, drawn
+    from one std::mt19937(seed) stream site by site in TeNeS's fill order
+    (first leg fastest), each scaled to unit Frobenius norm."""
+    gen = _mt19937(seed)
+    out = []
+    for _ in range(n_sites):
+        T = _uniform(gen, D ** 4 * d).reshape(d, D, D, D, D).transpose(4, 3, 2, 1, 0).copy()
+        out.append(T / np.linalg.norm(T))
+    return out
+
+
 def bond_hamiltonian(Jz=0.0, Jx=0.0, Jy=0.0, hx=0.0, hz=0.0, z=4):
     """H[in1, in2, out1, out2] = Jz SzSz + Jx SxSx + Jy Re(SySy)
     - (hx/z)(Sx 1 + 1 Sx) - (hz/z)(Sz 1 + 1 Sz)."""

@@ -363,3 +363,99 @@ def test_afh_real_goldens_on_gpu(gpu_ctx):
         L = 2
         xi = -L / np.log(abs(ev[1]) / abs(ev[0]))
         assert abs(xi - ref[2]) < 1e-10
+
+
+def _fixture_or_skip(name):
+    import golden_io as G
+    try:
+        return G.load(name)
+    except FileNotFoundError:
+        pytest.skip(f"fixture {name}.json not generated (tests/golden/make_oracle_runs.py)")
+
+
+def test_ctm_random_d4_chi64_30_sweeps(gpu_ctx):
+    """CTM-only case: 2x2, D=4, chi=64, unit-norm random site tensors, 30
+    sweeps with epsilon = 0; L2-normalised spectra of every C1..C4 against the
+    oracle's LAPACK-SVD run (tests/golden/ctm_random_d4_chi64.json) to 1e-10.
+    The spectrum of R decays slowly for random tensors (s_128/s_64 ~ 0.38, so
+    the range finder's subspace error after q iterations is ~0.38^(2q+1)):
+    measured worst spectrum differences after 30 sweeps are 2.5e-7 (q=4),
+    3.2e-10 (q=8) and 4.4e-15 (q=16); the test runs q=16."""
+    from tenes_amd import inputs
+
+    fx = _fixture_or_skip("ctm_random_d4_chi64")
+    Tn = inputs.random_tensors(4, fx["D"], 2, seed=fx["seed"])
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=fx["chi"], Max_CTM_Iteration=fx["sweeps"],
+                                                        CTM_Convergence_Epsilon=0.0, svd_power_iters=16))
+    assert n == fx["sweeps"]
+    worst = 0.0
+    for i in range(4):
+        for k in ("C1", "C2", "C3", "C4"):
+            ref = np.array(fx["spectra"][f"{k}{i}"])
+            got = genv.corner_spectrum(k, i)
+            assert got.shape == ref.shape
+            worst = max(worst, float(np.max(np.abs(got - ref))))
+    assert worst < 1e-10, worst
+
+
+def test_heisenberg_d5_chi100_energy(gpu_ctx):
+    """The benchmark problem (Heisenberg 2x2, D=5, chi=100) from the oracle's
+    own simple-update site tensors: the GPU's fixed-sweep CTM energy per site
+    and <Sz>, <Sx> of every site against the oracle's LAPACK-SVD run
+    (tests/golden/heisenberg_d5_chi100.json), default SVD settings."""
+    from tenes_amd import inputs
+
+    fx = _fixture_or_skip("heisenberg_d5_chi100")
+    D = fx["D"]
+    Tn = [np.array(t).reshape(D, D, D, D, 2) for t in fx["site_tensors"]]
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=fx["chi"], Max_CTM_Iteration=fx["sweeps"],
+                                                        CTM_Convergence_Epsilon=0.0))
+    assert n == fx["sweeps"]
+    H = inputs.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    E = 0.0
+    for s, leg in inputs.square_bonds(2, 2):
+        dx, dy = (1, 0) if leg == 2 else (0, 1)
+        E += genv.contract_two_sites_op12(s, dx, dy, H) / genv.contract_cluster(s, dx, dy)
+    E /= 4.0
+    assert abs(E - fx["energy_per_site"]) < 1e-10 * abs(fx["energy_per_site"]), (E, fx["energy_per_site"])
+    for i in range(4):
+        nrm = genv.contract_one_site(i)
+        assert abs(genv.contract_one_site(i, models.Sz) / nrm - fx["sz"][i]) < 1e-10
+        assert abs(genv.contract_one_site(i, models.Sx) / nrm - fx["sx"][i]) < 1e-10
+
+
+def test_tfi_plain_reference_config(gpu_ctx):
+    """Transverse-field Ising (Jz = -1, hx = 1), 2x2, D = 2, chi = 10: 1000
+    simple-update steps (tau = 0.01) from the ferro state on the GPU and
+    TeNeS's CTM driver (10 iterations, default convergence epsilon) -- energy
+    per site, <Sz> and <Sx> against the oracle running the same steps on the
+    same seeded inputs."""
+    from oracle import simple_update as SU
+
+    lat = SquareLattice(2, 2)
+    D = 2
+    Tn0 = SU.initial_tensors(lat, [2] * 4, [[D] * 4] * 4, [[1.0, 0.0]] * 4, [0.01] * 4, seed=11)
+    ham = models.bond_hamiltonian(Jz=-1.0, hx=1.0)
+    gate = models.evolution_twosite(ham, 0.01)
+    ops = [(s, leg, gate) for s, leg in models.square_bonds(lat)]
+    lam = [[np.ones(D) for _ in range(4)] for _ in range(4)]
+    Tn, lam = SU.simple_update_steps([t.copy() for t in Tn0], lam, lat, ops, 1000)
+    p = O.PEPSParameters(CHI=10, Max_CTM_Iteration=10)
+    env, count, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn0)
+    genv.simple_update(ops, 1000)
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=10, Max_CTM_Iteration=10))
+    assert n == count
+    E, mz = observables_oracle(env, Tn, lat, ham)
+    gE, gmz = observables_gpu(genv, lat, ham)
+    assert abs(gE - E) < 1e-10 * abs(E), (gE, E)
+    assert np.max(np.abs(gmz - mz)) < 1e-10
+    one = [{"name": "Sx", "group": 0, "site": i, "op": models.Sx} for i in range(4)]
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    gmx = [genv.contract_one_site(i, models.Sx) / genv.contract_one_site(i) for i in range(4)]
+    assert np.max(np.abs(np.array(gmx) - np.real([o[0][i] for i in range(4)]))) < 1e-10
