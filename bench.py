@@ -94,6 +94,19 @@ def load_traffic(kernel):
     return None
 
 
+def staggered_magnetization(env):
+    """|(1/N) sum_i parity(i) <S_i>| on the 2x2 cell (real states: S^y = 0)."""
+    from tenes_amd import inputs
+
+    par = [1, -1, -1, 1]
+    sz = sx = 0.0
+    for i in range(4):
+        nrm = env.contract_one_site(i)
+        sz += par[i] * env.contract_one_site(i, inputs.SZ) / nrm
+        sx += par[i] * env.contract_one_site(i, inputs.SX) / nrm
+    return float(np.hypot(sz, sx)) / 4.0
+
+
 def energy(env, H):
     from tenes_amd import inputs
 
@@ -192,7 +205,7 @@ def main():
     log(f"energy per site (FP64): {E:.15f}; GEMM share of timed region {gemm_ms / (dt * 1e3):.1%}")
     # accuracy against the oracle's FP64 (LAPACK SVD) run of the same problem:
     # a fresh environment, the oracle's sweep count, untimed
-    e_err = None
+    e_err = ms_err = xi_err = None
     if fx is not None:
         chk = ta.Environment(ctx, 2, 2)
         chk.set_site_tensors(Tn)
@@ -201,6 +214,14 @@ def main():
         e_err = abs(E_chk - fx["energy_per_site"]) / abs(fx["energy_per_site"])
         log(f"energy after {fx['sweeps']} sweeps: GPU {E_chk:.15f}, oracle {fx['energy_per_site']:.15f}, "
             f"rel err {e_err:.2e}")
+        ms = staggered_magnetization(chk)
+        ms_err = abs(ms - fx["staggered_magnetization"]) / fx["staggered_magnetization"]
+        c0 = fx["correlation"][0]
+        ev = chk.transfer_matrix_eigenvalues(c0["dir"], c0["fixed"], ta.tm_parameters(num_eigvals=10))
+        xi = -2.0 / np.log(abs(ev[1]) / abs(ev[0]))
+        xi_err = abs(xi - c0["xi"]) / c0["xi"]
+        log(f"staggered magnetisation {ms:.12f} (rel err {ms_err:.2e}); correlation length "
+            f"{xi:.12f} (rel err {xi_err:.2e}, 10 leading eigenvalues, chi^2 = {args.chi ** 2})")
     traffic = load_traffic(dom_name)
 
     cpu = None
@@ -224,7 +245,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: simple-update Heisenberg site tensors generated on the GPU (seeded)",
+        "data": ("synthetic: seeded simple-update Heisenberg site tensors (oracle run stored in "
+                 "tests/golden/heisenberg_d5_chi100.json)" if fx is not None else
+                 "synthetic: seeded simple-update Heisenberg site tensors generated on the GPU"),
         "config": {"workload": "CTMRG sweep, Heisenberg AFM 2x2 unit cell, D=5, d=2, chi=100, FP64",
                    "D": args.D, "chi": args.chi, "unit_cell": [2, 2], "su_steps": args.su_steps,
                    "projector": "corner", "svd": "GPU truncated FP64 SVD (randomized range finder + Jacobi)"},
@@ -234,6 +257,8 @@ def main():
         "cpu_baseline": cpu,
         "energy_per_site_fp64": E,
         "energy_rel_err_vs_fp64": e_err,
+        "staggered_magnetization_rel_err_vs_fp64": ms_err,
+        "correlation_length_rel_err_vs_fp64": xi_err,
         "gemm_tflops_overall": flops / dt / 1e12,
     }
     print(json.dumps(out), flush=True)

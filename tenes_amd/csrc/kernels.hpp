@@ -89,10 +89,11 @@ void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
            hipStream_t s);
 // Shifted Cholesky of a symmetric p x p matrix G (row-major, p <= 128)
 // using one workgroup: G + shift*I = R^T R, R upper triangular, shift =
-// shift_coef * trace(G) (computed on the device).  Also writes Rinv.
-// With rel_floor > 0, a pivot <= rel_floor * max diagonal drops its column
-// (rank-deficient input): that row of R is zero and Rinv has a unit pivot,
-// so Q = Y * Rinv keeps the (negligible) residual of the dropped column.
+// shift_coef * trace(G) (computed on the device), and Rinv = R^{-1}.
+// R may be nullptr (not written).  With rel_floor > 0, a pivot <= rel_floor
+// * max diagonal drops its column (rank-deficient input): that row of R is
+// zero and that column of Rinv is zero, so Q = Y * Rinv has a zero column
+// there (the other columns of Q do not depend on the dropped one).
 void chol_inv_upper(const double *G, int p, double shift_coef,
                     double rel_floor, double *R, double *Rinv, hipStream_t s);
 // One-sided (Hestenes) Jacobi SVD of a batch of p x p matrices with

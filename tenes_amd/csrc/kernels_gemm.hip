@@ -82,37 +82,38 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   const int M = g.M, N = g.N;
 
   // ---- loader geometry ----------------------------------------------------
-  // A tile: BM x BK.  MFAST: lanes walk m; else lanes walk k.
-  constexpr int A_KSTEP = A_MFAST ? (256 / BM) : 0;
-  constexpr int A_MSTEP = A_MFAST ? 0 : (256 / BK);
-  const int a_ml = A_MFAST ? (t % BM) : (t / BK);
-  const int a_kl = A_MFAST ? (t / BM) : (t % BK);
-  constexpr int B_KSTEP = B_NFAST ? (256 / BN) : 0;
-  constexpr int B_NSTEP = B_NFAST ? 0 : (256 / BK);
-  const int b_nl = B_NFAST ? (t % BN) : (t / BK);
-  const int b_kl = B_NFAST ? (t / BN) : (t % BK);
-
-  // Outer offsets are fixed for the whole K loop: keep them in registers.
-  int a_moff[A_MFAST ? 1 : A_PER];
-  bool a_mok[A_MFAST ? 1 : A_PER];
+  // Element p of this thread in the A tile (BM x BK) is e = t + 256 p of the
+  // tile in the lane-fast order: MFAST (consecutive lanes walk m, coalesced
+  // when A's m stride is 1) e = ml + BM kl, else (lanes walk k) e = kl + BK ml.
+  // Same for B with n.  (ml, kl) and the outer offsets are fixed for the
+  // whole K loop and kept in registers.
+  int a_ml[A_PER], a_kl[A_PER], a_moff[A_PER];
+  bool a_mok[A_PER];
 #pragma unroll
-  for (int p = 0; p < (A_MFAST ? 1 : A_PER); ++p) {
-    const int m = m0 + a_ml + p * A_MSTEP;
+  for (int p = 0; p < A_PER; ++p) {
+    const int e = t + 256 * p;
+    a_ml[p] = A_MFAST ? e % BM : e / BK;
+    a_kl[p] = A_MFAST ? e / BM : e % BK;
+    const int m = m0 + a_ml[p];
     a_mok[p] = m < M;
     a_moff[p] = a_mok[p] ? (g.a_m_stride ? m * g.a_m_stride : g.a_m[m]) : 0;
   }
-  int b_noff[B_NFAST ? 1 : B_PER];
-  bool b_nok[B_NFAST ? 1 : B_PER];
+  int b_nl[B_PER], b_kl[B_PER], b_noff[B_PER];
+  bool b_nok[B_PER];
 #pragma unroll
-  for (int p = 0; p < (B_NFAST ? 1 : B_PER); ++p) {
-    const int n = n0 + b_nl + p * B_NSTEP;
+  for (int p = 0; p < B_PER; ++p) {
+    const int e = t + 256 * p;
+    b_nl[p] = B_NFAST ? e % BN : e / BK;
+    b_kl[p] = B_NFAST ? e / BN : e % BK;
+    const int n = n0 + b_nl[p];
     b_nok[p] = n < N;
     b_noff[p] = b_nok[p] ? (g.b_n_stride ? n * g.b_n_stride : g.b_n[n]) : 0;
   }
 
-  // k of load p of this thread within the tile starting at k0
-  auto a_k_of = [&](int k0, int p) { return k0 + a_kl + (A_MFAST ? p * A_KSTEP : 0); };
-  auto b_k_of = [&](int k0, int p) { return k0 + b_kl + (B_NFAST ? p * B_KSTEP : 0); };
+  // k of load p of this thread within the tile starting at k0 (for the
+  // k-fast layouts all loads of a thread share one k)
+  auto a_k_of = [&](int k0, int p) { return k0 + a_kl[A_MFAST ? p : 0]; };
+  auto b_k_of = [&](int k0, int p) { return k0 + b_kl[B_NFAST ? p : 0]; };
   int ka_off[A_NK], kb_off[B_NK];  // table offsets of the next tile (!KA / !KB)
   auto fetch_koff = [&](int k0) {
     if (!KA) {
@@ -139,34 +140,24 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   auto load_tile = [&](int k0) {
 #pragma unroll
     for (int p = 0; p < A_PER; ++p) {
-      const int m_i = A_MFAST ? 0 : p;
       const int kk = a_k_of(k0, p);
-      oka[p] = a_mok[m_i] && kk < kend;
+      oka[p] = a_mok[p] && kk < kend;
       const int koff = KA ? kk * g.a_k_stride : ka_off[A_MFAST ? p : 0];
-      ra[p] = g.A[oka[p] ? a_moff[m_i] + koff : 0];
+      ra[p] = g.A[oka[p] ? a_moff[p] + koff : 0];
     }
 #pragma unroll
     for (int p = 0; p < B_PER; ++p) {
-      const int n_i = B_NFAST ? 0 : p;
       const int kk = b_k_of(k0, p);
-      okb[p] = b_nok[n_i] && kk < kend;
+      okb[p] = b_nok[p] && kk < kend;
       const int koff = KB ? kk * g.b_k_stride : kb_off[B_NFAST ? p : 0];
-      rb[p] = g.B[okb[p] ? b_noff[n_i] + koff : 0];
+      rb[p] = g.B[okb[p] ? b_noff[p] + koff : 0];
     }
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
-    for (int p = 0; p < A_PER; ++p) {
-      const int ml = a_ml + (A_MFAST ? 0 : p * A_MSTEP);
-      const int kl = a_kl + (A_MFAST ? p * A_KSTEP : 0);
-      As[buf][kl][ml] = oka[p] ? ra[p] : 0.0;
-    }
+    for (int p = 0; p < A_PER; ++p) As[buf][a_kl[p]][a_ml[p]] = oka[p] ? ra[p] : 0.0;
 #pragma unroll
-    for (int p = 0; p < B_PER; ++p) {
-      const int nl = b_nl + (B_NFAST ? 0 : p * B_NSTEP);
-      const int kl = b_kl + (B_NFAST ? p * B_KSTEP : 0);
-      Bs[buf][kl][nl] = okb[p] ? rb[p] : 0.0;
-    }
+    for (int p = 0; p < B_PER; ++p) Bs[buf][b_kl[p]][b_nl[p]] = okb[p] ? rb[p] : 0.0;
   };
 
   d4 acc[TM][TN];
@@ -266,12 +257,11 @@ Plan choose(int M, int N, int K) {
     p.kps = 0;
     return p;
   }
+  // Large outputs: 128 x 128 tiles.  (160 x 160 tiles would fill the 256
+  // CUs in one round at 2500 x 2500, but measured 1.03 ms against 0.78 ms:
+  // their 25 accumulator tiles per wave leave too few registers.)
   const int64_t tiles128 = (int64_t)ceil_div(M, 128) * ceil_div(N, 128);
-  if (M >= 128 && N >= 128 && tiles128 >= 200) {
-    p.BM = p.BN = 128;
-  } else {
-    p.BM = p.BN = 64;
-  }
+  p.BM = p.BN = (M >= 128 && N >= 128 && tiles128 >= 200) ? 128 : 64;
   const int64_t tiles = (int64_t)ceil_div(M, p.BM) * ceil_div(N, p.BN);
   // Split K when the tile grid cannot fill the 256 CUs and K is long.
   p.splits = 1;

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
     // has no range), Rinv = I so that Q = Y * Rinv stays zero.
     for (int r = r0; r < p; r += 4)
       for (int c = c0; c < p; c += 64) {
-        R[r * p + c] = 0.0;
+        if (R) R[r * p + c] = 0.0;
         Rinv[r * p + c] = (r == c) ? 1.0 : 0.0;
       }
     return;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
                         : ((r >> 4) == (c >> 4) ? Xd[r * kCholXdLd + (c & 15)]
                                                 : A[c * ld + r]);
         }
-        R[r * p + c] = rv;
+        if (R) R[r * p + c] = rv;
         Rinv[r * p + c] = xv;
       }
     }

@@ -84,7 +84,7 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
   const double u = 1.1102230246251565e-16;
   for (int pass = 0; pass < passes; ++pass) {
     Tensor G = tensordot(Y, Y, Axes{0}, Axes{0});  // p x p
-    Tensor R = empty(ctx, Shape{p, p});
+    Tensor R = R_out ? empty(ctx, Shape{p, p}) : Tensor();
     Tensor Ri = empty(ctx, Shape{p, p});
     // shifted CholeskyQR3: the first pass is shifted (stable for any
     // condition number of Y), the next two are plain CholeskyQR with a
@@ -92,8 +92,8 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
     const double coef =
         pass == 0 ? 11.0 * (double)(m * p + p * (p + 1)) * u : 0.0;
     const double rel_floor = pass == 0 ? 0.0 : 1e-13;
-    kern::chol_inv_upper(G.data(), (int)p, coef, rel_floor, R.data(),
-                         Ri.data(), ctx.stream());
+    kern::chol_inv_upper(G.data(), (int)p, coef, rel_floor,
+                         R_out ? R.data() : nullptr, Ri.data(), ctx.stream());
     Y = tensordot(Y, Ri, Axes{1}, Axes{0});
     if (R_out) Rtot = Rtot.defined() ? tensordot(R, Rtot, Axes{1}, Axes{0}) : R;
   }

@@ -402,9 +402,11 @@ def test_ctm_random_d4_chi64_30_sweeps(gpu_ctx):
 
 def test_heisenberg_d5_chi100_energy(gpu_ctx):
     """The benchmark problem (Heisenberg 2x2, D=5, chi=100) from the oracle's
-    own simple-update site tensors: the GPU's fixed-sweep CTM energy per site
-    and <Sz>, <Sx> of every site against the oracle's LAPACK-SVD run
-    (tests/golden/heisenberg_d5_chi100.json), default SVD settings."""
+    own simple-update site tensors: the GPU's fixed-sweep CTM energy per site,
+    <Sz>, <Sx> of every site, the staggered magnetisation and the 10 leading
+    transfer-matrix eigenvalues / correlation lengths (chi^2 = 10^4, Arnoldi)
+    against the oracle's LAPACK-SVD run (tests/golden/heisenberg_d5_chi100.json),
+    default SVD settings."""
     from tenes_amd import inputs
 
     fx = _fixture_or_skip("heisenberg_d5_chi100")
@@ -422,10 +424,28 @@ def test_heisenberg_d5_chi100_energy(gpu_ctx):
         E += genv.contract_two_sites_op12(s, dx, dy, H) / genv.contract_cluster(s, dx, dy)
     E /= 4.0
     assert abs(E - fx["energy_per_site"]) < 1e-10 * abs(fx["energy_per_site"]), (E, fx["energy_per_site"])
+    sz, sx = [], []
     for i in range(4):
         nrm = genv.contract_one_site(i)
-        assert abs(genv.contract_one_site(i, models.Sz) / nrm - fx["sz"][i]) < 1e-10
-        assert abs(genv.contract_one_site(i, models.Sx) / nrm - fx["sx"][i]) < 1e-10
+        sz.append(genv.contract_one_site(i, models.Sz) / nrm)
+        sx.append(genv.contract_one_site(i, models.Sx) / nrm)
+        assert abs(sz[-1] - fx["sz"][i]) < 1e-10
+        assert abs(sx[-1] - fx["sx"][i]) < 1e-10
+    par = [1, -1, -1, 1]
+    ms = np.hypot(np.dot(par, sz), np.dot(par, sx)) / 4.0
+    assert abs(ms - fx["staggered_magnetization"]) < 1e-10 * fx["staggered_magnetization"]
+    for c in fx["correlation"]:
+        ev = genv.transfer_matrix_eigenvalues(c["dir"], c["fixed"], ta.tm_parameters(num_eigvals=10))
+        ref = np.array(c["eigvals_re"]) + 1j * np.array(c["eigvals_im"])
+        xi, _ = TMo.correlation_length(ev, 2)
+        # leading eigenvalue: converged, 1e-10.  xi depends on |t1|, here a
+        # near-degenerate pair (0.32600207243, 0.32600207239) that one
+        # 50-vector Arnoldi pass (TeNeS's default) resolves to ~1e-9: 1e-8.
+        # The deeper Ritz values are not converged and follow rounding
+        # differences of the environment: 1e-6
+        assert abs(abs(ev[0]) - abs(ref[0])) < 1e-10 * abs(ref[0])
+        assert abs(xi - c["xi"]) < 1e-8 * c["xi"], (xi, c["xi"])
+        assert np.max(np.abs(np.abs(ev) - np.abs(ref)) / np.abs(ref[0])) < 1e-6
 
 
 def test_tfi_plain_reference_config(gpu_ctx):
