@@ -216,12 +216,17 @@ def main():
             f"rel err {e_err:.2e}")
         ms = staggered_magnetization(chk)
         ms_err = abs(ms - fx["staggered_magnetization"]) / fx["staggered_magnetization"]
-        c0 = fx["correlation"][0]
-        ev = chk.transfer_matrix_eigenvalues(c0["dir"], c0["fixed"], ta.tm_parameters(num_eigvals=10))
-        xi = -2.0 / np.log(abs(ev[1]) / abs(ev[0]))
-        xi_err = abs(xi - c0["xi"]) / c0["xi"]
-        log(f"staggered magnetisation {ms:.12f} (rel err {ms_err:.2e}); correlation length "
-            f"{xi:.12f} (rel err {xi_err:.2e}, 10 leading eigenvalues, chi^2 = {args.chi ** 2})")
+        # every correlation length of the fixture (both directions, both
+        # rows/columns); the worst relative error is reported
+        xi_err = 0.0
+        for c in fx["correlation"]:
+            ev = chk.transfer_matrix_eigenvalues(c["dir"], c["fixed"], ta.tm_parameters(num_eigvals=10))
+            xi = -2.0 / np.log(abs(ev[1]) / abs(ev[0]))
+            err = abs(xi - c["xi"]) / c["xi"]
+            xi_err = max(xi_err, err)
+            log(f"correlation length dir {c['dir']} row/col {c['fixed']}: {xi:.12f} (rel err {err:.2e}, "
+                f"10 leading eigenvalues, chi^2 = {args.chi ** 2})")
+        log(f"staggered magnetisation {ms:.12f} (rel err {ms_err:.2e})")
     traffic = load_traffic(dom_name)
 
     cpu = None

@@ -112,8 +112,18 @@ typedef struct tenes_peps_parameters {
   int Use_RSVD;
   double RSVD_Oversampling_factor;
   int print_level;
-  int svd_power_iters;  /* GPU range-finder iterations (full-SVD mode) */
+  int svd_power_iters;  /* GPU range-finder iterations from a Gaussian start
+                         * block (full-SVD mode) */
   int rsvd_power_iters; /* GPU range-finder iterations (Use_RSVD mode)  */
+  /* Warm start (no counterpart in the reference): each projector's range
+   * finder starts from the right Ritz basis the same projector (environment,
+   * move, coordinate, row) produced in its previous sweep, kept across
+   * tenes_calc_ctm_environment calls that continue an environment
+   * (initialize = 0), with svd_warm_power_iters iterations; start columns
+   * that are exactly zero (rank-deficient earlier sweeps) are replaced by
+   * Gaussian ones.  Defaults: 1, 8 (svd_power_iters: 8). */
+  int svd_warm_start;
+  int svd_warm_power_iters;
 } tenes_peps_parameters;
 /* Defaults of PEPS_Parameters.cpp:29-77. */
 void tenes_peps_parameters_default(tenes_peps_parameters *p);
@@ -198,6 +208,25 @@ int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
 int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
                                const double *op_source,
                                const double *op_target, double *out);
+/* Bond energy with the environment held fixed, and its gradient.
+ * E = (1/N_UNIT) sum_b <op12_b>, where <op12_b> is the op12 contraction of
+ * Contract_two_sites_{horizontal,vertical}_op12_iTPS_CTM
+ * (contract_itps_ctm/ctm.cpp:213-319) divided by the bond norm, i.e. the
+ * bond values of measure_twosite (twosite_obs.cpp:146-262) summed as the
+ * bond_hamiltonian energy of save_density (density.cpp:39-76); bond b is
+ * (source[b], dx[b], dy[b]) with |dx| + |dy| == 1 and its operator is the
+ * next (d_source d_target)^2 doubles of op12, row-major (source, target,
+ * source', target').  grad: NULL for the energy only, else N_UNIT tensors
+ * shaped like the site tensors, overwritten with dE/dTn_i (real tensors:
+ * the ket and the bra occurrence both counted).  The gradient has no
+ * counterpart in the reference (TeNeS optimises by imaginary-time
+ * evolution); it is the adjoint of the two-site reduced density matrices. */
+int tenes_env_bond_energy(tenes_env *env, int nbonds, const int *source,
+                          const int *dx, const int *dy, const double *op12,
+                          double *energy, tenes_tensor *const *grad);
+/* Copy a tensor into caller-owned device memory (size elements, the
+ * tensor's row-major order) and wait for the copy. */
+int tenes_tensor_to_device(const tenes_tensor *t, double *dst);
 /* Create_Environment_two_sites, core/full_update.cpp:51-83 */
 int tenes_create_environment_two_sites(const tenes_tensor *const *T,
                                        tenes_tensor **out);

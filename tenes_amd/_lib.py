@@ -30,6 +30,8 @@ class PepsParameters(C.Structure):
         ("print_level", C.c_int),
         ("svd_power_iters", C.c_int),
         ("rsvd_power_iters", C.c_int),
+        ("svd_warm_start", C.c_int),
+        ("svd_warm_power_iters", C.c_int),
     ]
 
 
@@ -93,6 +95,9 @@ _SIGS = {
     "tenes_env_contract_one_site": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p, c_dbl_p]),
     "tenes_env_contract_two_sites_op12": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
+    "tenes_env_bond_energy": (
+        C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_dbl_p, c_dbl_p, c_void_pp]),
+    "tenes_tensor_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tenes_env_contract_cluster": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p]),
     "tenes_create_environment_two_sites": (C.c_int, [C.POINTER(C.c_void_p), c_void_pp]),

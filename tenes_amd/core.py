@@ -123,6 +123,11 @@ class Tensor:
         self.ctx.check(self.ctx.lib.tenes_tensor_to_host(self.h, L.dptr(out)))
         return out
 
+    def copy_to_device(self, ptr: int):
+        """Copy into caller-owned device memory (a device pointer of the
+        same HIP runtime) and wait for the copy."""
+        self.ctx.check(self.ctx.lib.tenes_tensor_to_device(self.h, C.c_void_p(ptr)))
+
     def device_ptr(self) -> int:
         p = C.c_void_p()
         self.ctx.lib.tenes_tensor_device_ptr(self.h, C.byref(p))
@@ -135,6 +140,14 @@ class Tensor:
         except Exception:
             pass
         self.h = None
+
+
+def Tensor_zeros(ctx: Context, shape: Sequence[int]) -> Tensor:
+    """Zero-filled device tensor (tenes_tensor_zeros)."""
+    shp = np.array(shape, dtype=np.int64)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.tenes_tensor_zeros(ctx.h, len(shp), L.i64ptr(shp), C.byref(h)))
+    return Tensor(ctx, h)
 
 
 def tensordot(A: Tensor, B: Tensor, axA: Sequence[int], axB: Sequence[int],
@@ -335,6 +348,36 @@ class Environment:
         arr = (C.c_void_p * max(len(tens), 1))(*[t.h for t in tens])
         self.ctx.check(self.ctx.lib.tenes_env_simple_update(
             self.h, len(ops), L.iptr(src), L.iptr(leg), arr, nsteps, inverse_lambda_cut))
+
+    def bond_energy(self, bonds, grad=None):
+        """Energy per site of nearest-neighbour bond operators with the
+        environment held fixed (tenes_env_bond_energy).
+
+        bonds: [(source, dx, dy, op12 (d_source, d_target, d_source,
+        d_target))].  grad: None (energy only), True (returns host arrays
+        dE/dTn_i) or a list of N_UNIT device Tensors shaped like the site
+        tensors, overwritten in place.  Returns (E, grads or None)."""
+        src = np.array([b[0] for b in bonds], dtype=np.int32)
+        dx = np.array([b[1] for b in bonds], dtype=np.int32)
+        dy = np.array([b[2] for b in bonds], dtype=np.int32)
+        ops = np.concatenate([np.ascontiguousarray(b[3], dtype=np.float64).ravel() for b in bonds]) \
+            if bonds else np.zeros(1)
+        E = C.c_double()
+        out = None
+        arr = None
+        if grad is not None:
+            if grad is True:
+                out = [Tensor_zeros(self.ctx, self.get("Tn", i).shape) for i in range(self.N)]
+            else:
+                out = list(grad)
+            if len(out) != self.N:
+                raise ValueError("one gradient tensor per site expected")
+            arr = (C.c_void_p * self.N)(*[g.h for g in out])
+        self.ctx.check(self.ctx.lib.tenes_env_bond_energy(
+            self.h, len(bonds), L.iptr(src), L.iptr(dx), L.iptr(dy), L.dptr(ops), C.byref(E), arr))
+        if grad is True:
+            return E.value, [g.numpy() for g in out]
+        return E.value, out
 
     def get_lambda(self, site: int, leg: int) -> np.ndarray:
         out = np.zeros(4096, dtype=np.float64)

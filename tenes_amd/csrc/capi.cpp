@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/tenes_amd.h"
+#include "adjoint.hpp"
 #include "contract.hpp"
 #include "ctm.hpp"
 #include "linalg.hpp"
@@ -77,6 +78,8 @@ PEPS_Parameters to_params(const tenes_peps_parameters *p) {
   q.print_level = p->print_level;
   q.svd_power_iters = p->svd_power_iters;
   q.rsvd_power_iters = p->rsvd_power_iters;
+  q.svd_warm_start = p->svd_warm_start != 0;
+  q.svd_warm_power_iters = p->svd_warm_power_iters;
   TN_REQUIRE(q.CHI >= 1, "CHI must be positive");
   TN_REQUIRE(q.RSVD_Oversampling_factor >= 1.0,
              "RSVD oversampling factor must be >= 1");
@@ -149,7 +152,7 @@ Cluster cluster_of(const tenes::itps::SquareLattice &lat, int source, int dx,
 
 extern "C" {
 
-int tenes_abi_version(void) { return 100; }
+int tenes_abi_version(void) { return 202; }
 
 int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
   return guard(nullptr, [&] {
@@ -348,6 +351,8 @@ void tenes_peps_parameters_default(tenes_peps_parameters *p) {
   p->print_level = q.print_level;
   p->svd_power_iters = q.svd_power_iters;
   p->rsvd_power_iters = q.rsvd_power_iters;
+  p->svd_warm_start = q.svd_warm_start ? 1 : 0;
+  p->svd_warm_power_iters = q.svd_warm_power_iters;
 }
 
 int tenes_calc_projector_left_block(
@@ -427,6 +432,9 @@ int tenes_env_create(tenes_ctx *ctx, int LX, int LY, int skew,
 }
 
 int tenes_env_destroy(tenes_env *env) {
+  if (env && env->ctx && env->ctx->c)
+    env->ctx->c->forget_svd_starts(
+        tenes::itps::environment_id(env->arr[0]));
   delete env;
   return TENES_OK;
 }
@@ -589,6 +597,65 @@ int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
           a[4][right], a[5][right], a[6][right], a[6][left], a[7][left],
           a[8][left], a[8][right], o);
     }
+  });
+}
+
+int tenes_env_bond_energy(tenes_env *env, int nbonds, const int *source,
+                          const int *dx, const int *dy, const double *op12,
+                          double *energy, tenes_tensor *const *grad) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(energy != nullptr, "null output");
+    TN_REQUIRE(nbonds >= 0, "negative bond count");
+    TN_REQUIRE(nbonds == 0 || (source && dx && dy && op12), "null pointer");
+    auto &a = env->arr;
+    const int N = env->lat.N_UNIT;
+    std::vector<tenes::itps::core::BondTerm> terms(nbonds);
+    size_t off = 0;
+    for (int b = 0; b < nbonds; ++b) {
+      TN_REQUIRE(source[b] >= 0 && source[b] < N, "bad source site");
+      TN_REQUIRE(std::abs(dx[b]) + std::abs(dy[b]) == 1,
+                 "bond energy needs nearest-neighbour bonds");
+      Cluster c = cluster_of(env->lat, source[b], dx[b], dy[b]);
+      auto &t = terms[b];
+      t.vertical = c.nrow == 2;
+      t.s1 = c.idx[0][0];
+      t.s2 = t.vertical ? c.idx[1][0] : c.idx[0][1];
+      const int64_t d1 = a[8][t.s1].shape[4], d2 = a[8][t.s2].shape[4];
+      const double *o = op12 + off;
+      off += (size_t)(d1 * d2 * d1 * d2);
+      t.op.resize(d1 * d2 * d1 * d2);
+      const bool swap = t.s1 != source[b];  // given as (s2, s1, s2', s1')
+      for (int64_t p = 0; p < d1; ++p)
+        for (int64_t q = 0; q < d2; ++q)
+          for (int64_t r = 0; r < d1; ++r)
+            for (int64_t u = 0; u < d2; ++u)
+              t.op[((p * d2 + q) * d1 + r) * d2 + u] =
+                  swap ? o[((q * d1 + p) * d2 + u) * d1 + r]
+                       : o[((p * d2 + q) * d1 + r) * d2 + u];
+    }
+    if (grad) {
+      std::vector<tn::Tensor> G(N);
+      for (int i = 0; i < N; ++i) {
+        TN_REQUIRE(grad[i] != nullptr, "null gradient tensor");
+        G[i] = T(grad[i]);
+      }
+      *energy = tenes::itps::core::bond_energy(a, terms, N, &G);
+    } else {
+      *energy = tenes::itps::core::bond_energy(a, terms, N, nullptr);
+    }
+  });
+}
+
+int tenes_tensor_to_device(const tenes_tensor *t, double *dst) {
+  if (!t || !dst) return TENES_ERR_ARGUMENT;
+  return guard(t->owner, [&] {
+    const tn::Tensor &x = T(t);
+    hipStream_t s = x.ctx()->stream();
+    TN_HIP_CHECK(hipMemcpyAsync(dst, x.data(), x.size() * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+    TN_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
 

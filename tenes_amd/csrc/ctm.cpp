@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <map>
 
 #include "kernels.hpp"
@@ -178,15 +179,33 @@ constexpr int kSyncSlot = 61;  // pinned slot of the synchronous entry points
 // From the SVD of the (rows x cols) matrix `Rm`, build the oblique projector
 // pair: PU = Bblk * (VT_c^dagger s^-1/2), PL = Ablk * (U_c^* s^-1/2), both
 // transposed to (e_in, t, tc, cut) (ctm.cpp:290-301 / 466-475).
+// Projector identity (environment, move, coordinate, row) for the
+// warm-started range finder; nullptr: no warm start.
+using ProjKey = std::array<int64_t, 4>;
+
 void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
                       int64_t r1, int64_t c0, int64_t c1, int chi_cap,
                       const tensor &Ablk, const Axes &axA, const tensor &Bblk,
                       const Axes &axB, const PEPS_Parameters &pp, bool rsvd,
-                      int slot, ProjectorJob &job) {
+                      int slot, const ProjKey *key, ProjectorJob &job) {
   const int64_t nr = r0 * r1 * r0, nc = c0 * c1 * c0;
   job.kmax = (int)std::min<int64_t>(std::min<int64_t>(nr, nc), chi_cap);
+  tn::SvdOptions opts = svd_opts(pp, rsvd);
+  const bool warm = key != nullptr && pp.svd_warm_start && !rsvd;
+  std::array<int64_t, 5> wkey{};
+  if (warm) {
+    wkey = {(*key)[0], (*key)[1], (*key)[2], (*key)[3], nc};
+    auto it = ctx.svd_start.find(wkey);
+    if (it != ctx.svd_start.end()) {
+      opts.start = &it->second.first;
+      opts.start_sigma = &it->second.second;
+      opts.start_power_iters = pp.svd_warm_power_iters;
+    }
+  }
   job.sv = tn::svd_top_begin(ctx, tn::reshape(Rm, Shape{nr, nc}), job.kmax,
-                             svd_opts(pp, rsvd), slot);
+                             opts, slot);
+  if (warm && job.sv.r.Vbasis.defined())
+    ctx.svd_start[wkey] = {job.sv.r.Vbasis, job.sv.r.sbasis};
   job.Ablk = Ablk;
   job.Bblk = Bblk;
   job.axA = axA;
@@ -224,7 +243,8 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
 void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
                       const tensor &eT6, const tensor &eT7, const tensor &eT8,
                       const tensor &Tn1, const tensor &Tn4,
-                      const PEPS_Parameters &pp, int slot, ProjectorJob &job) {
+                      const PEPS_Parameters &pp, int slot, const ProjKey *key,
+                      ProjectorJob &job) {
   tn::Context &ctx = *C1.ctx();
   const int64_t e12 = eT1.shape[1];
   const int64_t e56 = eT6.shape[0];
@@ -250,7 +270,7 @@ void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
   // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273)
   tensor R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
   projectors_begin(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 1, 2}, LB,
-                   Axes{3, 4, 5}, pp, rsvd, slot, job);
+                   Axes{3, 4, 5}, pp, rsvd, slot, key, job);
 }
 
 // ctm.cpp:320-491 up to the SVD
@@ -259,7 +279,7 @@ void updown_blocks_begin(
     const tensor &eT1, const tensor &eT2, const tensor &eT3, const tensor &eT4,
     const tensor &eT5, const tensor &eT6, const tensor &eT7, const tensor &eT8,
     const tensor &Tn1, const tensor &Tn2, const tensor &Tn3, const tensor &Tn4,
-    const PEPS_Parameters &pp, int slot, ProjectorJob &job) {
+    const PEPS_Parameters &pp, int slot, const ProjKey *key, ProjectorJob &job) {
   tn::Context &ctx = *C1.ctx();
   const int64_t e34 = eT3.shape[1];
   const int64_t e78 = eT7.shape[1];
@@ -295,7 +315,7 @@ void updown_blocks_begin(
   tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{3, 4, 5});
   tensor M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
   projectors_begin(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2}, R2,
-                   Axes{0, 1, 2}, pp, rsvd, slot, job);
+                   Axes{0, 1, 2}, pp, rsvd, slot, key, job);
 }
 
 // The L projectors of a move are independent chains of small kernels, each
@@ -347,7 +367,8 @@ void Calc_projector_left_block(const tensor &C1, const tensor &C4,
                                const PEPS_Parameters &pp, tensor &PU,
                                tensor &PL) {
   ProjectorJob job;
-  left_block_begin(C1, C4, eT1, eT6, eT7, eT8, Tn1, Tn4, pp, kSyncSlot, job);
+  left_block_begin(C1, C4, eT1, eT6, eT7, eT8, Tn1, Tn4, pp, kSyncSlot, nullptr,
+                   job);
   projectors_finish(*C1.ctx(), job, pp);
   PU = job.PU;
   PL = job.PL;
@@ -362,7 +383,7 @@ void Calc_projector_updown_blocks(
     const PEPS_Parameters &pp, tensor &PU, tensor &PL) {
   ProjectorJob job;
   updown_blocks_begin(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, eT7, eT8,
-                      Tn1, Tn2, Tn3, Tn4, pp, kSyncSlot, job);
+                      Tn1, Tn2, Tn3, Tn4, pp, kSyncSlot, nullptr, job);
   projectors_finish(*C1.ctx(), job, pp);
   PU = job.PU;
   PL = job.PL;
@@ -405,17 +426,18 @@ void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
   run_projectors(
       ctx, L,
       [&](int iy, int slot, ProjectorJob &job) {
+        const ProjKey key{environment_id(C1), 0, ix, iy};
         const int i = lat.index(ix, iy);
         const int j = lat.right(i);
         const int k = lat.bottom(j);
         const int l = lat.left(k);
         if (pp.CTM_Projector_corner)
           left_block_begin(C1[i], C4[l], eTt[i], eTb[l], eTl[l], eTl[i], Tn[i],
-                           Tn[l], pp, slot, job);
+                           Tn[l], pp, slot, &key, job);
         else
           updown_blocks_begin(C1[i], C2[j], C3[k], C4[l], eTt[i], eTt[j],
                               eTr[j], eTr[k], eTb[k], eTb[l], eTl[l], eTl[i],
-                              Tn[i], Tn[j], Tn[k], Tn[l], pp, slot, job);
+                              Tn[i], Tn[j], Tn[k], Tn[l], pp, slot, &key, job);
       },
       pp, PUs, PLs);
   const std::vector<tensor> C1_bak = C1, C4_bak = C4, eTl_bak = eTl;
@@ -447,6 +469,7 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
   run_projectors(
       ctx, L,
       [&](int iy, int slot, ProjectorJob &job) {
+        const ProjKey key{environment_id(C1), 1, ix, iy};
         const int k = lat.index(ix, iy);
         const int l = lat.left(k);
         const int i = lat.top(l);
@@ -454,13 +477,13 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
         if (pp.CTM_Projector_corner)
           left_block_begin(C3[k], C2[j], eTb[k], eTt[j], eTr[j], eTr[k],
                            tn::transpose(Tn[k], rot), tn::transpose(Tn[j], rot),
-                           pp, slot, job);
+                           pp, slot, &key, job);
         else
           updown_blocks_begin(
               C3[k], C4[l], C1[i], C2[j], eTb[k], eTb[l], eTl[l], eTl[i],
               eTt[i], eTt[j], eTr[j], eTr[k], tn::transpose(Tn[k], rot),
               tn::transpose(Tn[l], rot), tn::transpose(Tn[i], rot),
-              tn::transpose(Tn[j], rot), pp, slot, job);
+              tn::transpose(Tn[j], rot), pp, slot, &key, job);
       },
       pp, PUs, PLs);
   const std::vector<tensor> C2_bak = C2, C3_bak = C3, eTr_bak = eTr;
@@ -494,6 +517,7 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
   run_projectors(
       ctx, L,
       [&](int ix, int slot, ProjectorJob &job) {
+        const ProjKey key{environment_id(C1), 2, iy, ix};
         const int j = lat.index(ix, iy);
         const int k = lat.bottom(j);
         const int l = lat.left(k);
@@ -501,13 +525,13 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
         if (pp.CTM_Projector_corner)
           left_block_begin(C2[j], C1[i], eTr[j], eTl[i], eTt[i], eTt[j],
                            tn::transpose(Tn[j], rot), tn::transpose(Tn[i], rot),
-                           pp, slot, job);
+                           pp, slot, &key, job);
         else
           updown_blocks_begin(
               C2[j], C3[k], C4[l], C1[i], eTr[j], eTr[k], eTb[k], eTb[l],
               eTl[l], eTl[i], eTt[i], eTt[j], tn::transpose(Tn[j], rot),
               tn::transpose(Tn[k], rot), tn::transpose(Tn[l], rot),
-              tn::transpose(Tn[i], rot), pp, slot, job);
+              tn::transpose(Tn[i], rot), pp, slot, &key, job);
       },
       pp, PUs, PLs);
   const std::vector<tensor> C1_bak = C1, C2_bak = C2, eTt_bak = eTt;
@@ -541,6 +565,7 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
   run_projectors(
       ctx, L,
       [&](int ix, int slot, ProjectorJob &job) {
+        const ProjKey key{environment_id(C1), 3, iy, ix};
         const int l = lat.index(ix, iy);
         const int i = lat.top(l);
         const int j = lat.right(i);
@@ -548,13 +573,13 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
         if (pp.CTM_Projector_corner)
           left_block_begin(C4[l], C3[k], eTl[l], eTr[k], eTb[k], eTb[l],
                            tn::transpose(Tn[l], rot), tn::transpose(Tn[k], rot),
-                           pp, slot, job);
+                           pp, slot, &key, job);
         else
           updown_blocks_begin(
               C4[l], C1[i], C2[j], C3[k], eTl[l], eTl[i], eTt[i], eTt[j],
               eTr[j], eTr[k], eTb[k], eTb[l], tn::transpose(Tn[l], rot),
               tn::transpose(Tn[i], rot), tn::transpose(Tn[j], rot),
-              tn::transpose(Tn[k], rot), pp, slot, job);
+              tn::transpose(Tn[k], rot), pp, slot, &key, job);
       },
       pp, PUs, PLs);
   const std::vector<tensor> C3_bak = C3, C4_bak = C4, eTb_bak = eTb;
@@ -622,7 +647,8 @@ bool Check_Convergence_CTM(const std::vector<tensor> &C1,
                            const std::vector<tensor> &C3_old,
                            const std::vector<tensor> &C4_old,
                            const PEPS_Parameters &pp,
-                           const SquareLattice &lat, double &sig_max) {
+                           const SquareLattice &lat, double &sig_max,
+                           double *sig_all) {
   const int N = lat.N_UNIT;
   std::vector<const tensor *> all;
   for (int i = 0; i < N; ++i) {
@@ -636,7 +662,7 @@ bool Check_Convergence_CTM(const std::vector<tensor> &C1,
     all.push_back(&C4_old[i]);
   }
   auto lam = spectra(all);
-  sig_max = 0.0;
+  std::vector<double> sigs(4 * (size_t)N);
   for (int i = 0; i < N; ++i) {
     for (int c = 0; c < 4; ++c) {
       const auto &ln = lam[8 * i + 2 * c];
@@ -648,13 +674,18 @@ bool Check_Convergence_CTM(const std::vector<tensor> &C1,
         const double o = j < lo.size() ? lo[j] : 0.0;
         sig += (ln[j] - o) * (ln[j] - o);
       }
-      sig = std::sqrt(sig);
-      if (sig > pp.CTM_Convergence_Epsilon) {
-        sig_max = sig;
-        return false;
-      } else if (sig > sig_max) {
-        sig_max = sig;
-      }
+      sigs[4 * i + c] = std::sqrt(sig);
+    }
+  }
+  if (sig_all) *sig_all = *std::max_element(sigs.begin(), sigs.end());
+  // the reference's order and early exit at the first corner above epsilon
+  sig_max = 0.0;
+  for (double sig : sigs) {
+    if (sig > pp.CTM_Convergence_Epsilon) {
+      sig_max = sig;
+      return false;
+    } else if (sig > sig_max) {
+      sig_max = sig;
     }
   }
   return true;
@@ -782,23 +813,29 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
                          const std::vector<tensor> &Tn,
                          const PEPS_Parameters &pp, const SquareLattice &lat,
                          bool initialize, double *sig_max_out) {
-  if (initialize) Initialize_CTM(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, pp, lat);
+  // a fresh environment forgets the warm-start bases of its projectors; a
+  // continued one keeps them
+  if (initialize) {
+    Tn[0].ctx()->forget_svd_starts(environment_id(C1));
+    Initialize_CTM(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, pp, lat);
+  }
   bool convergence = false;
   int count = 0;
   std::vector<tensor> C1_old = C1, C2_old = C2, C3_old = C3, C4_old = C4;
   double sig_max = 0.0;
   const int LX = lat.LX_noskew, LY = lat.LY_noskew;
   while (!convergence && count < pp.Max_CTM_Iteration) {
+    const PEPS_Parameters &ps = pp;
     for (int ix = 0; ix < LX; ++ix)
-      Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, pp, lat);
+      Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, ps, lat);
     for (int ix = 0; ix > -LX; --ix)
-      Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX, pp,
+      Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX, ps,
                  lat);
     for (int iy = 0; iy > -LY; --iy)
-      Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY, pp,
+      Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY, ps,
                lat);
     for (int iy = 0; iy < LY; ++iy)
-      Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, pp, lat);
+      Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, ps, lat);
     convergence = Check_Convergence_CTM(C1, C2, C3, C4, C1_old, C2_old,
                                         C3_old, C4_old, pp, lat, sig_max);
     count += 1;

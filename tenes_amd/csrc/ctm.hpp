@@ -28,9 +28,27 @@ struct PEPS_Parameters {
   double RSVD_Oversampling_factor = 2.0;
   int print_level = 1;
   // GPU truncated-SVD controls (no counterpart in the reference).
-  int svd_power_iters = 4;
+  // Range-finder iterations from a Gaussian start block (full-SVD mode).
+  int svd_power_iters = 8;
   int rsvd_power_iters = 1;
+  // Warm start: the range finder of each projector starts from the right
+  // Ritz basis the same projector (environment, move, coordinate, row)
+  // produced in its previous sweep, kept across Calc_CTM_Environment calls
+  // that continue an environment (initialize = false), and runs
+  // svd_warm_power_iters iterations.  Heisenberg D=5, chi=100, 12 sweeps:
+  // 8 iterations, warm or cold, put the correlation lengths at the level
+  // where any converged run lands (2.3e-8 from the exact-SVD run), warm at
+  // 90.6 instead of 96.3 ms per sweep; 4 iterations leave 3e-7 to 1.2e-6
+  // (seed dependent) at 72-76 ms per sweep.
+  bool svd_warm_start = true;
+  int svd_warm_power_iters = 8;
 };
+
+// Identity of an environment for the warm-start bases of its projectors:
+// the address of its C1 array (stable while the environment lives).
+inline int64_t environment_id(const std::vector<tn::Tensor> &C1) {
+  return (int64_t)(intptr_t)&C1;
+}
 
 // Restatement of SquareLattice (src/SquareLattice.{hpp,cpp}): site index
 // x + y*LX, neighbours 0:left 1:top 2:right 3:bottom, skew boundary.
@@ -121,7 +139,8 @@ bool Check_Convergence_CTM(const std::vector<tensor> &C1,
                            const std::vector<tensor> &C3_old,
                            const std::vector<tensor> &C4_old,
                            const PEPS_Parameters &peps_parameters,
-                           const SquareLattice &lattice, double &sig_max);
+                           const SquareLattice &lattice, double &sig_max,
+                           double *sig_all = nullptr);
 
 // Initial C/eT from the site tensors (ctm.cpp:984-1193).
 void Initialize_CTM(std::vector<tensor> &C1, std::vector<tensor> &C2,

@@ -87,6 +87,12 @@ void splitk_reduce(const double *P, int splits, int M, int N, double alpha,
 // Counter-based standard normal numbers (SplitMix64 hash + Box-Muller).
 void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
            hipStream_t s);
+// Columns of the n x p row-major X that are exactly zero, or whose
+// singular value sigma[j] <= rel * sigma[0] (device array of p values, may
+// be null), are refilled with normal numbers (seed, counter as randn).
+void refill_columns(double *X, int64_t n, int p, const double *sigma,
+                    double rel, uint64_t seed, uint64_t counter,
+                    hipStream_t s);
 // Shifted Cholesky of a symmetric p x p matrix G (row-major, p <= 128)
 // using one workgroup: G + shift*I = R^T R, R upper triangular, shift =
 // shift_coef * trace(G) (computed on the device), and Rinv = R^{-1}.

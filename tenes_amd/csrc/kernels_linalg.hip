@@ -48,6 +48,41 @@ __global__ void randn_kernel(double *p, int64_t n, uint64_t seed,
   }
 }
 
+// One workgroup per column j of the n x p row-major block X: a column whose
+// entries are all zero, or whose singular value sigma[j] is at most
+// rel * sigma[0] (sigma may be null), is refilled with normal numbers (same
+// generator and element numbering as randn_kernel).
+__global__ void refill_columns_kernel(double *X, int64_t n, int p,
+                                      const double *sigma, double rel,
+                                      uint64_t seed, uint64_t counter) {
+  __shared__ double part[256];
+  const int j = blockIdx.x;
+  const bool null_dir = sigma != nullptr && !(sigma[j] > rel * sigma[0]);
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = X[i * p + j];
+    acc += v * v;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (part[0] != 0.0 && !null_dir) return;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const int64_t e = i * p + j;
+    const uint64_t c = counter + (uint64_t)(e >> 1);
+    const uint64_t h1 = splitmix64(seed ^ (2 * c));
+    const uint64_t h2 = splitmix64(seed ^ (2 * c + 1));
+    const double u1 = ((double)(h1 >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    const double r = sqrt(-2.0 * log(u1));
+    const double th = 6.283185307179586476925286766559 * u2;
+    X[e] = (e & 1) ? r * sin(th) : r * cos(th);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Shifted Cholesky + inverse of the upper factor: blocked, one workgroup of
 // 4 waves, the padded matrix resident in LDS.
@@ -353,13 +388,17 @@ __device__ __forceinline__ double sum16(double v) {
 // Round-robin (circle method) pairing for pe (even) players, round s:
 // pair 0 = (s, pe-1); pair k>0 = ((s+k) mod (pe-1), (s-k) mod (pe-1)).
 __device__ __forceinline__ void rr_pair(int s, int k, int pe, int &i, int &j) {
+  // 0 <= s, k < q: the residues mod q need one conditional subtraction each
+  // (no integer division in the rotation loops)
   const int q = pe - 1;
   if (k == 0) {
     i = s;
     j = q;
   } else {
-    i = (s + k) % q;
-    j = (s - k + q) % q;
+    i = s + k;
+    if (i >= q) i -= q;
+    j = s - k;
+    if (j < 0) j += q;
   }
   if (i > j) {
     const int t = i;
@@ -528,6 +567,7 @@ __global__ __launch_bounds__(64) void jacobi_accum_kernel(const double *rot,
     }
   };
   if (nchunks > 0) fetch(0);
+  int s = 0;  // round-robin step of round st: st mod (pe - 1)
   for (int ch = 0; ch < nchunks; ++ch) {
     __syncthreads();  // the previous chunk has been replayed
 #pragma unroll
@@ -539,8 +579,7 @@ __global__ __launch_bounds__(64) void jacobi_accum_kernel(const double *rot,
     if (ch + 1 < nchunks) fetch(ch + 1);
     const int st0 = ch * kAccChunk;
     const int st1 = min(ns, st0 + kAccChunk);
-    for (int st = st0; st < st1; ++st) {
-      const int s = st % (pe - 1);
+    for (int st = st0; st < st1; ++st, s = (s + 1 == pe - 1) ? 0 : s + 1) {
       const double *cst = cs + (st - st0) * npairs * 2;
       for (int k = lane; k < npairs; k += 64) {
         int i, j;
@@ -601,6 +640,15 @@ void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(randn_kernel, dim3((int)g), dim3(256), 0, s, p, n, seed,
                      counter);
+  TN_HIP_CHECK(hipGetLastError());
+}
+
+void refill_columns(double *X, int64_t n, int p, const double *sigma,
+                    double rel, uint64_t seed, uint64_t counter,
+                    hipStream_t s) {
+  if (n <= 0 || p <= 0) return;
+  hipLaunchKernelGGL(refill_columns_kernel, dim3(p), dim3(256), 0, s, X, n, p,
+                     sigma, rel, seed, counter);
   TN_HIP_CHECK(hipGetLastError());
 }
 

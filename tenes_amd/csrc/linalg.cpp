@@ -114,7 +114,7 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
 
   // A ~= Lq * X * Rq^T with orthonormal Lq (m x r), Rq (n x c); X is r x c.
   Tensor Lq, Rq, X;
-  bool have_L = false, have_R = false;
+  bool have_L = false, have_R = false, randomized = false;
   if (m <= kJac && n <= kJac) {
     X = A;
   } else if (n <= kJac) {
@@ -130,16 +130,31 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     X = transpose(Rf, Axes{1, 0});
   } else {
     const int p = kJac;
-    Tensor Om = empty(ctx, Shape{n, p});
-    kern::randn(Om.data(), Om.size(), ctx.seed,
-                ctx.next_rng_counter((uint64_t)Om.size()), s);
+    const bool warm = opt.start && opt.start->defined() && opt.start->rank() == 2 &&
+                      opt.start->shape[0] == n && opt.start->shape[1] == p;
+    Tensor Om;
+    if (warm) {
+      // Ritz directions of a numerically null singular value (an earlier,
+      // lower-rank matrix) span nothing new and would keep the block in the
+      // old range: give them fresh Gaussian directions
+      Om = clone(*opt.start);
+      const bool sv = opt.start_sigma && opt.start_sigma->defined() &&
+                      opt.start_sigma->size() == p;
+      kern::refill_columns(Om.data(), n, p, sv ? opt.start_sigma->data() : nullptr,
+                           1e-12, ctx.seed,
+                           ctx.next_rng_counter((uint64_t)Om.size()), s);
+    } else {
+      Om = empty(ctx, Shape{n, p});
+      kern::randn(Om.data(), Om.size(), ctx.seed,
+                  ctx.next_rng_counter((uint64_t)Om.size()), s);
+    }
     // subspace iteration: the intermediate blocks only need every direction
     // normalised to a well-conditioned basis (shifted + one plain CholeskyQR
     // pass: cond ~ 1 + O(1e-2)); a single shifted pass is not enough, it
     // damps the directions below sqrt(shift) instead of normalising them and
     // the small kept singular directions die out over the iterations.  The
     // final range basis gets the full three passes.
-    const int q = std::max(0, opt.power_iters);
+    const int q = std::max(0, warm ? opt.start_power_iters : opt.power_iters);
     Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr,
                               q == 0 ? 3 : 2);
     for (int it = 0; it < q; ++it) {
@@ -151,7 +166,7 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     Tensor Bt = tensordot(A, Q, Axes{0}, Axes{0});  // n x p = A^T Q
     Rq = orthonormalize(ctx, Bt, &R2);              // Bt = Rq R2
     Lq = Q;
-    have_L = have_R = true;
+    have_L = have_R = randomized = true;
     X = transpose(R2, Axes{1, 0});  // B = Q^T A = R2^T Rq^T
   }
 
@@ -166,13 +181,21 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
   Tensor What = empty(ctx, Shape{rows, k});
   kern::gather_cols(jo.W.data(), 1, rows, jo.perm->ptr, jo.sigma_sorted.data(),
                     rows, k, What.data(), s);
-  Tensor Jp = empty(ctx, Shape{cols, k});
-  kern::gather_cols(jo.J.data(), cols, 1, jo.perm->ptr, nullptr, cols, k,
+  // the randomized path keeps all right Ritz vectors (a warm start block)
+  const int kj = randomized ? cols : k;
+  Tensor Jp = empty(ctx, Shape{cols, kj});
+  kern::gather_cols(jo.J.data(), cols, 1, jo.perm->ptr, nullptr, cols, kj,
                     Jp.data(), s);
   SvdPending pd;
   SvdResult &r = pd.r;
   r.U = have_L ? tensordot(Lq, What, Axes{1}, Axes{0}) : What;
-  r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
+  if (randomized) {
+    r.Vbasis = tensordot(Rq, Jp, Axes{1}, Axes{0});
+    r.sbasis = jo.sigma_sorted;
+    r.V = kj == k ? r.Vbasis : slice(r.Vbasis, 1, 0, k);
+  } else {
+    r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
+  }
   r.s_dev = slice(jo.sigma_sorted, 0, 0, k);
   // pinned slot: k <= 128 singular values, then the Jacobi round count
   double *host = ctx.pinned(slot);

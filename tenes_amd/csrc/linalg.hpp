@@ -11,6 +11,14 @@ struct SvdOptions {
   int power_iters = 4;        // subspace iterations for the range finder
   int max_sweeps = 30;        // Jacobi sweep cap
   double jacobi_tol = 0.0;    // 0: choose from the dimension
+  // Optional start block of the range finder (n x 128, orthonormal columns,
+  // e.g. a previous SvdResult::Vbasis); used when its shape fits, replacing
+  // the Gaussian block, with start_power_iters iterations.
+  const Tensor *start = nullptr;
+  // Singular values belonging to the start columns (p values, descending);
+  // columns with a value <= 1e-12 of the first are replaced by Gaussian ones.
+  const Tensor *start_sigma = nullptr;
+  int start_power_iters = 2;
 };
 
 struct SvdResult {
@@ -18,6 +26,10 @@ struct SvdResult {
   Tensor V;                 // (n, k): right singular vectors as columns
   Tensor s_dev;             // (k): singular values, descending
   std::vector<double> s;    // host copy of s_dev
+  // Randomized path only: all 128 right Ritz vectors (n x 128), descending,
+  // and their 128 singular values.
+  Tensor Vbasis;
+  Tensor sbasis;
 };
 
 // Leading k singular triplets of the m x n matrix A (row-major 2-D tensor).

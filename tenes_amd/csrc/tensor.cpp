@@ -110,6 +110,9 @@ Context::Context(int device, hipStream_t stream) : device_(device), alloc_(devic
 
 Context::~Context() {
   for (auto &l : lanes_) (void)hipStreamSynchronize(l.s);
+  // tensors held by the context return their storage to alloc_, which is
+  // destroyed before the public members: release them while it is alive
+  svd_start.clear();
   for (auto &p : pending_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -171,6 +174,15 @@ void Context::join() {
   forked_ = false;
   nforked_ = 0;
   alloc_.set_deferred(false);
+}
+
+void Context::forget_svd_starts(int64_t environment) {
+  for (auto it = svd_start.begin(); it != svd_start.end();) {
+    if (it->first[0] == environment)
+      it = svd_start.erase(it);
+    else
+      ++it;
+  }
 }
 
 void *Context::workspace(size_t bytes) {

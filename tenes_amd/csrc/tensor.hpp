@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -147,6 +148,10 @@ class Context {
   // (one round = one pass over p/2 disjoint column pairs)
   int64_t svd_calls = 0;
   int64_t jacobi_rounds = 0;
+  // Range-finder start blocks of the CTM projectors (Ritz basis, singular
+  // values), keyed by (environment, move, coordinate, row, n).
+  std::map<std::array<int64_t, 5>, std::pair<Tensor, Tensor>> svd_start;
+  void forget_svd_starts(int64_t environment);
 
   // Optional per-launch timing of the GEMM kernels with HIP events recorded
   // on this context's stream: per kernel instantiation, launch count, summed
