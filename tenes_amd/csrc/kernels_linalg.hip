@@ -85,7 +85,7 @@ __global__ void refill_columns_kernel(double *X, int64_t n, int p,
 
 // ---------------------------------------------------------------------------
 // Shifted Cholesky + inverse of the upper factor: blocked, one workgroup of
-// 4 waves, the padded matrix resident in LDS.
+// 4 waves, the padded matrix resident in LDS, diagonally scaled first.
 //
 // Blocks of 16.  Per block column: wave 0 factors the 16 x 16 diagonal block
 // (lane l / register t holds element (l/16 + 4t, l%16); the pivot row is
@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
     double *Rinv) {
   extern __shared__ double lds[];
   __shared__ int defic[kMaxDim];
+  __shared__ double s_dg[kMaxDim], s_scl[kMaxDim];
   __shared__ double s_shift, s_thr;
   __shared__ int s_zero;
   const int pp = (p + 15) / 16 * 16;  // padded size (identity padding)
@@ -155,6 +156,26 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
         if (r < pp && c < pp) A[r * ld + c] = v[u][h];
       }
   }
+  __syncthreads();
+  // Diagonal (Jacobi) scaling G -> D G D, D = diag(G)^{-1/2}: the factor of
+  // the scaled matrix is R~ with R = R~ D^{-1}, R^{-1} = D R~^{-1}.  Column
+  // norms no longer enter the conditioning (a block of nearly orthogonal
+  // columns of very different lengths factors as well as an orthonormal
+  // one); zero columns keep a zero diagonal.
+  for (int i = tid; i < pp; i += kCholThreads) {
+    const double g = A[i * ld + i];
+    const double dg = i < p ? (g > 0.0 ? sqrt(g) : 0.0) : 1.0;
+    s_dg[i] = dg;
+    s_scl[i] = dg > 0.0 ? 1.0 / dg : 0.0;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int u = 0; u < 32; ++u)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = r0 + 4 * u, c = c0 + 64 * h;
+      if (r < pp && c < pp) A[r * ld + c] = (A[r * ld + c] * s_scl[r]) * s_scl[c];
+    }
   __syncthreads();
   if (w == 0) {
     double tr = 0.0, mx = 0.0;
@@ -357,8 +378,8 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
                         : ((r >> 4) == (c >> 4) ? Xd[r * kCholXdLd + (c & 15)]
                                                 : A[c * ld + r]);
         }
-        if (R) R[r * p + c] = rv;
-        Rinv[r * p + c] = xv;
+        if (R) R[r * p + c] = rv * s_dg[c];
+        Rinv[r * p + c] = xv * s_scl[r];
       }
     }
 }

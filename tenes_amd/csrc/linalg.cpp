@@ -74,7 +74,7 @@ void run_jacobi(Context &ctx, Tensor &W, int rows, int cols, bool want_J,
 }  // namespace
 
 Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
-                      int passes) {
+                      int passes, bool shift_first) {
   TN_REQUIRE(Y_in.rank() == 2, "orthonormalize: 2-D input expected");
   TN_REQUIRE(passes >= 1 && passes <= 3, "orthonormalize: 1 to 3 passes");
   const int64_t m = Y_in.shape[0], p = Y_in.shape[1];
@@ -88,10 +88,13 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
     Tensor Ri = empty(ctx, Shape{p, p});
     // shifted CholeskyQR3: the first pass is shifted (stable for any
     // condition number of Y), the next two are plain CholeskyQR with a
-    // relative pivot floor that drops numerically dependent columns.
+    // relative pivot floor that drops numerically dependent columns.  The
+    // Gram matrix is diagonally scaled in the kernel, so plain passes are
+    // enough when the columns are nearly orthogonal whatever their lengths.
+    const bool shifted = pass == 0 && shift_first;
     const double coef =
-        pass == 0 ? 11.0 * (double)(m * p + p * (p + 1)) * u : 0.0;
-    const double rel_floor = pass == 0 ? 0.0 : 1e-13;
+        shifted ? 11.0 * (double)(m * p + p * (p + 1)) * u : 0.0;
+    const double rel_floor = shifted ? 0.0 : 1e-13;
     kern::chol_inv_upper(G.data(), (int)p, coef, rel_floor,
                          R_out ? R.data() : nullptr, Ri.data(), ctx.stream());
     Y = tensordot(Y, Ri, Axes{1}, Axes{0});
@@ -155,16 +158,23 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     // the small kept singular directions die out over the iterations.  The
     // final range basis gets the full three passes.
     const int q = std::max(0, warm ? opt.start_power_iters : opt.power_iters);
+    // A warm block holds Ritz vectors: its images under A and A^T stay
+    // nearly orthogonal columns of very different lengths, which one plain
+    // pass on the scaled Gram matrix orthonormalises (two for the final
+    // bases); a Gaussian block mixes all directions and needs the shifted
+    // passes.
+    const int mid = warm ? 1 : 2, fin = warm ? 2 : 3;
     Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr,
-                              q == 0 ? 3 : 2);
+                              q == 0 ? fin : mid, !warm);
     for (int it = 0; it < q; ++it) {
-      Tensor Z = orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}), nullptr, 2);
+      Tensor Z = orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}), nullptr,
+                                mid, !warm);
       Q = orthonormalize(ctx, tensordot(A, Z, Axes{1}, Axes{0}), nullptr,
-                         it + 1 == q ? 3 : 2);
+                         it + 1 == q ? fin : mid, !warm);
     }
     Tensor R2;
     Tensor Bt = tensordot(A, Q, Axes{0}, Axes{0});  // n x p = A^T Q
-    Rq = orthonormalize(ctx, Bt, &R2);              // Bt = Rq R2
+    Rq = orthonormalize(ctx, Bt, &R2, fin, !warm);  // Bt = Rq R2
     Lq = Q;
     have_L = have_R = randomized = true;
     X = transpose(R2, Axes{1, 0});  // B = Q^T A = R2^T Rq^T

@@ -62,12 +62,17 @@ SvdResult svd_top_end(SvdPending &pending);
 Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats);
 
 // Orthonormal basis of the column space of Y (m x p row-major) by shifted
-// CholeskyQR3.  If R != nullptr the p x p upper factor (Y = Q R) is returned.
-// passes = 2 (shifted + one plain pass) leaves every kept column normalised
-// with cond(Q) ~ 1 + O(1e-2), enough for the intermediate blocks of a
-// subspace iteration; passes = 1 (shifted only) damps the directions below
-// sqrt(shift) instead of normalising them.  Numerically dependent columns
-// (relative pivot <= 1e-13 in the plain passes) come out as zero columns.
-Tensor orthonormalize(Context &ctx, const Tensor &Y, Tensor *R, int passes = 3);
+// CholeskyQR3 on the diagonally scaled Gram matrix.  If R != nullptr the
+// p x p upper factor (Y = Q R) is returned.  passes = 2 (shifted + one plain
+// pass) leaves every kept column normalised with cond(Q) ~ 1 + O(1e-2),
+// enough for the intermediate blocks of a subspace iteration; passes = 1
+// (shifted only) damps the directions below sqrt(shift) instead of
+// normalising them.  shift_first = false makes every pass plain: for blocks
+// whose columns are already nearly orthogonal (e.g. images of Ritz vectors),
+// where the scaled Gram matrix is well conditioned.  Numerically dependent
+// columns (relative pivot <= 1e-13 in the plain passes) come out as zero
+// columns.
+Tensor orthonormalize(Context &ctx, const Tensor &Y, Tensor *R, int passes = 3,
+                      bool shift_first = true);
 
 }  // namespace tn
