@@ -172,6 +172,9 @@ struct ProjectorJob {
   int64_t r0 = 0, r1 = 0, c0 = 0, c1 = 0;
   int kmax = 0;
   tn::SvdPending sv;
+  // warm-start entry to update with the Ritz-value range once known
+  bool warm_store = false;
+  std::array<int64_t, 5> wkey{};
 };
 
 constexpr int kSyncSlot = 61;  // pinned slot of the synchronous entry points
@@ -197,15 +200,19 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
     wkey = {(*key)[0], (*key)[1], (*key)[2], (*key)[3], nc};
     auto it = ctx.svd_start.find(wkey);
     if (it != ctx.svd_start.end()) {
-      opts.start = &it->second.first;
-      opts.start_sigma = &it->second.second;
+      opts.start = &it->second.V;
+      opts.start_sigma = &it->second.s;
+      opts.start_range = it->second.range;
       opts.start_power_iters = pp.svd_warm_power_iters;
     }
   }
   job.sv = tn::svd_top_begin(ctx, tn::reshape(Rm, Shape{nr, nc}), job.kmax,
                              opts, slot);
-  if (warm && job.sv.r.Vbasis.defined())
-    ctx.svd_start[wkey] = {job.sv.r.Vbasis, job.sv.r.sbasis};
+  job.warm_store = warm && job.sv.r.Vbasis.defined();
+  if (job.warm_store) {
+    ctx.svd_start[wkey] = {job.sv.r.Vbasis, job.sv.r.sbasis, 0.0};
+    job.wkey = wkey;
+  }
   job.Ablk = Ablk;
   job.Bblk = Bblk;
   job.axA = axA;
@@ -220,6 +227,13 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
                        const PEPS_Parameters &pp) {
   if (job.done) return;
   tn::SvdResult sv = tn::svd_top_end(job.sv);
+  if (job.warm_store && !sv.s_all.empty()) {
+    auto it = ctx.svd_start.find(job.wkey);
+    if (it != ctx.svd_start.end())
+      it->second.range = sv.s_all.back() > 0.0
+                             ? sv.s_all.front() / sv.s_all.back()
+                             : 0.0;
+  }
   std::vector<double> s_c;
   const int cut =
       truncate_spectrum(sv.s, std::min<int>(job.kmax, (int)sv.s.size()),

@@ -166,9 +166,12 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     const int mid = warm ? 1 : 2, fin = warm ? 2 : 3;
     Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr,
                               q == 0 ? fin : mid, !warm);
+    const bool skip_z =
+        warm && opt.start_range > 0.0 && opt.start_range < 1e7;
     for (int it = 0; it < q; ++it) {
-      Tensor Z = orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}), nullptr,
-                                mid, !warm);
+      Tensor Z = skip_z ? tensordot(A, Q, Axes{0}, Axes{0})
+                        : orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}),
+                                         nullptr, mid, !warm);
       Q = orthonormalize(ctx, tensordot(A, Z, Axes{1}, Axes{0}), nullptr,
                          it + 1 == q ? fin : mid, !warm);
     }
@@ -207,9 +210,10 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
   }
   r.s_dev = slice(jo.sigma_sorted, 0, 0, k);
-  // pinned slot: k <= 128 singular values, then the Jacobi round count
+  // pinned slot: all <= 128 singular values, then the Jacobi round count
   double *host = ctx.pinned(slot);
-  TN_HIP_CHECK(hipMemcpyAsync(host, r.s_dev.data(), k * sizeof(double),
+  TN_HIP_CHECK(hipMemcpyAsync(host, jo.sigma_sorted.data(),
+                              (size_t)cols * sizeof(double),
                               hipMemcpyDeviceToHost, s));
   TN_HIP_CHECK(hipMemcpyAsync(host + 511, jo.nsteps->ptr, sizeof(int),
                               hipMemcpyDeviceToHost, s));
@@ -217,6 +221,7 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
   pd.stream = s;
   pd.slot = slot;
   pd.k = k;
+  pd.nall = cols;
   return pd;
 }
 
@@ -225,6 +230,7 @@ SvdResult svd_top_end(SvdPending &pd) {
   TN_HIP_CHECK(hipStreamSynchronize(pd.stream));
   const double *host = pd.ctx->pinned(pd.slot);
   pd.r.s.assign(host, host + pd.k);
+  pd.r.s_all.assign(host, host + pd.nall);
   int rounds = 0;
   std::memcpy(&rounds, host + 511, sizeof(int));
   pd.ctx->svd_calls++;

@@ -19,6 +19,12 @@ struct SvdOptions {
   // columns with a value <= 1e-12 of the first are replaced by Gaussian ones.
   const Tensor *start_sigma = nullptr;
   int start_power_iters = 2;
+  // Largest over smallest of those singular values (0: unknown).  Below
+  // 1e7 the warm iterations skip the orthonormalisation between the A^T and
+  // the A product: (A A^T) then amplifies the rounding left in a column
+  // towards the dominant directions by at most range^2 * eps ~ 1e-2, which
+  // the next orthonormalisation removes.
+  double start_range = 0.0;
 };
 
 struct SvdResult {
@@ -26,6 +32,9 @@ struct SvdResult {
   Tensor V;                 // (n, k): right singular vectors as columns
   Tensor s_dev;             // (k): singular values, descending
   std::vector<double> s;    // host copy of s_dev
+  // host copy of all singular values of the projected problem (the 128 Ritz
+  // values on the randomized path), descending
+  std::vector<double> s_all;
   // Randomized path only: all 128 right Ritz vectors (n x 128), descending,
   // and their 128 singular values.
   Tensor Vbasis;
@@ -51,6 +60,7 @@ struct SvdPending {
   hipStream_t stream = nullptr;
   int slot = 0;
   int k = 0;
+  int nall = 0;
 };
 SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
                          const SvdOptions &opt, int slot);

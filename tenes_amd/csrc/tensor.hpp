@@ -148,9 +148,14 @@ class Context {
   // (one round = one pass over p/2 disjoint column pairs)
   int64_t svd_calls = 0;
   int64_t jacobi_rounds = 0;
-  // Range-finder start blocks of the CTM projectors (Ritz basis, singular
-  // values), keyed by (environment, move, coordinate, row, n).
-  std::map<std::array<int64_t, 5>, std::pair<Tensor, Tensor>> svd_start;
+  // Range-finder start blocks of the CTM projectors (Ritz basis, its
+  // singular values on the device, their max/min ratio once known), keyed by
+  // (environment, move, coordinate, row, n).
+  struct SvdStart {
+    Tensor V, s;
+    double range = 0.0;
+  };
+  std::map<std::array<int64_t, 5>, SvdStart> svd_start;
   void forget_svd_starts(int64_t environment);
 
   // Optional per-launch timing of the GEMM kernels with HIP events recorded
