@@ -155,6 +155,8 @@ def main():
     ap.add_argument("--su-steps", type=int, default=500)
     ap.add_argument("--cpu-moves", type=int, default=1, help="oracle moves timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the untimed accuracy run against the oracle fixture")
     ap.add_argument("--power-iters", type=int, default=None)
     args = ap.parse_args()
 
@@ -203,10 +205,24 @@ def main():
     achieved = dom_fl / (dom_ms * 1e-3) / 1e12
     E = energy(env, H)
     log(f"energy per site (FP64): {E:.15f}; GEMM share of timed region {gemm_ms / (dt * 1e3):.1%}")
+    # one more sweep with the projector lanes serialised (one stream): the
+    # dominant GEMM then runs alone on the device, as under rocprofv3's kernel
+    # trace, whose average launch time this one matches
+    ctx.set_max_lanes(1)
+    ctx.reset_kernel_report()
+    ctx.set_profiling(True)
+    env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=1, **kw), initialize=False)
+    ctx.synchronize()
+    ctx.set_profiling(False)
+    ctx.set_max_lanes(8)
+    ser_n, ser_ms, ser_fl = ctx.kernel_report().get(dom_name, (0, 0.0, 0.0))
+    ser_achieved = ser_fl / (ser_ms * 1e-3) / 1e12 if ser_ms else None
+    log(f"dominant GEMM {dom_name}: {dom_ms / dom_n:.3f} ms per launch in the timed region (two lanes), "
+        f"{ser_ms / max(ser_n, 1):.3f} ms serialised -> {ser_achieved or 0:.2f} TFLOP/s")
     # accuracy against the oracle's FP64 (LAPACK SVD) run of the same problem:
     # a fresh environment, the oracle's sweep count, untimed
     e_err = ms_err = xi_err = None
-    if fx is not None:
+    if fx is not None and not args.no_check:
         chk = ta.Environment(ctx, 2, 2)
         chk.set_site_tensors(Tn)
         chk.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=fx["sweeps"], **kw), initialize=True)
@@ -258,7 +274,10 @@ def main():
                    "projector": "corner", "svd": "GPU truncated FP64 SVD (randomized range finder + Jacobi)"},
         "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic,
-                     "launches": dom_n, "avg_launch_ms": dom_ms / dom_n},
+                     "launches": dom_n, "avg_launch_ms": dom_ms / dom_n,
+                     "serialized_avg_launch_ms": ser_ms / ser_n if ser_n else None,
+                     "serialized_achieved": ser_achieved,
+                     "serialized_frac": ser_achieved / FP64_MFMA_PEAK_TFLOPS if ser_achieved else None},
         "cpu_baseline": cpu,
         "energy_per_site_fp64": E,
         "energy_rel_err_vs_fp64": e_err,
