@@ -173,6 +173,18 @@ int tenes_env_get(tenes_env *env, int kind, int site, tenes_tensor **out);
  * measure (either may be NULL). */
 int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
                                int initialize, int *count, double *sig_max);
+/* Calc_CTM_Environment (ctm.cpp:971-1253) on n independent environments
+ * at once.  Environments that belong to one context run in order on one host
+ * thread; different contexts run concurrently on their own streams, so
+ * spreading the problems over several contexts fills the device with
+ * several latency-bound projector chains.  params: 1 (shared) or n entries;
+ * count / sig_max: n entries each (may be NULL).  No counterpart in the
+ * reference (one problem per run); on error the message is available from
+ * tenes_ctx_last_error(NULL) and from the failing context. */
+int tenes_batch_calc_ctm_environment(tenes_env *const *envs, int n,
+                                     const tenes_peps_parameters *params,
+                                     int nparams, int initialize, int *count,
+                                     double *sig_max);
 /* Left_move / Right_move / Top_move / Bottom_move, ctm.cpp:562-813.
  * dir: 0 left, 1 right, 2 top, 3 bottom; coord = ix (left/right) or iy. */
 int tenes_env_move(tenes_env *env, int dir, int coord,

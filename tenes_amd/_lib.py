@@ -95,6 +95,8 @@ _SIGS = {
     "tenes_env_contract_one_site": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p, c_dbl_p]),
     "tenes_env_contract_two_sites_op12": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
+    "tenes_batch_calc_ctm_environment": (
+        C.c_int, [c_void_pp, C.c_int, C.POINTER(PepsParameters), C.c_int, C.c_int, c_int_p, c_dbl_p]),
     "tenes_env_bond_energy": (
         C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_dbl_p, c_dbl_p, c_void_pp]),
     "tenes_tensor_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -142,6 +142,31 @@ class Tensor:
         self.h = None
 
 
+def batch_calc_ctm_environment(envs, params, initialize: bool = True):
+    """Calc_CTM_Environment on several environments at once
+    (tenes_batch_calc_ctm_environment): environments of one context run in
+    order, different contexts concurrently.  params: one PepsParameters or
+    one per environment.  Returns (sweep counts, sig_max values)."""
+    envs = list(envs)
+    n = len(envs)
+    if isinstance(params, L.PepsParameters):
+        params = [params]
+    params = list(params)
+    if len(params) not in (1, n):
+        raise ValueError("one parameter set, or one per environment")
+    lib = L.load()
+    arr = (C.c_void_p * max(n, 1))(*[e.h for e in envs])
+    prm = (L.PepsParameters * len(params))(*params)
+    cnt = np.zeros(max(n, 1), dtype=np.int32)
+    sig = np.zeros(max(n, 1), dtype=np.float64)
+    rc = lib.tenes_batch_calc_ctm_environment(arr, n, prm, len(params), 1 if initialize else 0,
+                                              L.iptr(cnt), L.dptr(sig))
+    if rc != 0:
+        msg = lib.tenes_ctx_last_error(None)
+        raise TenesError(f"tenes_batch_calc_ctm_environment failed ({rc}): {msg.decode() if msg else ''}")
+    return [int(c) for c in cnt[:n]], [float(s) for s in sig[:n]]
+
+
 def Tensor_zeros(ctx: Context, shape: Sequence[int]) -> Tensor:
     """Zero-filled device tensor (tenes_tensor_zeros)."""
     shp = np.array(shape, dtype=np.int64)

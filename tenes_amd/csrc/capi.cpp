@@ -2,8 +2,10 @@
 // environment arrays, exception-free, one status code per call.
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tenes_amd.h"
@@ -473,6 +475,54 @@ int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
     if (count) *count = n;
     if (sig_max) *sig_max = sm;
   });
+}
+
+int tenes_batch_calc_ctm_environment(tenes_env *const *envs, int n,
+                                     const tenes_peps_parameters *p,
+                                     int nparams, int initialize, int *count,
+                                     double *sig_max) {
+  if (n < 0 || (n > 0 && (!envs || !p)) || !(nparams == 1 || nparams == n))
+    return TENES_ERR_ARGUMENT;
+  // environments of one context run in order on one host thread; contexts
+  // run concurrently (each has its own streams and allocator)
+  std::map<tenes_ctx *, std::vector<int>> groups;
+  for (int e = 0; e < n; ++e) {
+    if (!envs[e] || !envs[e]->ctx) return TENES_ERR_ARGUMENT;
+    groups[envs[e]->ctx].push_back(e);
+  }
+  std::vector<std::pair<tenes_ctx *, std::vector<int>>> work(groups.begin(),
+                                                             groups.end());
+  std::vector<int> rc(work.size(), TENES_OK);
+  std::vector<std::thread> threads;
+  threads.reserve(work.size());
+  for (size_t g = 0; g < work.size(); ++g) {
+    threads.emplace_back([&, g] {
+      tenes_ctx *ctx = work[g].first;
+      rc[g] = guard(ctx, [&] {
+        TN_HIP_CHECK(hipSetDevice(ctx->c->device()));
+        for (int e : work[g].second) {
+          tenes_env *env = envs[e];
+          for (const auto &t : env->arr[TENES_TN])
+            TN_REQUIRE(t.defined(), "site tensor not set");
+          double sm = 0.0;
+          auto &a = env->arr;
+          const int c = tenes::itps::core::Calc_CTM_Environment(
+              a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8],
+              to_params(&p[nparams == 1 ? 0 : e]), env->lat, initialize != 0,
+              &sm);
+          if (count) count[e] = c;
+          if (sig_max) sig_max[e] = sm;
+        }
+      });
+    });
+  }
+  for (auto &t : threads) t.join();
+  for (size_t g = 0; g < work.size(); ++g)
+    if (rc[g] != TENES_OK) {
+      g_noctx_err = work[g].first->err;
+      return rc[g];
+    }
+  return TENES_OK;
 }
 
 int tenes_env_move(tenes_env *env, int dir, int coord,
