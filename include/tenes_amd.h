@@ -265,6 +265,36 @@ int tenes_env_simple_update(tenes_env *env, int nops, const int *source_sites,
 int tenes_env_get_lambda(tenes_env *env, int site, int leg, double *out,
                          int *n_out);
 
+/* Full-update fields of PEPS_Parameters (PEPS_Parameters.hpp:29-94,
+ * defaults PEPS_Parameters.cpp:55-64). */
+typedef struct tenes_fu_parameters {
+  double Inverse_Env_cut;
+  double Full_Inverse_precision;
+  double Full_Convergence_Epsilon;
+  int Full_max_iteration;
+  int Full_Gauge_Fix;
+  int Full_Use_FastFullUpdate;
+} tenes_fu_parameters;
+void tenes_fu_parameters_default(tenes_fu_parameters *p);
+/* Full_update_bond, core/full_update.cpp:395-441 (horizontal kernel 85-393):
+ * T = {C1, C2, C3, C4, eT1, ..., eT6, Tn1, Tn2} (12 handles) as TeNeS passes
+ * them, op12 a host array (d1, d2, d1, d2), connect1 the leg of Tn1 on the
+ * bond.  The bond environment is built on the GPU; the ALS on tensors of at
+ * most (D d)^4 entries runs in FP64 on the host. */
+int tenes_full_update_bond(const tenes_tensor *const *T, const double *op12,
+                           int connect1, const tenes_fu_parameters *fp,
+                           tenes_tensor **Tn1_new, tenes_tensor **Tn2_new);
+/* The full-update driver, src/iTPS/full_update.cpp:154-186: a fresh CTM
+ * environment, then nsteps passes over the nops two-site operators (bond
+ * (source[k], source_leg[k]), operator the next (d_s d_t)^2 doubles of ops,
+ * order (source, target, source', target')), each followed by the
+ * fast-full-update re-moves (full_update.cpp:114-151) or, with
+ * Full_Use_FastFullUpdate = 0, a fresh environment. */
+int tenes_env_full_update(tenes_env *env, int nops, const int *source,
+                          const int *source_leg, const double *ops, int nsteps,
+                          const tenes_peps_parameters *pp,
+                          const tenes_fu_parameters *fp);
+
 /* ---- transfer matrix (transfer_matrix.hpp:49-139, arnoldi.hpp:32-71) ---- */
 typedef struct tenes_tm_parameters {
   int num_eigvals;

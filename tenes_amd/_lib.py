@@ -35,6 +35,17 @@ class PepsParameters(C.Structure):
     ]
 
 
+class FuParameters(C.Structure):
+    _fields_ = [
+        ("Inverse_Env_cut", C.c_double),
+        ("Full_Inverse_precision", C.c_double),
+        ("Full_Convergence_Epsilon", C.c_double),
+        ("Full_max_iteration", C.c_int),
+        ("Full_Gauge_Fix", C.c_int),
+        ("Full_Use_FastFullUpdate", C.c_int),
+    ]
+
+
 class TmParameters(C.Structure):
     _fields_ = [
         ("num_eigvals", C.c_int),
@@ -97,6 +108,12 @@ _SIGS = {
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
     "tenes_batch_calc_ctm_environment": (
         C.c_int, [c_void_pp, C.c_int, C.POINTER(PepsParameters), C.c_int, C.c_int, c_int_p, c_dbl_p]),
+    "tenes_fu_parameters_default": (None, [C.POINTER(FuParameters)]),
+    "tenes_full_update_bond": (
+        C.c_int, [c_void_pp, c_dbl_p, C.c_int, C.POINTER(FuParameters), c_void_pp, c_void_pp]),
+    "tenes_env_full_update": (
+        C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_dbl_p, C.c_int, C.POINTER(PepsParameters),
+                  C.POINTER(FuParameters)]),
     "tenes_env_bond_energy": (
         C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_dbl_p, c_dbl_p, c_void_pp]),
     "tenes_tensor_to_device": (C.c_int, [C.c_void_p, C.c_void_p]),

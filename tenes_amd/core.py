@@ -216,6 +216,29 @@ def peps_parameters(**kw) -> L.PepsParameters:
     return p
 
 
+def fu_parameters(**kw) -> L.FuParameters:
+    """tenes_fu_parameters with TeNeS's defaults, overridden by kw."""
+    p = L.FuParameters()
+    L.load().tenes_fu_parameters_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def full_update_bond(tensors: Sequence[Tensor], op12: np.ndarray, connect1: int, params=None):
+    """Full_update_bond (core/full_update.cpp:395-441): tensors = C1..C4,
+    eT1..eT6, Tn1, Tn2 (device Tensors).  Returns (Tn1_new, Tn2_new)."""
+    ctx = tensors[0].ctx
+    arr = (C.c_void_p * 12)(*[t.h for t in tensors])
+    o = np.ascontiguousarray(op12, dtype=np.float64)
+    a, b = C.c_void_p(), C.c_void_p()
+    prm = params or fu_parameters()
+    ctx.check(ctx.lib.tenes_full_update_bond(arr, L.dptr(o), connect1, C.byref(prm), C.byref(a), C.byref(b)))
+    return Tensor(ctx, a), Tensor(ctx, b)
+
+
 def tm_parameters(**kw) -> L.TmParameters:
     p = L.TmParameters()
     L.load().tenes_tm_parameters_default(C.byref(p))
@@ -373,6 +396,18 @@ class Environment:
         arr = (C.c_void_p * max(len(tens), 1))(*[t.h for t in tens])
         self.ctx.check(self.ctx.lib.tenes_env_simple_update(
             self.h, len(ops), L.iptr(src), L.iptr(leg), arr, nsteps, inverse_lambda_cut))
+
+    def full_update(self, ops, nsteps: int, ctm_params=None, fu_params=None):
+        """TeNeS's full-update driver (src/iTPS/full_update.cpp:154-186):
+        ops = [(source_site, source_leg, op (d_s, d_t, d_s, d_t)), ...]."""
+        src = np.array([o[0] for o in ops], dtype=np.int32)
+        leg = np.array([o[1] for o in ops], dtype=np.int32)
+        flat = np.concatenate([np.ascontiguousarray(o[2], dtype=np.float64).ravel() for o in ops]) \
+            if ops else np.zeros(1)
+        pp = ctm_params or peps_parameters()
+        fp = fu_params or fu_parameters()
+        self.ctx.check(self.ctx.lib.tenes_env_full_update(
+            self.h, len(ops), L.iptr(src), L.iptr(leg), L.dptr(flat), nsteps, C.byref(pp), C.byref(fp)))
 
     def bond_energy(self, bonds, grad=None):
         """Energy per site of nearest-neighbour bond operators with the

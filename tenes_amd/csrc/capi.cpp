@@ -12,6 +12,7 @@
 #include "adjoint.hpp"
 #include "contract.hpp"
 #include "ctm.hpp"
+#include "full_update.hpp"
 #include "linalg.hpp"
 #include "simple_update.hpp"
 #include "transfer.hpp"
@@ -767,6 +768,91 @@ int tenes_create_environment_two_sites(const tenes_tensor *const *Ts,
                          T(Ts[0]), T(Ts[1]), T(Ts[2]), T(Ts[3]), T(Ts[4]),
                          T(Ts[5]), T(Ts[6]), T(Ts[7]), T(Ts[8]), T(Ts[9]),
                          T(Ts[10]), T(Ts[11])));
+  });
+}
+
+namespace {
+
+tenes::itps::FU_Parameters to_fu(const tenes_fu_parameters *p) {
+  tenes::itps::FU_Parameters q;
+  if (p) {
+    q.Inverse_Env_cut = p->Inverse_Env_cut;
+    q.Full_Inverse_precision = p->Full_Inverse_precision;
+    q.Full_Convergence_Epsilon = p->Full_Convergence_Epsilon;
+    q.Full_max_iteration = p->Full_max_iteration;
+    q.Full_Gauge_Fix = p->Full_Gauge_Fix != 0;
+    q.Full_Use_FastFullUpdate = p->Full_Use_FastFullUpdate != 0;
+  }
+  return q;
+}
+
+}  // namespace
+
+void tenes_fu_parameters_default(tenes_fu_parameters *p) {
+  if (!p) return;
+  const tenes::itps::FU_Parameters q;
+  p->Inverse_Env_cut = q.Inverse_Env_cut;
+  p->Full_Inverse_precision = q.Full_Inverse_precision;
+  p->Full_Convergence_Epsilon = q.Full_Convergence_Epsilon;
+  p->Full_max_iteration = q.Full_max_iteration;
+  p->Full_Gauge_Fix = q.Full_Gauge_Fix ? 1 : 0;
+  p->Full_Use_FastFullUpdate = q.Full_Use_FastFullUpdate ? 1 : 0;
+}
+
+int tenes_full_update_bond(const tenes_tensor *const *Ts, const double *op12,
+                           int connect1, const tenes_fu_parameters *fp,
+                           tenes_tensor **Tn1_new, tenes_tensor **Tn2_new) {
+  if (!Ts || !Ts[0]) return TENES_ERR_ARGUMENT;
+  tenes_ctx *ctx = ctx_of(Ts[0]);
+  return guard(ctx, [&] {
+    TN_REQUIRE(op12 && Tn1_new && Tn2_new, "null pointer");
+    for (int i = 0; i < 12; ++i) TN_REQUIRE(Ts[i] != nullptr, "null tensor handle");
+    const tn::Tensor &A = T(Ts[10]), &B = T(Ts[11]);
+    TN_REQUIRE(A.rank() == 5 && B.rank() == 5, "site tensors must have 5 legs");
+    const int64_t n = A.shape[4] * B.shape[4] * A.shape[4] * B.shape[4];
+    std::vector<double> op(op12, op12 + n);
+    tn::Tensor n1, n2;
+    tenes::itps::core::Full_update_bond(
+        T(Ts[0]), T(Ts[1]), T(Ts[2]), T(Ts[3]), T(Ts[4]), T(Ts[5]), T(Ts[6]),
+        T(Ts[7]), T(Ts[8]), T(Ts[9]), A, B, op, connect1, to_fu(fp), n1, n2);
+    *Tn1_new = wrap(ctx, n1);
+    *Tn2_new = wrap(ctx, n2);
+  });
+}
+
+int tenes_env_full_update(tenes_env *env, int nops, const int *source,
+                          const int *source_leg, const double *ops, int nsteps,
+                          const tenes_peps_parameters *pp,
+                          const tenes_fu_parameters *fp) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    for (const auto &t : env->arr[TENES_TN])
+      TN_REQUIRE(t.defined(), "site tensor not set");
+    TN_REQUIRE(nops >= 0 && (nops == 0 || (source && source_leg && ops)),
+               "null pointer");
+    auto &a = env->arr;
+    const int N = env->lat.N_UNIT;
+    std::vector<std::vector<double>> op(nops);
+    size_t off = 0;
+    for (int k = 0; k < nops; ++k) {
+      TN_REQUIRE(source[k] >= 0 && source[k] < N, "bad source site");
+      TN_REQUIRE(source_leg[k] >= 0 && source_leg[k] < 4, "bad source leg");
+      const int t = env->lat.neighbor(source[k], source_leg[k]);
+      const int64_t ds = a[8][source[k]].shape[4], dt = a[8][t].shape[4];
+      op[k].assign(ops + off, ops + off + ds * dt * ds * dt);
+      off += (size_t)(ds * dt * ds * dt);
+    }
+    if (nsteps <= 0) return;
+    const auto q = to_params(pp);
+    const auto f = to_fu(fp);
+    tenes::itps::core::Calc_CTM_Environment(a[0], a[1], a[2], a[3], a[4], a[5],
+                                            a[6], a[7], a[8], q, env->lat,
+                                            true, nullptr);
+    for (int step = 0; step < nsteps; ++step)
+      for (int k = 0; k < nops; ++k)
+        tenes::itps::full_update_step(a[0], a[1], a[2], a[3], a[4], a[5], a[6],
+                                      a[7], a[8], source[k], source_leg[k],
+                                      op[k], q, f, env->lat);
   });
 }
 
