@@ -62,6 +62,11 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx);
  * order on the context stream).  No counterpart in the reference, whose
  * moves loop over the rows (src/iTPS/core/ctm.cpp:577-615). */
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n);
+/* Contractions issued from now on (tenes_tensordot and the rest) run on the
+ * FP32 MFMA when on != 0 (FP64 data, FP32 operands and accumulation); the
+ * CTM moves set it themselves from tenes_peps_parameters.contraction_fp32.
+ * No counterpart in the reference. */
+int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on);
 /* Truncated SVDs (tenes_svd_top and the projectors) since the last reset and
  * the Jacobi rounds they took (a round orthogonalises p/2 disjoint column
  * pairs of the p x p projected matrix; a sweep is p - 1 rounds). */
@@ -124,6 +129,12 @@ typedef struct tenes_peps_parameters {
    * Gaussian ones.  Defaults: 1, 8 (svd_power_iters: 8). */
   int svd_warm_start;
   int svd_warm_power_iters;
+  /* Mixed precision (no counterpart in the reference): the contractions of
+   * the moves on the FP32 MFMA (FP64 data, FP32 operands and accumulation):
+   * 1 = all but the projector matrix R = LT LB, 2 = R as well; projector
+   * SVDs, QRs, the convergence check, Arnoldi and observables stay FP64.
+   * Default 0. */
+  int contraction_fp32;
 } tenes_peps_parameters;
 /* Defaults of PEPS_Parameters.cpp:29-77. */
 void tenes_peps_parameters_default(tenes_peps_parameters *p);

@@ -51,6 +51,10 @@ class Context:
         """Side streams a CTM move may overlap its rows on (1: none)."""
         self.check(self.lib.tenes_ctx_set_max_lanes(self.h, int(n)))
 
+    def set_fp32_contractions(self, on: bool):
+        """Contractions from now on run on the FP32 MFMA (FP64 data)."""
+        self.check(self.lib.tenes_ctx_set_fp32_contractions(self.h, 1 if on else 0))
+
     def svd_stats(self):
         """(truncated SVDs, Jacobi rounds) since the last reset_stats()."""
         a = C.c_int64()

@@ -83,6 +83,7 @@ PEPS_Parameters to_params(const tenes_peps_parameters *p) {
   q.rsvd_power_iters = p->rsvd_power_iters;
   q.svd_warm_start = p->svd_warm_start != 0;
   q.svd_warm_power_iters = p->svd_warm_power_iters;
+  q.contraction_fp32 = p->contraction_fp32;
   TN_REQUIRE(q.CHI >= 1, "CHI must be positive");
   TN_REQUIRE(q.RSVD_Oversampling_factor >= 1.0,
              "RSVD oversampling factor must be >= 1");
@@ -155,7 +156,7 @@ Cluster cluster_of(const tenes::itps::SquareLattice &lat, int source, int dx,
 
 extern "C" {
 
-int tenes_abi_version(void) { return 202; }
+int tenes_abi_version(void) { return 203; }
 
 int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
   return guard(nullptr, [&] {
@@ -202,6 +203,11 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx) {
     ctx->c->svd_calls = 0;
     ctx->c->jacobi_rounds = 0;
   });
+}
+
+int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on) {
+  if (!ctx) return TENES_ERR_ARGUMENT;
+  return guard(ctx, [&] { ctx->c->set_fp32_contractions(on != 0); });
 }
 
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {
@@ -356,6 +362,7 @@ void tenes_peps_parameters_default(tenes_peps_parameters *p) {
   p->rsvd_power_iters = q.rsvd_power_iters;
   p->svd_warm_start = q.svd_warm_start ? 1 : 0;
   p->svd_warm_power_iters = q.svd_warm_power_iters;
+  p->contraction_fp32 = q.contraction_fp32;
 }
 
 int tenes_calc_projector_left_block(

@@ -281,8 +281,13 @@ void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
         std::min<int64_t>(std::min<int64_t>(pp.CHI, e78 * t41 * t41),
                           e12 * t12 * t12),
         e56 * t34 * t34);
-  // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273)
-  tensor R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
+  // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273); in
+  // mixed mode 1 the matrix the SVD truncates stays FP64
+  tensor R;
+  {
+    tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
+    R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
+  }
   projectors_begin(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 1, 2}, LB,
                    Axes{3, 4, 5}, pp, rsvd, slot, key, job);
 }
@@ -327,7 +332,11 @@ void updown_blocks_begin(
   // R1 = RT*LT, R2 = RB*LB, legs (t23, e34, tc23, t41, e78, tc41)
   tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 1, 2});
   tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{3, 4, 5});
-  tensor M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
+  tensor M;
+  {
+    tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
+    M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
+  }
   projectors_begin(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2}, R2,
                    Axes{0, 1, 2}, pp, rsvd, slot, key, job);
 }
@@ -436,6 +445,7 @@ void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
                const PEPS_Parameters &pp, const SquareLattice &lat) {
   const int L = lat.LY_noskew;
   tn::Context &ctx = *Tn[0].ctx();
+  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
   std::vector<tensor> PUs(L), PLs(L);
   run_projectors(
       ctx, L,
@@ -479,6 +489,7 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
   const int L = lat.LY_noskew;
   const Axes rot{2, 3, 0, 1, 4};
   tn::Context &ctx = *Tn[0].ctx();
+  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
   std::vector<tensor> PUs(L), PLs(L);
   run_projectors(
       ctx, L,
@@ -527,6 +538,7 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
   const int L = lat.LX_noskew;
   const Axes rot{1, 2, 3, 0, 4};
   tn::Context &ctx = *Tn[0].ctx();
+  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
   std::vector<tensor> PUs(L), PLs(L);
   run_projectors(
       ctx, L,
@@ -575,6 +587,7 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
   const int L = lat.LX_noskew;
   const Axes rot{3, 0, 1, 2, 4};
   tn::Context &ctx = *Tn[0].ctx();
+  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
   std::vector<tensor> PUs(L), PLs(L);
   run_projectors(
       ctx, L,

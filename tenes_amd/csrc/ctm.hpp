@@ -42,6 +42,12 @@ struct PEPS_Parameters {
   // (seed dependent) at 72-76 ms per sweep.
   bool svd_warm_start = true;
   int svd_warm_power_iters = 8;
+  // Mixed precision, the contractions of the four moves on the FP32 MFMA:
+  // 1 = enlarged corners, projectors, Calc_Next_CTM / Calc_Next_eT, with the
+  // matrix the SVD truncates (R = LT LB) kept FP64; 2 = R as well.  The
+  // projector SVD and its QRs, the convergence check and everything outside
+  // the moves stay FP64.  0 = all FP64.
+  int contraction_fp32 = 0;
 };
 
 // Identity of an environment for the warm-start bases of its projectors:

@@ -35,6 +35,9 @@ struct GemmArgs {
   // the loader computes the offset instead of reading the table.  0 = use the
   // table, otherwise the stride.
   int a_m_stride, a_k_stride, b_k_stride, b_n_stride;
+  // Compute in FP32 on the MFMA (operands rounded to FP32 when they are
+  // staged into LDS, FP32 accumulation; HBM data and the result stay FP64).
+  bool fp32 = false;
 };
 
 // Scratch needed for split-K on the given shape (bytes, 0 if none).

@@ -1,4 +1,7 @@
-// FP64 gather-GEMM for gfx950 (CDNA4) on v_mfma_f64_16x16x4_f64.
+// FP64 gather-GEMM for gfx950 (CDNA4) on v_mfma_f64_16x16x4_f64, with an
+// FP32-compute variant on v_mfma_f32_16x16x4_f32 (same tile shape, twice the
+// matrix rate: operands rounded to FP32 when staged into LDS, FP32
+// accumulation, FP64 in HBM on both sides).
 //
 // One kernel serves every tensordot of the CTMRG path.  Operands are tensors
 // in row-major HBM layout; the tensordot's leg permutation is expressed as
@@ -24,7 +27,26 @@ namespace kern {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int BK = 16;
+
+// Compute-type traits: accumulator vector and the 16x16x4 MFMA.
+template <class CT>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ d4 run(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ f4 run(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
 
 struct KArgs {
   GemmArgs g;
@@ -54,8 +76,8 @@ __device__ __forceinline__ void tile_of(int id, int nbm, int nbn, int &bm, int &
   bn = r / gsize;
 }
 
-template <int BM, int BN, bool A_MFAST, bool B_NFAST, bool KA, bool KB>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
+template <class CT, int BM, int BN, bool A_MFAST, bool B_NFAST, bool KA, bool KB>
+__global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
   // KA / KB: the contracted-index offsets of A / B form an arithmetic
   // progression (computed), otherwise they come from the table, prefetched
   // one K tile ahead so that the tile loads never wait on a table read.
@@ -66,8 +88,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   constexpr int B_PER = BN * BK / 256;
   constexpr int A_NK = A_MFAST ? A_PER : 1;  // distinct k per thread and tile
   constexpr int B_NK = B_NFAST ? B_PER : 1;
-  __shared__ double As[2][BK][BM + 1];  // double-buffered K tiles
-  __shared__ double Bs[2][BK][BN + 1];
+  __shared__ CT As[2][BK][BM + 1];  // double-buffered K tiles
+  __shared__ CT Bs[2][BK][BN + 1];
+  typedef typename Mfma<CT>::acc_t acc_t;
 
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -155,16 +178,16 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
-    for (int p = 0; p < A_PER; ++p) As[buf][a_kl[p]][a_ml[p]] = oka[p] ? ra[p] : 0.0;
+    for (int p = 0; p < A_PER; ++p) As[buf][a_kl[p]][a_ml[p]] = (CT)(oka[p] ? ra[p] : 0.0);
 #pragma unroll
-    for (int p = 0; p < B_PER; ++p) Bs[buf][b_kl[p]][b_nl[p]] = okb[p] ? rb[p] : 0.0;
+    for (int p = 0; p < B_PER; ++p) Bs[buf][b_kl[p]][b_nl[p]] = (CT)(okb[p] ? rb[p] : 0.0);
   };
 
-  d4 acc[TM][TN];
+  acc_t acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int li = lane & 15;  // row (A) / column (B) inside a 16x16 block
   const int lk = lane >> 4;  // k inside the 16x16x4 step
@@ -190,7 +213,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
     }
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
-      double a[TM], b[TN];
+      CT a[TM], b[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) a[i] = As[buf][ks * 4 + lk][wrow + i * 16 + li];
 #pragma unroll
@@ -198,9 +221,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j],
-                                                           acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<CT>::run(a[i], b[j], acc[i][j]);
     }
     if (next) store_tile(buf ^ 1);
     __syncthreads();
@@ -208,8 +229,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
   }
 
   // ---- epilogue -------------------------------------------------------------
-  // v_mfma_f64_16x16x4 output layout (measured on gfx950): lane l, register
-  // r -> row (l/16) + 4*r, column l%16.
+  // Output layouts (lane l, register r, column l%16): v_mfma_f64_16x16x4
+  // row (l/16) + 4*r (measured on gfx950); v_mfma_f32_16x16x4 row
+  // 4*(l/16) + r.
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -218,9 +240,10 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(KArgs ka) {
       if (n >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wrow + i * 16 + lk + 4 * r;
+        const int row = sizeof(CT) == 8 ? lk + 4 * r : 4 * lk + r;
+        const int m = m0 + wrow + i * 16 + row;
         if (m >= M) continue;
-        const double v = acc[i][j][r];
+        const double v = (double)acc[i][j][r];
         if (ka.partial) {
           ka.partial[((int64_t)blockIdx.z * M + m) * N + n] = v;
         } else {
@@ -278,10 +301,10 @@ Plan choose(int M, int N, int K) {
 }
 
 // Static name of an instantiation: "gemm_f64_kernel<BM,BN,A_MFAST,B_NFAST,
-// KA,KB>" (used by the per-launch timing report).
-template <int BM, bool AM, bool BNF, bool KAA, bool KBA>
+// KA,KB>" or "gemm_f32_kernel<...>" (used by the per-launch timing report).
+template <class CT, int BM, bool AM, bool BNF, bool KAA, bool KBA>
 const char *kernel_name() {
-  static const std::string n = std::string("gemm_f64_kernel<") +
+  static const std::string n = std::string(sizeof(CT) == 8 ? "gemm_f64_kernel<" : "gemm_f32_kernel<") +
                                std::to_string(BM) + "," + std::to_string(BM) +
                                (AM ? ",true" : ",false") +
                                (BNF ? ",true" : ",false") +
@@ -290,14 +313,14 @@ const char *kernel_name() {
   return n.c_str();
 }
 
-template <int BM, int BN, bool AM, bool BNF>
+template <class CT, int BM, int BN, bool AM, bool BNF>
 const char *launch_k(const KArgs &ka, dim3 grid, hipStream_t s) {
   const bool ka_aff = ka.g.a_k_stride != 0, kb_aff = ka.g.b_k_stride != 0;
 #define TN_LAUNCH(KAA, KBA)                                                  \
   do {                                                                       \
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AM, BNF, KAA, KBA>), grid,   \
+    hipLaunchKernelGGL((gemm_kernel<CT, BM, BN, AM, BNF, KAA, KBA>), grid,   \
                        dim3(256), 0, s, ka);                                 \
-    return kernel_name<BM, AM, BNF, KAA, KBA>();                             \
+    return kernel_name<CT, BM, AM, BNF, KAA, KBA>();                         \
   } while (0)
   if (ka_aff && kb_aff) TN_LAUNCH(true, true);
   if (ka_aff) TN_LAUNCH(true, false);
@@ -306,13 +329,13 @@ const char *launch_k(const KArgs &ka, dim3 grid, hipStream_t s) {
 #undef TN_LAUNCH
 }
 
-template <int BM, int BN>
+template <class CT, int BM, int BN>
 const char *launch_cfg(const KArgs &ka, dim3 grid, hipStream_t s) {
   const bool am = ka.g.a_mfast, bn = ka.g.b_nfast;
-  if (am && bn) return launch_k<BM, BN, true, true>(ka, grid, s);
-  if (am) return launch_k<BM, BN, true, false>(ka, grid, s);
-  if (bn) return launch_k<BM, BN, false, true>(ka, grid, s);
-  return launch_k<BM, BN, false, false>(ka, grid, s);
+  if (am && bn) return launch_k<CT, BM, BN, true, true>(ka, grid, s);
+  if (am) return launch_k<CT, BM, BN, true, false>(ka, grid, s);
+  if (bn) return launch_k<CT, BM, BN, false, true>(ka, grid, s);
+  return launch_k<CT, BM, BN, false, false>(ka, grid, s);
 }
 
 }  // namespace
@@ -345,8 +368,11 @@ const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
   }
   if (g.K == 0) ka.k_per_split = 0;
   dim3 grid(ceil_div(g.N, p.BN), ceil_div(g.M, p.BM), splits);
-  const char *name = (p.BM == 128) ? launch_cfg<128, 128>(ka, grid, s)
-                                   : launch_cfg<64, 64>(ka, grid, s);
+  const char *name =
+      g.fp32 ? ((p.BM == 128) ? launch_cfg<float, 128, 128>(ka, grid, s)
+                              : launch_cfg<float, 64, 64>(ka, grid, s))
+             : ((p.BM == 128) ? launch_cfg<double, 128, 128>(ka, grid, s)
+                              : launch_cfg<double, 64, 64>(ka, grid, s));
   TN_HIP_CHECK(hipGetLastError());
   if (ka.partial) {
     const int64_t MN = (int64_t)g.M * g.N;

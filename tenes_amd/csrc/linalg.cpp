@@ -107,6 +107,7 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
 SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
                          const SvdOptions &opt, int slot) {
   TN_REQUIRE(slot >= 0 && slot < 63, "svd_top_begin: pinned slot 0..62");
+  ContractionPrecision fp64(ctx, false);  // the SVD and its QRs stay FP64
   TN_REQUIRE(A.rank() == 2, "svd_top: 2-D input expected");
   const int m = (int)A.shape[0], n = (int)A.shape[1];
   TN_REQUIRE(m >= 1 && n >= 1, "svd_top: empty matrix");
@@ -247,6 +248,7 @@ SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
 
 Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats) {
   TN_REQUIRE(!mats.empty(), "svd_values_batched: empty batch");
+  ContractionPrecision fp64(ctx, false);
   const int m = (int)mats[0].shape[0], n = (int)mats[0].shape[1];
   TN_REQUIRE(m <= kJac && n <= kJac,
              "svd_values_batched: matrices larger than 128 x 128");

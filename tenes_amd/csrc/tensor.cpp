@@ -577,6 +577,7 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
   g.a_k_stride = p.a_k_stride;
   g.b_k_stride = p.b_k_stride;
   g.b_n_stride = p.b_n_stride;
+  g.fp32 = ctx.fp32_contractions();
   const size_t ws = kern::gemm_workspace_bytes(p.M, p.N, p.K);
   void *w = ws ? ctx.workspace(ws) : nullptr;
   const double flops = 2.0 * p.M * (double)p.N * p.K;

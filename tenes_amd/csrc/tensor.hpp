@@ -168,6 +168,10 @@ class Context {
   };
   void set_profiling(bool on);
   bool profiling() const { return profiling_; }
+  // Contractions issued while this is on run on the FP32 MFMA (FP64 data in
+  // HBM, operands rounded to FP32, FP32 accumulation).
+  void set_fp32_contractions(bool on) { fp32_ = on; }
+  bool fp32_contractions() const { return fp32_; }
   void profile_begin(hipEvent_t *a);
   void profile_end(hipEvent_t a, const char *name, double flops);
   std::map<std::string, KernelStat> kernel_stats();  // flushes pending
@@ -192,6 +196,7 @@ class Context {
   double *pinned_ = nullptr;
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
+  bool fp32_ = false;
   bool profiling_ = false;
   struct Pending {
     hipEvent_t a, b;
@@ -204,6 +209,21 @@ class Context {
   std::map<std::string, KernelStat> kstats_;
   hipEvent_t get_event();
   void flush_pending();
+};
+
+// Sets the contraction precision of a context for a scope.
+class ContractionPrecision {
+ public:
+  ContractionPrecision(Context &ctx, bool fp32) : ctx_(ctx), old_(ctx.fp32_contractions()) {
+    ctx.set_fp32_contractions(fp32);
+  }
+  ~ContractionPrecision() { ctx_.set_fp32_contractions(old_); }
+  ContractionPrecision(const ContractionPrecision &) = delete;
+  ContractionPrecision &operator=(const ContractionPrecision &) = delete;
+
+ private:
+  Context &ctx_;
+  bool old_;
 };
 
 // ---- construction / transfer ----------------------------------------------

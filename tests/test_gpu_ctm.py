@@ -484,6 +484,44 @@ def test_heisenberg_d5_chi100_energy(gpu_ctx):
         assert np.max(np.abs(np.abs(ev) - np.abs(ref)) / np.abs(ref[0])) < 1e-6
 
 
+@pytest.mark.parametrize("mode,e_tol,xi_tol", [(1, 1e-7, 5e-6), (2, 5e-7, 1e-5)])
+def test_heisenberg_d5_chi100_mixed_precision(gpu_ctx, mode, e_tol, xi_tol):
+    """The benchmark problem with the contractions of the moves on the FP32
+    MFMA (projector SVDs, QRs, convergence check, transfer-matrix Arnoldi and
+    observables in FP64), against the oracle's FP64 LAPACK run.  Mode 1 keeps
+    the projector matrix R = LT LB in FP64: energy within the mixed-mode
+    target 1e-7 (measured 7.6e-9 to 2.8e-8 over seeds), correlation lengths
+    within 5e-6 (6e-7 to 1.1e-6).  Mode 2 puts R on the FP32 MFMA too: its
+    smallest kept singular values (7.5e-7 of the largest) sit near the FP32
+    accumulation error of R, and the energy moves by 2.3e-7 to 2.5e-7 (bar
+    5e-7), the correlation lengths by 3.6e-6 to 4.7e-6 (bar 1e-5)."""
+    from tenes_amd import inputs
+
+    fx = _fixture_or_skip("heisenberg_d5_chi100")
+    D = fx["D"]
+    Tn = [np.array(t).reshape(D, D, D, D, 2) for t in fx["site_tensors"]]
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=fx["chi"], Max_CTM_Iteration=fx["sweeps"],
+                                                        CTM_Convergence_Epsilon=0.0, contraction_fp32=mode))
+    assert n == fx["sweeps"]
+    H = inputs.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    E = 0.0
+    for s, leg in inputs.square_bonds(2, 2):
+        dx, dy = (1, 0) if leg == 2 else (0, 1)
+        E += genv.contract_two_sites_op12(s, dx, dy, H) / genv.contract_cluster(s, dx, dy)
+    E /= 4.0
+    rel = abs(E - fx["energy_per_site"]) / abs(fx["energy_per_site"])
+    xis = []
+    for c in fx["correlation"]:
+        ev = genv.transfer_matrix_eigenvalues(c["dir"], c["fixed"], ta.tm_parameters(num_eigvals=10))
+        xi, _ = TMo.correlation_length(ev, 2)
+        xis.append(abs(xi - c["xi"]) / c["xi"])
+    print(f"mixed precision {mode}: energy rel err {rel:.2e}, xi rel errs {['%.1e' % x for x in xis]}")
+    assert rel < e_tol, rel
+    assert max(xis) < xi_tol, xis
+
+
 def test_tfi_plain_reference_config(gpu_ctx):
     """Transverse-field Ising (Jz = -1, hx = 1), 2x2, D = 2, chi = 10: 1000
     simple-update steps (tau = 0.01) from the ferro state on the GPU and

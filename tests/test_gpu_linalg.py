@@ -48,6 +48,30 @@ def test_tensordot_permuted(gpu_ctx, case):
     assert _rel(got, ref) < RTOL
 
 
+@pytest.mark.parametrize("M,N,K", [(16, 16, 4), (5, 7, 3), (300, 260, 129), (2500, 2500, 64), (64, 64, 5000)])
+def test_matmul_fp32_compute(gpu_ctx, M, N, K):
+    """The FP32-compute GEMM: inputs rounded to FP32, FP32 accumulation.
+    Against the FP64 product the error is that of FP32 sums of K terms
+    (bound: K * 2^-23 * sum|a||b|); its output layout is checked by the
+    small shapes (a wrong row order would be an O(1) error)."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    gpu_ctx.set_fp32_contractions(True)
+    try:
+        C = ta.tensordot(gpu_ctx.tensor(A), gpu_ctx.tensor(B), [1], [0]).numpy()
+    finally:
+        gpu_ctx.set_fp32_contractions(False)
+    ref = A @ B
+    bound = K * 2.0 ** -23 * (np.abs(A) @ np.abs(B))
+    assert np.all(np.abs(C - ref) <= bound), float(np.max(np.abs(C - ref) / bound))
+    A32, B32 = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    assert _rel(C, A32 @ B32) < 1e-5
+    # and the FP64 path is unaffected afterwards
+    C64 = ta.tensordot(gpu_ctx.tensor(A), gpu_ctx.tensor(B), [1], [0]).numpy()
+    assert _rel(C64, ref) < RTOL
+
+
 def test_transpose(gpu_ctx):
     rng = np.random.default_rng(3)
     A = rng.standard_normal((3, 4, 5, 6, 2))
