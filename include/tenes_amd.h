@@ -250,6 +250,14 @@ int tenes_env_bond_energy(tenes_env *env, int nbonds, const int *source,
 /* Copy a tensor into caller-owned device memory (size elements, the
  * tensor's row-major order) and wait for the copy. */
 int tenes_tensor_to_device(const tenes_tensor *t, double *dst);
+/* Correlation functions, correlation_function.cpp:44-220 with the
+ * contractions of contract_itps_ctm/correlation.cpp:34-105:
+ * out[r * nright + j] = <O_left(site) O_j(site + r+1)> / <1 1> for
+ * r = 0..r_max-1 along +x (direction 0) or +y (direction 1); left_op and
+ * the nright right_ops are host (d, d) arrays (one d for all sites). */
+int tenes_env_correlation(tenes_env *env, int left_site, const double *left_op,
+                          int nright, const double *right_ops, int r_max,
+                          int direction, double *out);
 /* Create_Environment_two_sites, core/full_update.cpp:51-83 */
 int tenes_create_environment_two_sites(const tenes_tensor *const *T,
                                        tenes_tensor **out);

@@ -1,6 +1,6 @@
 """CTM-environment contractions restated in numpy
 (TeNeS src/iTPS/core/contract_itps_ctm/ctm.cpp, ctm_1x1.cpp, ctm_1x2.cpp,
-ctm_2x1.cpp, src/iTPS/core/full_update.cpp:51-83)."""
+ctm_2x1.cpp, correlation.cpp, src/iTPS/core/full_update.cpp:51-83)."""
 from __future__ import annotations
 
 import numpy as np
@@ -130,3 +130,32 @@ def Create_Environment_two_sites(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, Q
             ([1, 3], [0, 2]))
     left = td(td(C1, eT1, ([1], [0])), lq, ([0, 2, 3], [0, 4, 2]))
     return td(right, left, ([0, 1], [0, 1])).transpose(3, 1, 2, 0)
+
+
+def StartCorrelation_iTPS_CTM(C1, C4, eT1, eT3, eT4, Tn1, op):
+    """contract_itps_ctm/correlation.cpp:34-58: the left end of a row with the
+    operator on Tn1; legs (e1r, e3r, n1r, n2r)."""
+    A = td(eT1, td(Tn1, td(td(cj(Tn1), op, ([4], [1])),
+                           td(eT3, td(C1, td(C4, eT4, ([1], [0])), ([0], [1])), ([1], [1])),
+                           ([0, 3], [5, 2])),
+                   ([0, 3, 4], [6, 4, 2])),
+           ([0, 2, 3], [5, 0, 2]))
+    return A.transpose(0, 3, 1, 2)
+
+
+def Transfer_iTPS_CTM(A, eT1, eT3, Tn1):
+    """contract_itps_ctm/correlation.cpp:60-78: one more column."""
+    A = td(eT1, td(Tn1, td(cj(Tn1), td(A, eT3, ([1], [1])), ([0, 3], [2, 5])), ([0, 3, 4], [4, 6, 2])),
+           ([0, 2, 3], [4, 0, 2]))
+    return A.transpose(0, 3, 1, 2)
+
+
+def FinishCorrelation_iTPS_CTM(A, C2, C3, eT1, eT2, eT3, Tn1, op):
+    """contract_itps_ctm/correlation.cpp:80-105: the right end with the
+    operator on Tn1."""
+    return trace(op, td(Tn1, td(td(A, eT1, ([0], [0])),
+                                td(cj(Tn1), td(eT3, td(C2, td(C3, eT2, ([0], [1])), ([1], [1])), ([0], [1])),
+                                   ([2, 3], [5, 2])),
+                                ([0, 2, 3, 5], [3, 0, 5, 1])),
+                        ([0, 1, 2, 3], [0, 1, 4, 3])),
+                 [0, 1], [0, 1])

@@ -1,7 +1,8 @@
 """Measurement drivers restated in numpy: one-site and two-site observables
 and the energy density (TeNeS src/iTPS/onesite_obs.cpp:35-123,
 twosite_obs.cpp:31-288, density.cpp:30-140) and the correlation length
-(correlation_length.cpp:24-124).  Operators are given as plain dicts:
+(correlation_length.cpp:24-124) and the correlation functions
+(correlation_function.cpp:44-220).  Operators are given as plain dicts:
 
   onesite: {"name", "group", "site", "op" (d, d), "coeff"}
   twosite: {"name", "group", "source", "dx", "dy", "coeff",
@@ -112,6 +113,60 @@ def density(onesite_vals, twosite_vals, lat, pdims, onesite_names, twosite_names
             energy += np.real(v)
     res["Energy"] = energy / nsites
     return res
+
+
+def measure_correlation(env, Tn, lat, site_ops, r_max, pairs):
+    """correlation_function.cpp:44-220 (iTPS, CTM environment): for every
+    left site and left operator, <O_left(0) O_right(r)> / <1 1> along +x
+    (dx = r) and +y (dy = r) for r = 1..r_max.  site_ops[(site, group)] is
+    the d x d operator; pairs = [(left_group, right_group)].  Returns
+    {(left_group, left_site, right_group, dx, dy): value}."""
+    out = {}
+    r_ops = {}
+    for lg, rg in pairs:
+        r_ops.setdefault(lg, []).append(rg)
+    rot = (3, 0, 1, 2, 4)
+    for left in range(lat.N_UNIT):
+        eye_l = np.eye(Tn[left].shape[4])
+        for lg, rgs in r_ops.items():
+            if (left, lg) not in site_ops:
+                continue
+            op = site_ops[(left, lg)]
+            # horizontal
+            A = K.StartCorrelation_iTPS_CTM(env.C1[left], env.C4[left], env.eTt[left], env.eTb[left],
+                                            env.eTl[left], Tn[left], op)
+            An = K.StartCorrelation_iTPS_CTM(env.C1[left], env.C4[left], env.eTt[left], env.eTb[left],
+                                             env.eTl[left], Tn[left], eye_l)
+            right = left
+            for r in range(r_max):
+                right = lat.right(right)
+                args = (env.C2[right], env.C3[right], env.eTt[right], env.eTr[right], env.eTb[right], Tn[right])
+                norm = K.FinishCorrelation_iTPS_CTM(An, *args, np.eye(Tn[right].shape[4]))
+                for rg in rgs:
+                    if (right, rg) in site_ops:
+                        out[(lg, left, rg, r + 1, 0)] = K.FinishCorrelation_iTPS_CTM(
+                            A, *args, site_ops[(right, rg)]) / norm
+                A = K.Transfer_iTPS_CTM(A, env.eTt[right], env.eTb[right], Tn[right])
+                An = K.Transfer_iTPS_CTM(An, env.eTt[right], env.eTb[right], Tn[right])
+            # vertical: the column seen as a row (site tensors rotated)
+            t = Tn[left].transpose(rot)
+            A = K.StartCorrelation_iTPS_CTM(env.C4[left], env.C3[left], env.eTl[left], env.eTr[left],
+                                            env.eTb[left], t, op)
+            An = K.StartCorrelation_iTPS_CTM(env.C4[left], env.C3[left], env.eTl[left], env.eTr[left],
+                                             env.eTb[left], t, eye_l)
+            right = left
+            for r in range(r_max):
+                right = lat.top(right)
+                t = Tn[right].transpose(rot)
+                args = (env.C1[right], env.C2[right], env.eTl[right], env.eTt[right], env.eTr[right], t)
+                norm = K.FinishCorrelation_iTPS_CTM(An, *args, np.eye(Tn[right].shape[4]))
+                for rg in rgs:
+                    if (right, rg) in site_ops:
+                        out[(lg, left, rg, 0, r + 1)] = K.FinishCorrelation_iTPS_CTM(
+                            A, *args, site_ops[(right, rg)]) / norm
+                A = K.Transfer_iTPS_CTM(A, env.eTl[right], env.eTr[right], t)
+                An = K.Transfer_iTPS_CTM(An, env.eTl[right], env.eTr[right], t)
+    return out
 
 
 def correlation_lengths(env, Tn, lat, prm: TM.TMParameters):

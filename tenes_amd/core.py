@@ -413,6 +413,18 @@ class Environment:
         self.ctx.check(self.ctx.lib.tenes_env_full_update(
             self.h, len(ops), L.iptr(src), L.iptr(leg), L.dptr(flat), nsteps, C.byref(pp), C.byref(fp)))
 
+    def correlation(self, left_site: int, left_op: np.ndarray, right_ops, r_max: int,
+                    direction: int = 0) -> np.ndarray:
+        """<O_left(site) O_j(site + r)> / <1 1> for r = 1..r_max along +x
+        (direction 0) or +y (1): array (r_max, len(right_ops))."""
+        lo = np.ascontiguousarray(left_op, dtype=np.float64)
+        ro = np.ascontiguousarray(np.stack([np.asarray(o, dtype=np.float64) for o in right_ops])) \
+            if len(right_ops) else np.zeros(1)
+        out = np.zeros(max(r_max * len(right_ops), 1), dtype=np.float64)
+        self.ctx.check(self.ctx.lib.tenes_env_correlation(
+            self.h, left_site, L.dptr(lo), len(right_ops), L.dptr(ro), r_max, direction, L.dptr(out)))
+        return out[: r_max * len(right_ops)].reshape(r_max, len(right_ops))
+
     def bond_energy(self, bonds, grad=None):
         """Energy per site of nearest-neighbour bond operators with the
         environment held fixed (tenes_env_bond_energy).

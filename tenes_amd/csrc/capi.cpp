@@ -717,6 +717,70 @@ int tenes_tensor_to_device(const tenes_tensor *t, double *dst) {
   });
 }
 
+int tenes_env_correlation(tenes_env *env, int left_site, const double *left_op,
+                          int nright, const double *right_ops, int r_max,
+                          int direction, double *out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(left_op && out && (nright == 0 || right_ops), "null pointer");
+    TN_REQUIRE(left_site >= 0 && left_site < env->lat.N_UNIT, "bad site index");
+    TN_REQUIRE(direction == 0 || direction == 1, "direction must be 0 or 1");
+    TN_REQUIRE(r_max >= 0 && nright >= 0, "negative count");
+    auto &a = env->arr;
+    tn::Context &c = *env->ctx->c;
+    const int64_t d = a[8][left_site].shape[4];
+    for (int i = 0; i < env->lat.N_UNIT; ++i)
+      TN_REQUIRE(a[8][i].shape[4] == d, "one physical dimension for all sites");
+    std::vector<double> eye(d * d, 0.0);
+    for (int64_t i = 0; i < d; ++i) eye[i * d + i] = 1.0;
+    const tn::Tensor I = tn::from_host(c, tn::Shape{d, d}, eye.data());
+    const tn::Tensor L = tn::from_host(c, tn::Shape{d, d}, left_op);
+    std::vector<tn::Tensor> R(nright);
+    for (int j = 0; j < nright; ++j)
+      R[j] = tn::from_host(c, tn::Shape{d, d}, right_ops + (size_t)j * d * d);
+    namespace K = tenes::itps::core;
+    const tn::Axes rot{3, 0, 1, 2, 4};
+    const int s = left_site;
+    tn::Tensor A, An;
+    // the column of a vertical correlation is handled as a row of the
+    // rotated site tensors (correlation_function.cpp:146-216)
+    if (direction == 0) {
+      K::StartCorrelation_iTPS_CTM(A, a[0][s], a[3][s], a[4][s], a[6][s], a[7][s], a[8][s], L);
+      K::StartCorrelation_iTPS_CTM(An, a[0][s], a[3][s], a[4][s], a[6][s], a[7][s], a[8][s], I);
+    } else {
+      const tn::Tensor t = tn::transpose(a[8][s], rot);
+      K::StartCorrelation_iTPS_CTM(A, a[3][s], a[2][s], a[7][s], a[5][s], a[6][s], t, L);
+      K::StartCorrelation_iTPS_CTM(An, a[3][s], a[2][s], a[7][s], a[5][s], a[6][s], t, I);
+    }
+    int r_site = s;
+    for (int r = 0; r < r_max; ++r) {
+      if (direction == 0) {
+        r_site = env->lat.right(r_site);
+        const int q = r_site;
+        const double norm = K::FinishCorrelation_iTPS_CTM(
+            An, a[1][q], a[2][q], a[4][q], a[5][q], a[6][q], a[8][q], I);
+        for (int j = 0; j < nright; ++j)
+          out[(size_t)r * nright + j] = K::FinishCorrelation_iTPS_CTM(
+              A, a[1][q], a[2][q], a[4][q], a[5][q], a[6][q], a[8][q], R[j]) / norm;
+        K::Transfer_iTPS_CTM(A, a[4][q], a[6][q], a[8][q]);
+        K::Transfer_iTPS_CTM(An, a[4][q], a[6][q], a[8][q]);
+      } else {
+        r_site = env->lat.top(r_site);
+        const int q = r_site;
+        const tn::Tensor t = tn::transpose(a[8][q], rot);
+        const double norm = K::FinishCorrelation_iTPS_CTM(
+            An, a[0][q], a[1][q], a[7][q], a[4][q], a[5][q], t, I);
+        for (int j = 0; j < nright; ++j)
+          out[(size_t)r * nright + j] = K::FinishCorrelation_iTPS_CTM(
+              A, a[0][q], a[1][q], a[7][q], a[4][q], a[5][q], t, R[j]) / norm;
+        K::Transfer_iTPS_CTM(A, a[7][q], a[5][q], t);
+        K::Transfer_iTPS_CTM(An, a[7][q], a[5][q], t);
+      }
+    }
+  });
+}
+
 int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
                                const double *op_source,
                                const double *op_target, double *out) {

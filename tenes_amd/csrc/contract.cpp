@@ -297,6 +297,49 @@ tensor Create_Environment_two_sites(const tensor &C1, const tensor &C2,
                          Axes{3, 1, 2, 0});
 }
 
+// contract_itps_ctm/correlation.cpp:34-58
+void StartCorrelation_iTPS_CTM(tensor &A, const tensor &C1, const tensor &C4,
+                               const tensor &eT1, const tensor &eT3,
+                               const tensor &eT4, const tensor &Tn1,
+                               const tensor &op) {
+  const tensor inner = tensordot(
+      tensordot(conj(Tn1), op, Axes{4}, Axes{1}),
+      tensordot(eT3, tensordot(C1, tensordot(C4, eT4, Axes{1}, Axes{0}),
+                               Axes{0}, Axes{1}),
+                Axes{1}, Axes{1}),
+      Axes{0, 3}, Axes{5, 2});
+  A = tn::tensordot_t(eT1, tensordot(Tn1, inner, Axes{0, 3, 4}, Axes{6, 4, 2}),
+                      Axes{0, 2, 3}, Axes{5, 0, 2}, Axes{0, 3, 1, 2});
+}
+
+// contract_itps_ctm/correlation.cpp:60-78
+void Transfer_iTPS_CTM(tensor &A, const tensor &eT1, const tensor &eT3,
+                       const tensor &Tn1) {
+  const tensor inner = tensordot(conj(Tn1), tensordot(A, eT3, Axes{1}, Axes{1}),
+                                 Axes{0, 3}, Axes{2, 5});
+  A = tn::tensordot_t(eT1, tensordot(Tn1, inner, Axes{0, 3, 4}, Axes{4, 6, 2}),
+                      Axes{0, 2, 3}, Axes{4, 0, 2}, Axes{0, 3, 1, 2});
+}
+
+// contract_itps_ctm/correlation.cpp:80-105
+double FinishCorrelation_iTPS_CTM(const tensor &A, const tensor &C2,
+                                  const tensor &C3, const tensor &eT1,
+                                  const tensor &eT2, const tensor &eT3,
+                                  const tensor &Tn1, const tensor &op) {
+  const tensor right = tensordot(
+      conj(Tn1),
+      tensordot(eT3, tensordot(C2, tensordot(C3, eT2, Axes{0}, Axes{1}),
+                               Axes{1}, Axes{1}),
+                Axes{0}, Axes{1}),
+      Axes{2, 3}, Axes{5, 2});
+  const tensor X = tensordot(
+      Tn1,
+      tensordot(tensordot(A, eT1, Axes{0}, Axes{0}), right, Axes{0, 2, 3, 5},
+                Axes{3, 0, 5, 1}),
+      Axes{0, 1, 2, 3}, Axes{0, 1, 4, 3});
+  return trace(op, X, Axes{0, 1}, Axes{0, 1});
+}
+
 }  // namespace core
 }  // namespace itps
 }  // namespace tenes

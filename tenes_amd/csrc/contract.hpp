@@ -60,6 +60,20 @@ tensor Create_Environment_two_sites(const tensor &C1, const tensor &C2,
                                     const tensor &eT5, const tensor &eT6,
                                     const tensor &Q1, const tensor &Q2);
 
+// contract_itps_ctm/correlation.cpp:34-105: a row of the environment
+// absorbed column by column, for <O(0) O(r)> along the row.
+// A: (e1r, e3r, n1r, n2r).
+void StartCorrelation_iTPS_CTM(tensor &A, const tensor &C1, const tensor &C4,
+                               const tensor &eT1, const tensor &eT3,
+                               const tensor &eT4, const tensor &Tn1,
+                               const tensor &op);
+void Transfer_iTPS_CTM(tensor &A, const tensor &eT1, const tensor &eT3,
+                       const tensor &Tn1);
+double FinishCorrelation_iTPS_CTM(const tensor &A, const tensor &C2,
+                                  const tensor &C3, const tensor &eT1,
+                                  const tensor &eT2, const tensor &eT3,
+                                  const tensor &Tn1, const tensor &op);
+
 }  // namespace core
 }  // namespace itps
 }  // namespace tenes

@@ -133,13 +133,14 @@ def input_fixture(name: str) -> dict:
          "dimensions": o["dimensions"], "elements": parse_elements(o["elements"], 4)}
         for o in t.get("evolution", {}).get("full", [])]
     fx["correlation_length"] = t.get("correlation_length", {})
+    fx["correlation"] = t.get("correlation", {})
     return fx
 
 
 def outputs(dirname: str, prefix: str = "") -> dict:
     d = REF / dirname
     res = {}
-    for f in ("density", "onesite_obs", "twosite_obs", "correlation_length"):
+    for f in ("density", "onesite_obs", "twosite_obs", "correlation_length", "correlation"):
         p = d / f"{prefix}{f}.dat"
         if p.exists():
             res[f] = read_dat(p)

@@ -434,6 +434,41 @@ def test_ctm_warm_start_rank_growth(gpu_ctx):
         assert worst < 1e-10, (split, worst)
 
 
+def test_afh_real_correlation_functions_on_gpu(gpu_ctx):
+    """Correlation functions <Sz(0) Sz(r)>, <Sz(0) Sx(r)>, r = 1..3 along +x
+    and +y from every site, on the GPU CTM environment of the
+    AntiferroHeisenberg_real site tensors, against TeNeS's correlation.dat
+    (48 values) to 1e-10."""
+    import golden_io as G
+
+    fx, lat, pdims, Tn = G.afh_real_site_tensors()
+    chi = fx["parameter"]["ctm"]["dimension"]
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=chi))
+    one, _, _, _ = G.operators(fx, lat.N_UNIT, True)
+    site_ops = {(o["site"], o["group"]): o["op"] for o in one}
+    r_max = fx["correlation"]["r_max"]
+    pairs = [tuple(x) for x in fx["correlation"]["operators"]]
+    gold = {}
+    for r in fx["outputs"]["correlation"]:
+        f = r.split()
+        gold[(int(f[0]), int(f[1]), int(f[2]), int(f[3]), int(f[4]))] = float(f[5])
+    n = 0
+    for left in range(4):
+        for lg in sorted({p[0] for p in pairs}):
+            rgs = [p[1] for p in pairs if p[0] == lg]
+            for direction in (0, 1):
+                vals = genv.correlation(left, site_ops[(left, lg)], [site_ops[(left, rg)] for rg in rgs],
+                                        r_max, direction)
+                for r in range(r_max):
+                    for j, rg in enumerate(rgs):
+                        key = (lg, left, rg, r + 1 if direction == 0 else 0, r + 1 if direction == 1 else 0)
+                        assert abs(vals[r, j] - gold[key]) < 1e-10, (key, vals[r, j], gold[key])
+                        n += 1
+    assert n == len(gold) == 48
+
+
 def test_heisenberg_d5_chi100_energy(gpu_ctx):
     """The benchmark problem (Heisenberg 2x2, D=5, chi=100) from the oracle's
     own simple-update site tensors: the GPU's fixed-sweep CTM energy per site,

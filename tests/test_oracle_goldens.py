@@ -195,3 +195,13 @@ def test_afh_real_end_to_end_goldens():
         assert (ref[0], ref[1]) == (d, f)
         assert abs(xi - ref[2]) < 1e-12
         assert np.max(np.abs(np.array(es) - np.array(ref[3:]))) < 1e-11
+    # correlation functions (correlation.dat: left_op left_site right_op dx dy re im)
+    site_ops = {(o["site"], o["group"]): o["op"] for o in one}
+    cor = itps.measure_correlation(env, Tn, lat, site_ops, fx["correlation"]["r_max"],
+                                   [tuple(x) for x in fx["correlation"]["operators"]])
+    rows = fx["outputs"]["correlation"]
+    assert len(rows) == len(cor) == 48
+    for r in rows:
+        f = r.split()
+        key = (int(f[0]), int(f[1]), int(f[2]), int(f[3]), int(f[4]))
+        assert abs(cor[key] - complex(float(f[5]), float(f[6]))) < 1e-12, (key, cor[key], f)
