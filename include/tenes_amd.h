@@ -250,6 +250,13 @@ int tenes_env_bond_energy(tenes_env *env, int nbonds, const int *source,
 /* Copy a tensor into caller-owned device memory (size elements, the
  * tensor's row-major order) and wait for the copy. */
 int tenes_tensor_to_device(const tenes_tensor *t, double *dst);
+/* Multisite observable, multisite_obs.cpp:46-220: <prod_i O_i> / <1> on the
+ * smallest cluster holding the source and the nothers sites (dx[i], dy[i])
+ * from it (Contract_iTPS_CTM: 1x1, 1x2, 2x1, 2x2 clusters); ops holds the
+ * (d, d) host operators of the source, then of each other site. */
+int tenes_env_contract_multisite(tenes_env *env, int source, int nothers,
+                                 const int *dx, const int *dy,
+                                 const double *ops, double *out);
 /* Correlation functions, correlation_function.cpp:44-220 with the
  * contractions of contract_itps_ctm/correlation.cpp:34-105:
  * out[r * nright + j] = <O_left(site) O_j(site + r+1)> / <1 1> for

@@ -109,8 +109,24 @@ def _c2x1(C, eTt, eTr, eTb, eTl, Tn, op):
     return trace(op[0][0], X, [0, 1], [0, 1])
 
 
+def _c2x2(C, eTt, eTr, eTb, eTl, Tn, op):
+    """contract_itps_ctm/ctm_2x2.cpp:25-104"""
+    T00, T01, T10, T11 = Tn[0][0], Tn[0][1], Tn[1][0], Tn[1][1]
+    tr = td(T01, td(td(cj(T01), op[0][1], ([4], [1])), td(eTt[1], td(C[1], eTr[0], ([1], [0])), ([1], [0])),
+                    ([1, 2], [2, 5])), ([1, 2, 4], [4, 6, 2]))
+    bl = td(T10, td(td(cj(T10), op[1][0], ([4], [1])), td(eTb[0], td(C[3], eTl[1], ([1], [0])), ([1], [0])),
+                    ([0, 3], [5, 2])), ([0, 3, 4], [6, 4, 2]))
+    br = td(T11, td(td(cj(T11), op[1][1], ([4], [1])), td(eTr[1], td(C[2], eTb[1], ([1], [0])), ([1], [0])),
+                    ([2, 3], [2, 5])), ([2, 3, 4], [4, 6, 2]))
+    rest = td(tr, td(bl, br, ([1, 3, 4], [0, 2, 5])), ([1, 3, 5], [3, 4, 5]))
+    top = td(eTt[0], td(C[0], eTl[0], ([0], [1])), ([0], [0]))
+    X = td(T00, td(top, td(cj(T00), rest, ([2, 3], [1, 4])), ([0, 2, 3, 5], [4, 1, 6, 0])),
+           ([0, 1, 2, 3], [1, 0, 3, 4]))
+    return trace(op[0][0], X, [0, 1], [0, 1])
+
+
 def Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op):
-    """contract_itps_ctm/ctm.cpp:45-87 for 1x1, 1x2, 2x1 clusters."""
+    """contract_itps_ctm/ctm.cpp:45-87 for 1x1, 1x2, 2x1 and 2x2 clusters."""
     nrow, ncol = len(Tn), len(Tn[0])
     if (nrow, ncol) == (1, 1):
         return _c1x1(C, eTt, eTr, eTb, eTl, Tn, op)
@@ -118,6 +134,8 @@ def Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op):
         return _c1x2(C, eTt, eTr, eTb, eTl, Tn, op)
     if (nrow, ncol) == (2, 1):
         return _c2x1(C, eTt, eTr, eTb, eTl, Tn, op)
+    if (nrow, ncol) == (2, 2):
+        return _c2x2(C, eTt, eTr, eTb, eTl, Tn, op)
     raise NotImplementedError(f"Contract_{nrow}_{ncol}_iTPS_CTM is not implemented")
 
 

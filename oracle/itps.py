@@ -115,6 +115,34 @@ def density(onesite_vals, twosite_vals, lat, pdims, onesite_names, twosite_names
     return res
 
 
+def measure_multisite(env, Tn, lat, site_ops, multisite_ops):
+    """multisite_obs.cpp:46-220 (CTM environment): multisite_ops =
+    [{"group", "source", "dx": [...], "dy": [...], "ops": [g0, g1, ...],
+    "coeff"}]; value = coeff * <prod O> / <1> on the smallest cluster holding
+    the sites.  Returns {(group, source, dx..., dy...): value}."""
+    out = {}
+    norms = {}
+    for m in multisite_ops:
+        src, dxs, dys = m["source"], list(m["dx"]), list(m["dy"])
+        mindx, maxdx = min([0] + dxs), max([0] + dxs)
+        mindy, maxdy = min([0] + dys), max([0] + dys)
+        ncol, nrow = maxdx - mindx + 1, maxdy - mindy + 1
+        scol, srow = -mindx, maxdy
+        idx = [[lat.other(src, c - scol, srow - r) for c in range(ncol)] for r in range(nrow)]
+        Tn_ = [[Tn[idx[r][c]] for c in range(ncol)] for r in range(nrow)]
+        ops_ = [[np.eye(Tn[idx[r][c]].shape[4]) for c in range(ncol)] for r in range(nrow)]
+        C, eTt, eTr, eTb, eTl = _cluster_args(env, idx, nrow, ncol)
+        key = (idx[nrow - 1][0], nrow - 1, ncol - 1)
+        if key not in norms:
+            norms[key] = K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
+        ops_[srow][scol] = site_ops[(src, m["ops"][0])]
+        for i, (dx, dy) in enumerate(zip(dxs, dys)):
+            ops_[srow - dy][scol + dx] = site_ops[(lat.other(src, dx, dy), m["ops"][i + 1])]
+        val = K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
+        out[(m["group"], src, *[v for pair in zip(dxs, dys) for v in pair])] = m.get("coeff", 1.0) * val / norms[key]
+    return out
+
+
 def measure_correlation(env, Tn, lat, site_ops, r_max, pairs):
     """correlation_function.cpp:44-220 (iTPS, CTM environment): for every
     left site and left operator, <O_left(0) O_right(r)> / <1 1> along +x

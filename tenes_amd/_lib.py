@@ -116,6 +116,8 @@ _SIGS = {
     "tenes_env_full_update": (
         C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_dbl_p, C.c_int, C.POINTER(PepsParameters),
                   C.POINTER(FuParameters)]),
+    "tenes_env_contract_multisite": (
+        C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p, c_int_p, c_dbl_p, c_dbl_p]),
     "tenes_env_correlation": (
         C.c_int, [C.c_void_p, C.c_int, c_dbl_p, C.c_int, c_dbl_p, C.c_int, C.c_int, c_dbl_p]),
     "tenes_env_bond_energy": (

@@ -413,6 +413,18 @@ class Environment:
         self.ctx.check(self.ctx.lib.tenes_env_full_update(
             self.h, len(ops), L.iptr(src), L.iptr(leg), L.dptr(flat), nsteps, C.byref(pp), C.byref(fp)))
 
+    def contract_multisite(self, source: int, dx, dy, ops) -> float:
+        """<prod O_i> / <1> of one-site operators on the source and the
+        sites (dx[i], dy[i]) from it (tenes_env_contract_multisite)."""
+        dxa = np.ascontiguousarray(dx, dtype=np.int32)
+        dya = np.ascontiguousarray(dy, dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(o, dtype=np.float64).ravel() for o in ops]))
+        out = C.c_double()
+        self.ctx.check(self.ctx.lib.tenes_env_contract_multisite(
+            self.h, source, len(dxa), L.iptr(dxa) if len(dxa) else None, L.iptr(dya) if len(dya) else None,
+            L.dptr(flat), C.byref(out)))
+        return out.value
+
     def correlation(self, left_site: int, left_op: np.ndarray, right_ops, r_max: int,
                     direction: int = 0) -> np.ndarray:
         """<O_left(site) O_j(site + r)> / <1 1> for r = 1..r_max along +x

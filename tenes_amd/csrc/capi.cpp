@@ -1,5 +1,6 @@
 // extern "C" boundary: handles around tn::Context / tn::Tensor and the
 // environment arrays, exception-free, one status code per call.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -714,6 +715,65 @@ int tenes_tensor_to_device(const tenes_tensor *t, double *dst) {
     TN_HIP_CHECK(hipMemcpyAsync(dst, x.data(), x.size() * sizeof(double),
                                 hipMemcpyDeviceToDevice, s));
     TN_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int tenes_env_contract_multisite(tenes_env *env, int source, int nothers,
+                                 const int *dx, const int *dy,
+                                 const double *ops, double *out) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    check_env(env);
+    TN_REQUIRE(out && ops && (nothers == 0 || (dx && dy)), "null pointer");
+    TN_REQUIRE(nothers >= 0 && source >= 0 && source < env->lat.N_UNIT,
+               "bad multisite operator");
+    auto &a = env->arr;
+    // the smallest cluster holding the sites, multisite_obs.cpp:52-155
+    int mindx = 0, maxdx = 0, mindy = 0, maxdy = 0;
+    for (int i = 0; i < nothers; ++i) {
+      mindx = std::min(mindx, dx[i]);
+      maxdx = std::max(maxdx, dx[i]);
+      mindy = std::min(mindy, dy[i]);
+      maxdy = std::max(maxdy, dy[i]);
+    }
+    const int ncol = maxdx - mindx + 1, nrow = maxdy - mindy + 1;
+    const int scol = -mindx, srow = maxdy;
+    std::vector<std::vector<int>> idx(nrow, std::vector<int>(ncol));
+    std::vector<const tn::Tensor *> C(4), eTt(ncol), eTr(nrow), eTb(ncol), eTl(nrow);
+    std::vector<std::vector<const tn::Tensor *>> Tn(nrow, std::vector<const tn::Tensor *>(ncol)),
+        op(nrow, std::vector<const tn::Tensor *>(ncol));
+    std::vector<tn::Tensor> keep;
+    keep.reserve(2 * nrow * ncol + 1);
+    for (int row = 0; row < nrow; ++row) {
+      for (int col = 0; col < ncol; ++col) {
+        idx[row][col] = env->lat.other(source, col - scol, srow - row);
+        Tn[row][col] = &a[8][idx[row][col]];
+        keep.push_back(identity_op(env->ctx, a[8][idx[row][col]].shape[4]));
+        op[row][col] = &keep.back();
+      }
+      eTl[row] = &a[7][idx[row][0]];
+      eTr[row] = &a[5][idx[row][ncol - 1]];
+    }
+    for (int col = 0; col < ncol; ++col) {
+      eTt[col] = &a[4][idx[0][col]];
+      eTb[col] = &a[6][idx[nrow - 1][col]];
+    }
+    C[0] = &a[0][idx[0][0]];
+    C[1] = &a[1][idx[0][ncol - 1]];
+    C[2] = &a[2][idx[nrow - 1][ncol - 1]];
+    C[3] = &a[3][idx[nrow - 1][0]];
+    const double norm =
+        tenes::itps::core::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op);
+    size_t off = 0;
+    for (int i = 0; i <= nothers; ++i) {
+      const int row = i == 0 ? srow : srow - dy[i - 1];
+      const int col = i == 0 ? scol : scol + dx[i - 1];
+      const int64_t d = a[8][idx[row][col]].shape[4];
+      keep.push_back(host_op(env->ctx, ops + off, d));
+      off += (size_t)(d * d);
+      op[row][col] = &keep.back();
+    }
+    *out = tenes::itps::core::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op) / norm;
   });
 }
 

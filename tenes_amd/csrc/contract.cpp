@@ -253,6 +253,49 @@ double contract_2x1(const std::vector<const tensor *> &C,
   return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
 }
 
+// contract_itps_ctm/ctm_2x2.cpp:25-104
+double contract_2x2(const std::vector<const tensor *> &C,
+                    const std::vector<const tensor *> &eTt,
+                    const std::vector<const tensor *> &eTr,
+                    const std::vector<const tensor *> &eTb,
+                    const std::vector<const tensor *> &eTl,
+                    const std::vector<std::vector<const tensor *>> &Tn,
+                    const std::vector<std::vector<const tensor *>> &op) {
+  const tensor &T00 = *Tn[0][0], &T01 = *Tn[0][1], &T10 = *Tn[1][0],
+               &T11 = *Tn[1][1];
+  const tensor tr = tensordot(
+      T01,
+      tensordot(tensordot(conj(T01), *op[0][1], Axes{4}, Axes{1}),
+                tensordot(*eTt[1], tensordot(*C[1], *eTr[0], Axes{1}, Axes{0}),
+                          Axes{1}, Axes{0}),
+                Axes{1, 2}, Axes{2, 5}),
+      Axes{1, 2, 4}, Axes{4, 6, 2});
+  const tensor bl = tensordot(
+      T10,
+      tensordot(tensordot(conj(T10), *op[1][0], Axes{4}, Axes{1}),
+                tensordot(*eTb[0], tensordot(*C[3], *eTl[1], Axes{1}, Axes{0}),
+                          Axes{1}, Axes{0}),
+                Axes{0, 3}, Axes{5, 2}),
+      Axes{0, 3, 4}, Axes{6, 4, 2});
+  const tensor br = tensordot(
+      T11,
+      tensordot(tensordot(conj(T11), *op[1][1], Axes{4}, Axes{1}),
+                tensordot(*eTr[1], tensordot(*C[2], *eTb[1], Axes{1}, Axes{0}),
+                          Axes{1}, Axes{0}),
+                Axes{2, 3}, Axes{2, 5}),
+      Axes{2, 3, 4}, Axes{4, 6, 2});
+  const tensor rest = tensordot(tr, tensordot(bl, br, Axes{1, 3, 4}, Axes{0, 2, 5}),
+                                Axes{1, 3, 5}, Axes{3, 4, 5});
+  const tensor top = tensordot(*eTt[0], tensordot(*C[0], *eTl[0], Axes{0}, Axes{1}),
+                               Axes{0}, Axes{0});
+  const tensor X = tensordot(
+      T00,
+      tensordot(top, tensordot(conj(T00), rest, Axes{2, 3}, Axes{1, 4}),
+                Axes{0, 2, 3, 5}, Axes{4, 1, 6, 0}),
+      Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
+  return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
+}
+
 }  // namespace
 
 double Contract_iTPS_CTM(const std::vector<const tensor *> &C,
@@ -267,6 +310,7 @@ double Contract_iTPS_CTM(const std::vector<const tensor *> &C,
   if (nrow == 1 && ncol == 1) return contract_1x1(C, eTt, eTr, eTb, eTl, Tn, op);
   if (nrow == 1 && ncol == 2) return contract_1x2(C, eTt, eTr, eTb, eTl, Tn, op);
   if (nrow == 2 && ncol == 1) return contract_2x1(C, eTt, eTr, eTb, eTl, Tn, op);
+  if (nrow == 2 && ncol == 2) return contract_2x2(C, eTt, eTr, eTb, eTl, Tn, op);
   std::ostringstream ss;
   ss << "Contract_" << nrow << "_" << ncol << "_iTPS_CTM is not implemented";
   throw tn::Error(ss.str(), -5);

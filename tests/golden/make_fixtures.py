@@ -124,6 +124,12 @@ def input_fixture(name: str) -> dict:
         else:
             e["ops"] = o["ops"]
         fx["twosite"].append(e)
+    fx["multisite"] = []
+    for o in obs.get("multisite", []):
+        rows = [[int(x) for x in l.split()] for l in o["multisites"].splitlines() if l.strip()]
+        fx["multisite"].append({"name": o["name"], "group": o["group"], "ops": o["ops"],
+                                "coeff": o.get("coeff", 1.0),
+                                "sites": [{"source": r[0], "dx": r[1::2], "dy": r[2::2]} for r in rows]})
     fx["evolution"] = [
         {"group": o.get("group", 0), "source_site": o["source_site"], "source_leg": o["source_leg"],
          "dimensions": o["dimensions"], "elements": parse_elements(o["elements"], 4)}
@@ -140,7 +146,8 @@ def input_fixture(name: str) -> dict:
 def outputs(dirname: str, prefix: str = "") -> dict:
     d = REF / dirname
     res = {}
-    for f in ("density", "onesite_obs", "twosite_obs", "correlation_length", "correlation"):
+    for f in ("density", "onesite_obs", "twosite_obs", "correlation_length", "correlation",
+              "multisite_obs_4"):
         p = d / f"{prefix}{f}.dat"
         if p.exists():
             res[f] = read_dat(p)

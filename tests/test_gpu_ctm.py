@@ -436,9 +436,10 @@ def test_ctm_warm_start_rank_growth(gpu_ctx):
 
 def test_afh_real_correlation_functions_on_gpu(gpu_ctx):
     """Correlation functions <Sz(0) Sz(r)>, <Sz(0) Sx(r)>, r = 1..3 along +x
-    and +y from every site, on the GPU CTM environment of the
-    AntiferroHeisenberg_real site tensors, against TeNeS's correlation.dat
-    (48 values) to 1e-10."""
+    and +y from every site, and the four-site SzSzSzSz observables on the 2x2
+    cluster, on the GPU CTM environment of the AntiferroHeisenberg_real site
+    tensors, against TeNeS's correlation.dat (48 values) and
+    multisite_obs_4.dat (8 values) to 1e-10."""
     import golden_io as G
 
     fx, lat, pdims, Tn = G.afh_real_site_tensors()
@@ -467,6 +468,20 @@ def test_afh_real_correlation_functions_on_gpu(gpu_ctx):
                         assert abs(vals[r, j] - gold[key]) < 1e-10, (key, vals[r, j], gold[key])
                         n += 1
     assert n == len(gold) == 48
+    # four-site observables on the 2x2 cluster (multisite_obs_4.dat)
+    rows = fx["outputs"]["multisite_obs_4"]
+    n = 0
+    for m in fx["multisite"]:
+        for s in m["sites"]:
+            src = s["source"]
+            ops = [site_ops[(src, m["ops"][0])]] + [site_ops[(lat.other(src, x, y), g)]
+                                                   for x, y, g in zip(s["dx"], s["dy"], m["ops"][1:])]
+            v = m["coeff"] * genv.contract_multisite(src, s["dx"], s["dy"], ops)
+            key = " ".join(str(x) for x in [m["group"], src] + [v_ for pair in zip(s["dx"], s["dy"]) for v_ in pair])
+            ref = [float(r.split()[8]) for r in rows if r.startswith(key + " ")]
+            assert len(ref) == 1 and abs(v - ref[0]) < 1e-10, (key, v, ref)
+            n += 1
+    assert n == len(rows) == 8
 
 
 def test_heisenberg_d5_chi100_energy(gpu_ctx):

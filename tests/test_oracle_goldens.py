@@ -205,3 +205,12 @@ def test_afh_real_end_to_end_goldens():
         f = r.split()
         key = (int(f[0]), int(f[1]), int(f[2]), int(f[3]), int(f[4]))
         assert abs(cor[key] - complex(float(f[5]), float(f[6]))) < 1e-12, (key, cor[key], f)
+    # four-site observables (multisite_obs_4.dat: group source dx1 dy1 ... re im)
+    ms = [dict(group=m["group"], ops=m["ops"], coeff=m["coeff"], **s) for m in fx["multisite"] for s in m["sites"]]
+    got = itps.measure_multisite(env, Tn, lat, site_ops, ms)
+    rows = fx["outputs"]["multisite_obs_4"]
+    assert len(rows) == len(got) == 8
+    for r in rows:
+        f = r.split()
+        key = tuple(int(x) for x in f[:8])
+        assert abs(got[key] - complex(float(f[8]), float(f[9]))) < 1e-12, (key, got[key], f)
