@@ -32,7 +32,8 @@ def parse_toml_subset(text: str) -> dict:
         if s in ("true", "false"):
             return s == "true"
         if s.startswith("["):
-            return json.loads(s.replace("'", '"'))
+            # multi-line arrays may end rows with a trailing comma
+            return json.loads(re.sub(r",\s*\]", "]", s.replace("'", '"')))
         try:
             return int(s)
         except ValueError:
@@ -156,7 +157,8 @@ def outputs(dirname: str, prefix: str = "") -> dict:
 
 def main():
     fixtures = {}
-    for name, prefix in (("TE_TFI", "TE_"), ("AntiferroHeisenberg_real", ""), ("J1J2_AFH", "")):
+    for name, prefix in (("TE_TFI", "TE_"), ("AntiferroHeisenberg_real", ""), ("J1J2_AFH", ""),
+                         ("Honeycomb", ""), ("Honeycomb_skew", "")):
         fx = input_fixture(name)
         fx["outputs"] = outputs(f"output_{name}", prefix)
         fixtures[name] = fx

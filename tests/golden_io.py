@@ -90,6 +90,37 @@ def density_table(rows, timed: bool):
     return out
 
 
+def pipeline_site_tensors(name: str):
+    """A TeNeS test input run through the oracle's input pipeline: initial
+    tensors (seed 11), the simple-update steps, then the full-update steps
+    (fast full update).  Returns (fixture, lattice, pdims, Tn)."""
+    from oracle import ctm, full_update as FU, simple_update as SU
+    from oracle.lattice import SquareLattice
+
+    fx = load(name)
+    LX, LY, skew, pdims, vdims, dirs, noises = unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=True)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    Tn, lam = SU.simple_update_steps(Tn, lam, lat, evolution(fx, True), fx["parameter"]["simple_update"]["num_step"])
+    p = ctm_parameters(fx)
+    Tn = FU.full_update(Tn, lat, evolution(fx, True, "evolution_full"), fx["parameter"]["full_update"]["num_step"],
+                        p, FU.FUParameters())
+    return fx, lat, pdims, Tn
+
+
+def ctm_parameters(fx):
+    """PEPSParameters of the oracle from [parameter.ctm] (dimension,
+    iteration_max)."""
+    from oracle import ctm
+
+    c = fx["parameter"]["ctm"]
+    kw = {"CHI": c["dimension"]}
+    if "iteration_max" in c:
+        kw["Max_CTM_Iteration"] = c["iteration_max"]
+    return ctm.PEPSParameters(**kw)
+
+
 def afh_real_site_tensors():
     """AntiferroHeisenberg_real: initial tensors (seed 11), 100 simple-update
     steps and 1 full-update step (fast full update) through the oracle, as the
