@@ -224,10 +224,12 @@ int tenes_env_contract_one_site(tenes_env *env, int site, const double *op,
 int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
                                       int dy, const double *op12,
                                       double *out);
-/* Contract_iTPS_CTM (contract_itps_ctm/ctm.cpp:45-87) for a 1x1, 1x2 or 2x1
- * cluster around the bond (source, dx, dy), with a one-site operator on the
- * source and one on the target (NULL = identity), twosite_obs.cpp:146-260.
- * With both operators NULL this is the bond norm of twosite_obs.cpp:168. */
+/* Contract_iTPS_CTM (contract_itps_ctm/ctm.cpp:45-87, ctm_NxM.cpp) for the
+ * (|dy|+1) x (|dx|+1) cluster spanned by the bond (source, dx, dy), with a
+ * one-site operator on the source and one on the target (NULL = identity),
+ * twosite_obs.cpp:146-260.  With both operators NULL this is the bond norm of
+ * twosite_obs.cpp:168.  |dx|, |dy| <= 3 (TENES_ERR_UNSUPPORTED beyond, where
+ * twosite_obs.cpp:53-60 skips the bond). */
 int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
                                const double *op_source,
                                const double *op_target, double *out);
@@ -252,7 +254,7 @@ int tenes_env_bond_energy(tenes_env *env, int nbonds, const int *source,
 int tenes_tensor_to_device(const tenes_tensor *t, double *dst);
 /* Multisite observable, multisite_obs.cpp:46-220: <prod_i O_i> / <1> on the
  * smallest cluster holding the source and the nothers sites (dx[i], dy[i])
- * from it (Contract_iTPS_CTM: 1x1, 1x2, 2x1, 2x2 clusters); ops holds the
+ * from it (Contract_iTPS_CTM: clusters up to 4x4); ops holds the
  * (d, d) host operators of the source, then of each other site. */
 int tenes_env_contract_multisite(tenes_env *env, int source, int nothers,
                                  const int *dx, const int *dy,

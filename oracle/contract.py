@@ -1,6 +1,9 @@
 """CTM-environment contractions restated in numpy
 (TeNeS src/iTPS/core/contract_itps_ctm/ctm.cpp, ctm_1x1.cpp, ctm_1x2.cpp,
-ctm_2x1.cpp, correlation.cpp, src/iTPS/core/full_update.cpp:51-83)."""
+ctm_2x1.cpp, ctm_2x2.cpp, correlation.cpp, src/iTPS/core/full_update.cpp:51-83;
+the other cluster shapes of ctm_1x3.cpp ... ctm_4x4.cpp by one row sweep of
+double-layer sites, _cluster_rows).  Test infrastructure: the checker of the
+GPU library, never part of it."""
 from __future__ import annotations
 
 import numpy as np
@@ -125,8 +128,45 @@ def _c2x2(C, eTt, eTr, eTb, eTl, Tn, op):
     return trace(op[0][0], X, [0, 1], [0, 1])
 
 
+def _ldot(A, la, B, lb):
+    """tensordot of two labelled tensors over every shared label; result
+    labels: free labels of A, then of B."""
+    sa = [i for i, l in enumerate(la) if l in lb]
+    sb = [lb.index(la[i]) for i in sa]
+    out = [l for l in la if l not in lb] + [l for l in lb if l not in la]
+    return td(A, B, (sa, sb)), out
+
+
+def _cluster_rows(C, eTt, eTr, eTb, eTl, Tn, op):
+    """Any nrow x ncol cluster (the networks of contract_itps_ctm/ctm_NxM.cpp):
+    leg orders of ctm.cpp:984-1192, site legs (l, u, r, d, p), the operator's
+    first index on the ket, its second on the bra (as in ctm_1x1.cpp:27-72).
+    Contracted row by row, top to bottom, with each site fused into its
+    double-layer tensor a = T . op . conj(T) first."""
+    nr, nc = len(Tn), len(Tn[0])
+    B, lb = C[0], [("l", 0), ("t", 0)]
+    for c in range(nc):
+        B, lb = _ldot(B, lb, eTt[c], [("t", c), ("t", c + 1), ("vk", 0, c), ("vb", 0, c)])
+    B, lb = _ldot(B, lb, C[1], [("t", nc), ("r", 0)])
+    for r in range(nr):
+        B, lb = _ldot(B, lb, eTl[r], [("l", r + 1), ("l", r), ("hk", r, 0), ("hb", r, 0)])
+        for c in range(nc):
+            a = np.einsum("lurdp,pq,LURDq->lLuUrRdD", Tn[r][c], op[r][c], cj(Tn[r][c]))
+            B, lb = _ldot(B, lb, a, [("hk", r, c), ("hb", r, c), ("vk", r, c), ("vb", r, c),
+                                     ("hk", r, c + 1), ("hb", r, c + 1),
+                                     ("vk", r + 1, c), ("vb", r + 1, c)])
+        B, lb = _ldot(B, lb, eTr[r], [("r", r), ("r", r + 1), ("hk", r, nc), ("hb", r, nc)])
+    B, lb = _ldot(B, lb, C[3], [("b", 0), ("l", nr)])
+    for c in range(nc):
+        B, lb = _ldot(B, lb, eTb[c], [("b", c + 1), ("b", c), ("vk", nr, c), ("vb", nr, c)])
+    B, lb = _ldot(B, lb, C[2], [("r", nr), ("b", nc)])
+    assert not lb
+    return complex(B)
+
+
 def Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op):
-    """contract_itps_ctm/ctm.cpp:45-87 for 1x1, 1x2, 2x1 and 2x2 clusters."""
+    """contract_itps_ctm/ctm.cpp:45-87: the 1x1, 1x2, 2x1 and 2x2 trees of
+    ctm_1x1/1x2/2x1/2x2.cpp, any other cluster up to 4x4 by _cluster_rows."""
     nrow, ncol = len(Tn), len(Tn[0])
     if (nrow, ncol) == (1, 1):
         return _c1x1(C, eTt, eTr, eTb, eTl, Tn, op)
@@ -136,6 +176,8 @@ def Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op):
         return _c2x1(C, eTt, eTr, eTb, eTl, Tn, op)
     if (nrow, ncol) == (2, 2):
         return _c2x2(C, eTt, eTr, eTb, eTl, Tn, op)
+    if 1 <= nrow <= 4 and 1 <= ncol <= 4:
+        return _cluster_rows(C, eTt, eTr, eTb, eTl, Tn, op)
     raise NotImplementedError(f"Contract_{nrow}_{ncol}_iTPS_CTM is not implemented")
 
 

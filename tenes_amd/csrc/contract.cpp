@@ -3,6 +3,7 @@
 // final scalar is a device dot product read back once.
 #include "contract.hpp"
 
+#include <algorithm>
 #include <sstream>
 
 namespace tenes {
@@ -296,7 +297,139 @@ double contract_2x2(const std::vector<const tensor *> &C,
   return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
 }
 
+
+// A tensor with one integer label per leg.
+struct Labeled {
+  tensor t;
+  std::vector<int> lab;
+};
+
+// Contracts every label the operands share; the result carries the free
+// legs of A, then those of B (the tensordot order).
+Labeled dot(const Labeled &A, const Labeled &B) {
+  Axes ax, bx;
+  std::vector<int> out;
+  for (size_t i = 0; i < A.lab.size(); ++i) {
+    auto it = std::find(B.lab.begin(), B.lab.end(), A.lab[i]);
+    if (it == B.lab.end()) {
+      out.push_back(A.lab[i]);
+    } else {
+      ax.push_back((int)i);
+      bx.push_back((int)(it - B.lab.begin()));
+    }
+  }
+  for (int l : B.lab)
+    if (std::find(A.lab.begin(), A.lab.end(), l) == A.lab.end())
+      out.push_back(l);
+  TN_REQUIRE(!ax.empty(), "cluster sweep: operands share no leg");
+  return {tensordot(A.t, B.t, ax, bx), out};
+}
+
+// Full contraction of two tensors carrying the same labels.
+double close(const Labeled &A, const Labeled &B) {
+  TN_REQUIRE(A.lab.size() == B.lab.size(), "cluster sweep: open legs left");
+  Axes ax, bx;
+  for (size_t i = 0; i < A.lab.size(); ++i) {
+    auto it = std::find(B.lab.begin(), B.lab.end(), A.lab[i]);
+    TN_REQUIRE(it != B.lab.end(), "cluster sweep: open legs left");
+    ax.push_back((int)i);
+    bx.push_back((int)(it - B.lab.begin()));
+  }
+  return trace(A.t, B.t, ax, bx);
+}
+
 }  // namespace
+
+// Leg labels of the cluster network (C/eT leg orders of ctm.cpp:984-1192,
+// site legs (l, u, r, d, p)); the operator's first index meets the ket's
+// physical leg, its second the bra's, as in the ctm_NxM.cpp trees.
+double contract_cluster_sweep(const std::vector<const tensor *> &C,
+                              const std::vector<const tensor *> &eTt,
+                              const std::vector<const tensor *> &eTr,
+                              const std::vector<const tensor *> &eTb,
+                              const std::vector<const tensor *> &eTl,
+                              const std::vector<std::vector<const tensor *>> &Tn,
+                              const std::vector<std::vector<const tensor *>> &op) {
+  const int nr = (int)Tn.size();
+  const int nc = nr ? (int)Tn[0].size() : 0;
+  TN_REQUIRE(nr > 0 && nc > 0 && C.size() == 4 && (int)eTt.size() == nc &&
+                 (int)eTb.size() == nc && (int)eTl.size() == nr &&
+                 (int)eTr.size() == nr && (int)op.size() == nr,
+             "cluster sweep: inconsistent cluster");
+  // ring bonds: top chain top(0..nc), right chain rgt(0..nr), bottom chain
+  // bot(0..nc) (bot(0) at C4), left chain lft(0..nr) (lft(0) at C1); ket /
+  // bra bonds: horizontal h(r, 0..nc), vertical v(0..nr, c); physical p(r, c)
+  const int nh = nr * (nc + 1), nv = (nr + 1) * nc;
+  auto top = [&](int c) { return c; };
+  auto rgt = [&](int r) { return nc + 1 + r; };
+  auto bot = [&](int c) { return nc + nr + 2 + c; };
+  auto lft = [&](int r) { return 2 * nc + nr + 3 + r; };
+  const int base = 2 * (nc + nr) + 4;
+  auto hk = [&](int r, int c) { return base + r * (nc + 1) + c; };
+  auto hb = [&](int r, int c) { return base + nh + r * (nc + 1) + c; };
+  auto vk = [&](int r, int c) { return base + 2 * nh + r * nc + c; };
+  auto vb = [&](int r, int c) { return base + 2 * nh + nv + r * nc + c; };
+  auto ph = [&](int r, int c) { return base + 2 * nh + 2 * nv + r * nc + c; };
+
+  const Labeled c1{*C[0], {lft(0), top(0)}};
+  const Labeled c2{*C[1], {top(nc), rgt(0)}};
+  const Labeled c3{*C[2], {rgt(nr), bot(nc)}};
+  const Labeled c4{*C[3], {bot(0), lft(nr)}};
+  auto et = [&](int c) {
+    return Labeled{*eTt[c], {top(c), top(c + 1), vk(0, c), vb(0, c)}};
+  };
+  auto er = [&](int r) {
+    return Labeled{*eTr[r], {rgt(r), rgt(r + 1), hk(r, nc), hb(r, nc)}};
+  };
+  auto eb = [&](int c) {
+    return Labeled{*eTb[c], {bot(c + 1), bot(c), vk(nr, c), vb(nr, c)}};
+  };
+  auto el = [&](int r) {
+    return Labeled{*eTl[r], {lft(r + 1), lft(r), hk(r, 0), hb(r, 0)}};
+  };
+  auto ket = [&](int r, int c) {
+    return Labeled{*Tn[r][c],
+                   {hk(r, c), vk(r, c), hk(r, c + 1), vk(r + 1, c), ph(r, c)}};
+  };
+  auto bra = [&](int r, int c) {
+    TN_REQUIRE((int)op[r].size() == nc && op[r][c] && Tn[r][c],
+               "cluster sweep: missing site tensor or operator");
+    return Labeled{tensordot(conj(*Tn[r][c]), *op[r][c], Axes{4}, Axes{1}),
+                   {hb(r, c), vb(r, c), hb(r, c + 1), vb(r + 1, c), ph(r, c)}};
+  };
+
+  if (nr <= nc) {  // columns left to right; the boundary spans nr rows
+    Labeled B = c1;
+    for (int r = 0; r < nr; ++r) B = dot(B, el(r));
+    B = dot(B, c4);
+    for (int c = 0; c < nc; ++c) {
+      B = dot(B, et(c));
+      for (int r = 0; r < nr; ++r) {
+        B = dot(B, ket(r, c));
+        B = dot(B, bra(r, c));
+      }
+      B = dot(B, eb(c));
+    }
+    B = dot(B, c2);
+    for (int r = 0; r < nr; ++r) B = dot(B, er(r));
+    return close(B, c3);
+  }
+  // rows top to bottom; the boundary spans nc columns
+  Labeled B = c1;
+  for (int c = 0; c < nc; ++c) B = dot(B, et(c));
+  B = dot(B, c2);
+  for (int r = 0; r < nr; ++r) {
+    B = dot(B, el(r));
+    for (int c = 0; c < nc; ++c) {
+      B = dot(B, ket(r, c));
+      B = dot(B, bra(r, c));
+    }
+    B = dot(B, er(r));
+  }
+  B = dot(B, c4);
+  for (int c = 0; c < nc; ++c) B = dot(B, eb(c));
+  return close(B, c3);
+}
 
 double Contract_iTPS_CTM(const std::vector<const tensor *> &C,
                          const std::vector<const tensor *> &eTt,
@@ -311,6 +444,8 @@ double Contract_iTPS_CTM(const std::vector<const tensor *> &C,
   if (nrow == 1 && ncol == 2) return contract_1x2(C, eTt, eTr, eTb, eTl, Tn, op);
   if (nrow == 2 && ncol == 1) return contract_2x1(C, eTt, eTr, eTb, eTl, Tn, op);
   if (nrow == 2 && ncol == 2) return contract_2x2(C, eTt, eTr, eTb, eTl, Tn, op);
+  if (nrow >= 1 && nrow <= 4 && ncol >= 1 && ncol <= 4)
+    return contract_cluster_sweep(C, eTt, eTr, eTb, eTl, Tn, op);
   std::ostringstream ss;
   ss << "Contract_" << nrow << "_" << ncol << "_iTPS_CTM is not implemented";
   throw tn::Error(ss.str(), -5);
