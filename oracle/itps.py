@@ -2,7 +2,9 @@
 and the energy density (TeNeS src/iTPS/onesite_obs.cpp:35-123,
 twosite_obs.cpp:31-288, density.cpp:30-140) and the correlation length
 (correlation_length.cpp:24-124) and the correlation functions
-(correlation_function.cpp:44-220).  Operators are given as plain dicts:
+(correlation_function.cpp:44-220).  The contraction module K is
+oracle.contract (iTPS, double layer) or oracle.contract_density (iTPO,
+single layer; TPO site tensors with two physical legs).  Operators are given as plain dicts:
 
   onesite: {"name", "group", "site", "op" (d, d), "coeff"}
   twosite: {"name", "group", "source", "dx", "dy", "coeff",
@@ -16,7 +18,7 @@ from . import contract as K
 from . import transfer as TM
 
 
-def measure_onesite(env, Tn, lat, onesite_ops):
+def measure_onesite(env, Tn, lat, onesite_ops, K=K):
     """onesite_obs.cpp:85-98.  Returns ({group: {site: value}}, norms)."""
     norms = []
     for i in range(lat.N_UNIT):
@@ -52,7 +54,7 @@ def _cluster_args(env, idx, nrow, ncol):
     return C, eTt, eTr, eTb, eTl
 
 
-def measure_twosite(env, Tn, lat, twosite_ops, onesite_ops):
+def measure_twosite(env, Tn, lat, twosite_ops, onesite_ops, K=K):
     """twosite_obs.cpp:46-262.  Returns ({group: {(source, dx, dy): value}}, norms)."""
     site_op = {(o["site"], o["group"]): o["op"] for o in onesite_ops}
     norms = {}
@@ -115,7 +117,7 @@ def density(onesite_vals, twosite_vals, lat, pdims, onesite_names, twosite_names
     return res
 
 
-def measure_multisite(env, Tn, lat, site_ops, multisite_ops):
+def measure_multisite(env, Tn, lat, site_ops, multisite_ops, K=K):
     """multisite_obs.cpp:46-220 (CTM environment): multisite_ops =
     [{"group", "source", "dx": [...], "dy": [...], "ops": [g0, g1, ...],
     "coeff"}]; value = coeff * <prod O> / <1> on the smallest cluster holding
@@ -143,7 +145,7 @@ def measure_multisite(env, Tn, lat, site_ops, multisite_ops):
     return out
 
 
-def measure_correlation(env, Tn, lat, site_ops, r_max, pairs):
+def measure_correlation(env, Tn, lat, site_ops, r_max, pairs, K=K):
     """correlation_function.cpp:44-220 (iTPS, CTM environment): for every
     left site and left operator, <O_left(0) O_right(r)> / <1 1> along +x
     (dx = r) and +y (dy = r) for r = 1..r_max.  site_ops[(site, group)] is
@@ -153,7 +155,7 @@ def measure_correlation(env, Tn, lat, site_ops, r_max, pairs):
     r_ops = {}
     for lg, rg in pairs:
         r_ops.setdefault(lg, []).append(rg)
-    rot = (3, 0, 1, 2, 4)
+    rot = (3, 0, 1, 2, 4) if Tn[0].ndim == 5 else (3, 0, 1, 2, 4, 5)
     for left in range(lat.N_UNIT):
         eye_l = np.eye(Tn[left].shape[4])
         for lg, rgs in r_ops.items():

@@ -36,6 +36,16 @@ class TransferMatrixCTM:
     def __init__(self, lat, Tn, env):
         self.lat = lat
         self.Tn = Tn
+        if env.eTt[0].ndim == 3:
+            # iTPO edges (chi, chi, D) (transfer_matrix.cpp:204-218, 399-409):
+            # the same contractions with a bra leg of dimension one
+            from .ctm import Env
+            e4 = Env(len(env.C1))
+            for k in ("C1", "C2", "C3", "C4"):
+                setattr(e4, k, list(getattr(env, k)))
+            for k in ("eTt", "eTr", "eTb", "eTl"):
+                setattr(e4, k, [t[..., None] for t in getattr(env, k)])
+            env = e4
         self.e = env
 
     def dim(self, d, fixed):
