@@ -184,6 +184,24 @@ int tenes_env_get(tenes_env *env, int kind, int site, tenes_tensor **out);
  * measure (either may be NULL). */
 int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
                                int initialize, int *count, double *sig_max);
+/* Calc_CTM_Environment_density, ctm_single.cpp:737-1057: the single-layer
+ * CTM environment of an iTPO (finite-temperature density operator).  The
+ * environment's site tensors (TENES_TN) are TPOs (D_l, D_u, D_r, D_d, d, d)
+ * (physical ket leg, then bra leg); the CTM runs on their traces over the
+ * physical legs (Make_single_tensor_density, ctm_single.cpp:43-50).  The
+ * initial corners and edges keep the site bond dimensions (no padding to
+ * CHI, ctm_single.cpp:751-995); edges are stored as (chi, chi, D, 1).
+ * Corner projectors only (CTM_Projector_corner != 0).  Once the site
+ * tensors are TPOs, the contraction entry points below (one-site, op12,
+ * cluster, multisite, correlation, transfer matrix) evaluate TeNeS's
+ * density versions (contract_density_ctm/): Tr(rho O) with each one-site
+ * operator O[a][b] traced against the TPO's (ket a, bra b) legs, and an op12
+ * operator split into one-site products by its SVD, as twosite_obs.cpp:
+ * 223-243 does for longer bonds. */
+int tenes_calc_ctm_environment_density(tenes_env *env,
+                                       const tenes_peps_parameters *p,
+                                       int initialize, int *count,
+                                       double *sig_max);
 /* Calc_CTM_Environment (ctm.cpp:971-1253) on n independent environments
  * at once.  Environments that belong to one context run in order on one host
  * thread; different contexts run concurrently on their own streams, so

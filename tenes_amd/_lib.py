@@ -99,6 +99,8 @@ _SIGS = {
     "tenes_env_get": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_void_pp]),
     "tenes_calc_ctm_environment": (
         C.c_int, [C.c_void_p, C.POINTER(PepsParameters), C.c_int, c_int_p, c_dbl_p]),
+    "tenes_calc_ctm_environment_density": (
+        C.c_int, [C.c_void_p, C.POINTER(PepsParameters), C.c_int, c_int_p, c_dbl_p]),
     "tenes_env_move": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(PepsParameters)]),
     "tenes_env_save_corners": (C.c_int, [C.c_void_p]),
     "tenes_env_check_convergence": (

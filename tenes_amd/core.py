@@ -349,6 +349,15 @@ class Environment:
             self.h, C.byref(params), 1 if initialize else 0, C.byref(n), C.byref(sig)))
         return n.value, sig.value
 
+    def calc_ctm_environment_density(self, params, initialize: bool = True):
+        """Single-layer CTM of an iTPO (site tensors (D, D, D, D, d, d)),
+        Calc_CTM_Environment_density of ctm_single.cpp:737-1057."""
+        n = C.c_int()
+        sig = C.c_double()
+        self.ctx.check(self.ctx.lib.tenes_calc_ctm_environment_density(
+            self.h, C.byref(params), 1 if initialize else 0, C.byref(n), C.byref(sig)))
+        return n.value, sig.value
+
     def move(self, direction: int, coord: int, params):
         self.ctx.check(self.ctx.lib.tenes_env_move(self.h, direction, coord, C.byref(params)))
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <string>
@@ -16,6 +17,7 @@
 #include "full_update.hpp"
 #include "linalg.hpp"
 #include "simple_update.hpp"
+#include "small_dense.hpp"
 #include "transfer.hpp"
 
 struct tenes_ctx {
@@ -112,6 +114,50 @@ tn::Tensor identity_op(tenes_ctx *ctx, int64_t d) {
   return tn::from_host(*ctx->c, tn::Shape{d, d}, id.data());
 }
 
+// iTPO environments hold site tensors (D,D,D,D,d,d) (phys_ket, phys_bra)
+// and are contracted as TeNeS's single-layer networks
+// (contract_density_ctm/, ctm_single.cpp): every site enters as its TPO
+// traced against the one-site operator placed on it, sum_ab T[..,a,b]
+// op[a,b], shaped (D,D,D,D,1), under the context's single-layer mode and
+// with a 1x1 identity in the operator's place.
+bool is_tpo(const tenes_env *env) {
+  return env->arr[TENES_TN][0].rank() == 6;
+}
+
+tn::Tensor tpo_ket(const tn::Tensor &T, const tn::Tensor &op) {
+  TN_REQUIRE(T.rank() == 6 && op.rank() == 2 && op.shape[0] == T.shape[4] &&
+                 op.shape[1] == T.shape[5],
+             "TPO site tensor and operator do not match");
+  return tn::reshape(tn::tensordot(T, op, tn::Axes{4, 5}, tn::Axes{0, 1}),
+                     tn::Shape{T.shape[0], T.shape[1], T.shape[2], T.shape[3], 1});
+}
+
+using Grid = std::vector<std::vector<const tn::Tensor *>>;
+
+// Contract_iTPS_CTM, or Contract_density_CTM (contract_density_ctm/ctm.cpp:
+// 45-85) for an iTPO environment.
+double cluster_value(tenes_env *env, const std::vector<const tn::Tensor *> &C,
+                     const std::vector<const tn::Tensor *> &eTt,
+                     const std::vector<const tn::Tensor *> &eTr,
+                     const std::vector<const tn::Tensor *> &eTb,
+                     const std::vector<const tn::Tensor *> &eTl,
+                     const Grid &Tn, const Grid &op) {
+  namespace K = tenes::itps::core;
+  if (!is_tpo(env)) return K::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op);
+  std::deque<tn::Tensor> keep;
+  keep.push_back(identity_op(env->ctx, 1));
+  const tn::Tensor *one = &keep.back();
+  Grid kets = Tn, ones = op;
+  for (size_t r = 0; r < Tn.size(); ++r)
+    for (size_t c = 0; c < Tn[r].size(); ++c) {
+      keep.push_back(tpo_ket(*Tn[r][c], *op[r][c]));
+      kets[r][c] = &keep.back();
+      ones[r][c] = one;
+    }
+  tn::SingleLayer single(*env->ctx->c, true);
+  return K::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, kets, ones);
+}
+
 void check_env(const tenes_env *env) {
   TN_REQUIRE(env != nullptr, "null environment");
   for (int k = 0; k < 9; ++k)
@@ -151,6 +197,41 @@ Cluster cluster_of(const tenes::itps::SquareLattice &lat, int source, int dx,
       c.idx[row][col] =
           lat.other(source, col - c.source_col, c.source_row - row);
   return c;
+}
+
+
+// The cluster of a bond (twosite_obs.cpp:119-260) with one-site operators on
+// its source and target (nullptr = identity).
+double bond_cluster_value(tenes_env *env, const Cluster &c,
+                          const tn::Tensor *op_source,
+                          const tn::Tensor *op_target) {
+  auto &a = env->arr;
+  std::vector<const tn::Tensor *> C(4), eTt(c.ncol), eTr(c.nrow), eTb(c.ncol),
+      eTl(c.nrow);
+  Grid Tn(c.nrow, std::vector<const tn::Tensor *>(c.ncol)),
+      op(c.nrow, std::vector<const tn::Tensor *>(c.ncol));
+  std::deque<tn::Tensor> keep;
+  for (int row = 0; row < c.nrow; ++row) {
+    for (int col = 0; col < c.ncol; ++col) {
+      const int idx = c.idx[row][col];
+      Tn[row][col] = &a[8][idx];
+      keep.push_back(identity_op(env->ctx, a[8][idx].shape[4]));
+      op[row][col] = &keep.back();
+    }
+    eTl[row] = &a[7][c.idx[row][0]];
+    eTr[row] = &a[5][c.idx[row][c.ncol - 1]];
+  }
+  for (int col = 0; col < c.ncol; ++col) {
+    eTt[col] = &a[4][c.idx[0][col]];
+    eTb[col] = &a[6][c.idx[c.nrow - 1][col]];
+  }
+  C[0] = &a[0][c.idx[0][0]];
+  C[1] = &a[1][c.idx[0][c.ncol - 1]];
+  C[2] = &a[2][c.idx[c.nrow - 1][c.ncol - 1]];
+  C[3] = &a[3][c.idx[c.nrow - 1][0]];
+  if (op_source) op[c.source_row][c.source_col] = op_source;
+  if (op_target) op[c.target_row][c.target_col] = op_target;
+  return cluster_value(env, C, eTt, eTr, eTb, eTl, Tn, op);
 }
 
 }  // namespace
@@ -486,6 +567,33 @@ int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
   });
 }
 
+int tenes_calc_ctm_environment_density(tenes_env *env,
+                                       const tenes_peps_parameters *p,
+                                       int initialize, int *count,
+                                       double *sig_max) {
+  if (!env) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    for (const auto &t : env->arr[TENES_TN])
+      TN_REQUIRE(t.defined() && t.rank() == 6,
+                 "iTPO site tensors (D,D,D,D,d,d) not set");
+    auto &a = env->arr;
+    // Make_single_tensor_density (ctm_single.cpp:43-50)
+    std::vector<tn::Tensor> A(a[8].size());
+    for (size_t i = 0; i < A.size(); ++i)
+      A[i] = tpo_ket(a[8][i], identity_op(env->ctx, a[8][i].shape[4]));
+    tenes::itps::PEPS_Parameters pp = to_params(p);
+    TN_REQUIRE(pp.CTM_Projector_corner,
+               "the single-layer CTM supports corner projectors only");
+    tn::SingleLayer single(*env->ctx->c, true);
+    double sm = 0.0;
+    const int n = tenes::itps::core::Calc_CTM_Environment(
+        a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], A, pp, env->lat,
+        initialize != 0, &sm);
+    if (count) *count = n;
+    if (sig_max) *sig_max = sm;
+  });
+}
+
 int tenes_batch_calc_ctm_environment(tenes_env *const *envs, int n,
                                      const tenes_peps_parameters *p,
                                      int nparams, int initialize, int *count,
@@ -610,6 +718,14 @@ int tenes_env_contract_one_site(tenes_env *env, int site, const double *op,
     auto &a = env->arr;
     const int64_t d = a[8][site].shape[4];
     tn::Tensor o = op ? host_op(env->ctx, op, d) : identity_op(env->ctx, d);
+    if (is_tpo(env)) {  // Contract_one_site_density_CTM, ctm.cpp:87-112
+      const tn::Tensor ket = tpo_ket(a[8][site], o);
+      tn::SingleLayer single(*env->ctx->c, true);
+      *out = tenes::itps::core::Contract_one_site_iTPS_CTM(
+          a[0][site], a[1][site], a[2][site], a[3][site], a[4][site],
+          a[5][site], a[6][site], a[7][site], ket, identity_op(env->ctx, 1));
+      return;
+    }
     *out = tenes::itps::core::Contract_one_site_iTPS_CTM(
         a[0][site], a[1][site], a[2][site], a[3][site], a[4][site], a[5][site],
         a[6][site], a[7][site], a[8][site], o);
@@ -627,6 +743,36 @@ int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
     TN_REQUIRE(op12 && out, "null pointer");
     auto &a = env->arr;
     Cluster c = cluster_of(env->lat, source, dx, dy);
+    if (is_tpo(env)) {
+      // the operator split into products of one-site operators by its SVD
+      // over (source legs | target legs), twosite_obs.cpp:223-243; each
+      // product is one single-layer bond cluster
+      const int64_t d1 = a[8][source].shape[4];
+      const int64_t d2 = a[8][c.idx[c.target_row][c.target_col]].shape[4];
+      std::vector<double> M((size_t)(d1 * d1 * d2 * d2));
+      for (int64_t p = 0; p < d1; ++p)
+        for (int64_t q = 0; q < d2; ++q)
+          for (int64_t r = 0; r < d1; ++r)
+            for (int64_t u = 0; u < d2; ++u)
+              M[(size_t)((p * d1 + r) * d2 * d2 + q * d2 + u)] =
+                  op12[((p * d2 + q) * d1 + r) * d2 + u];
+      std::vector<double> U, s, VT;
+      tn::host::svd(M, (int)(d1 * d1), (int)(d2 * d2), U, s, VT);
+      const int k = (int)s.size();
+      double value = 0.0;
+      for (int n = 0; n < k; ++n) {
+        if (s[n] == 0.0) continue;
+        std::vector<double> A((size_t)(d1 * d1)), B((size_t)(d2 * d2));
+        for (int64_t i = 0; i < d1 * d1; ++i) A[i] = U[(size_t)(i * k + n)];
+        for (int64_t i = 0; i < d2 * d2; ++i)
+          B[i] = VT[(size_t)(n * d2 * d2 + i)];
+        const tn::Tensor oa = host_op(env->ctx, A.data(), d1);
+        const tn::Tensor ob = host_op(env->ctx, B.data(), d2);
+        value += s[n] * bond_cluster_value(env, c, &oa, &ob);
+      }
+      *out = value;
+      return;
+    }
     if (c.nrow == 2) {
       const int top = c.idx[0][0], bottom = c.idx[1][0];
       const int64_t d1 = a[8][top].shape[4], d2 = a[8][bottom].shape[4];
@@ -763,7 +909,7 @@ int tenes_env_contract_multisite(tenes_env *env, int source, int nothers,
     C[2] = &a[2][idx[nrow - 1][ncol - 1]];
     C[3] = &a[3][idx[nrow - 1][0]];
     const double norm =
-        tenes::itps::core::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op);
+        cluster_value(env, C, eTt, eTr, eTb, eTl, Tn, op);
     size_t off = 0;
     for (int i = 0; i <= nothers; ++i) {
       const int row = i == 0 ? srow : srow - dy[i - 1];
@@ -773,7 +919,7 @@ int tenes_env_contract_multisite(tenes_env *env, int source, int nothers,
       off += (size_t)(d * d);
       op[row][col] = &keep.back();
     }
-    *out = tenes::itps::core::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op) / norm;
+    *out = cluster_value(env, C, eTt, eTr, eTb, eTl, Tn, op) / norm;
   });
 }
 
@@ -801,39 +947,52 @@ int tenes_env_correlation(tenes_env *env, int left_site, const double *left_op,
       R[j] = tn::from_host(c, tn::Shape{d, d}, right_ops + (size_t)j * d * d);
     namespace K = tenes::itps::core;
     const tn::Axes rot{3, 0, 1, 2, 4};
+    // iTPO: the StartCorrelation/Transfer/FinishCorrelation_density_CTM of
+    // contract_density_ctm/correlation.cpp:34-95 as the same functions on
+    // the single-layer network (the operator traced into the site tensor)
+    const bool tpo = is_tpo(env);
+    const tn::Tensor one = identity_op(env->ctx, 1);
+    auto site = [&](int q, const tn::Tensor &o) {
+      const tn::Tensor t = tpo ? tpo_ket(a[8][q], o) : a[8][q];
+      return direction == 0 ? t : tn::transpose(t, rot);
+    };
+    auto opx = [&](const tn::Tensor &o) -> const tn::Tensor & {
+      return tpo ? one : o;
+    };
+    tn::SingleLayer single(c, tpo);
     const int s = left_site;
     tn::Tensor A, An;
     // the column of a vertical correlation is handled as a row of the
     // rotated site tensors (correlation_function.cpp:146-216)
     if (direction == 0) {
-      K::StartCorrelation_iTPS_CTM(A, a[0][s], a[3][s], a[4][s], a[6][s], a[7][s], a[8][s], L);
-      K::StartCorrelation_iTPS_CTM(An, a[0][s], a[3][s], a[4][s], a[6][s], a[7][s], a[8][s], I);
+      K::StartCorrelation_iTPS_CTM(A, a[0][s], a[3][s], a[4][s], a[6][s], a[7][s], site(s, L), opx(L));
+      K::StartCorrelation_iTPS_CTM(An, a[0][s], a[3][s], a[4][s], a[6][s], a[7][s], site(s, I), opx(I));
     } else {
-      const tn::Tensor t = tn::transpose(a[8][s], rot);
-      K::StartCorrelation_iTPS_CTM(A, a[3][s], a[2][s], a[7][s], a[5][s], a[6][s], t, L);
-      K::StartCorrelation_iTPS_CTM(An, a[3][s], a[2][s], a[7][s], a[5][s], a[6][s], t, I);
+      K::StartCorrelation_iTPS_CTM(A, a[3][s], a[2][s], a[7][s], a[5][s], a[6][s], site(s, L), opx(L));
+      K::StartCorrelation_iTPS_CTM(An, a[3][s], a[2][s], a[7][s], a[5][s], a[6][s], site(s, I), opx(I));
     }
     int r_site = s;
     for (int r = 0; r < r_max; ++r) {
       if (direction == 0) {
         r_site = env->lat.right(r_site);
         const int q = r_site;
+        const tn::Tensor t = site(q, I);
         const double norm = K::FinishCorrelation_iTPS_CTM(
-            An, a[1][q], a[2][q], a[4][q], a[5][q], a[6][q], a[8][q], I);
+            An, a[1][q], a[2][q], a[4][q], a[5][q], a[6][q], t, opx(I));
         for (int j = 0; j < nright; ++j)
           out[(size_t)r * nright + j] = K::FinishCorrelation_iTPS_CTM(
-              A, a[1][q], a[2][q], a[4][q], a[5][q], a[6][q], a[8][q], R[j]) / norm;
-        K::Transfer_iTPS_CTM(A, a[4][q], a[6][q], a[8][q]);
-        K::Transfer_iTPS_CTM(An, a[4][q], a[6][q], a[8][q]);
+              A, a[1][q], a[2][q], a[4][q], a[5][q], a[6][q], site(q, R[j]), opx(R[j])) / norm;
+        K::Transfer_iTPS_CTM(A, a[4][q], a[6][q], t);
+        K::Transfer_iTPS_CTM(An, a[4][q], a[6][q], t);
       } else {
         r_site = env->lat.top(r_site);
         const int q = r_site;
-        const tn::Tensor t = tn::transpose(a[8][q], rot);
+        const tn::Tensor t = site(q, I);
         const double norm = K::FinishCorrelation_iTPS_CTM(
-            An, a[0][q], a[1][q], a[7][q], a[4][q], a[5][q], t, I);
+            An, a[0][q], a[1][q], a[7][q], a[4][q], a[5][q], t, opx(I));
         for (int j = 0; j < nright; ++j)
           out[(size_t)r * nright + j] = K::FinishCorrelation_iTPS_CTM(
-              A, a[0][q], a[1][q], a[7][q], a[4][q], a[5][q], t, R[j]) / norm;
+              A, a[0][q], a[1][q], a[7][q], a[4][q], a[5][q], site(q, R[j]), opx(R[j])) / norm;
         K::Transfer_iTPS_CTM(A, a[7][q], a[5][q], t);
         K::Transfer_iTPS_CTM(An, a[7][q], a[5][q], t);
       }
@@ -850,42 +1009,15 @@ int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
     TN_REQUIRE(out != nullptr, "null output");
     auto &a = env->arr;
     Cluster c = cluster_of(env->lat, source, dx, dy);
-    std::vector<const tn::Tensor *> C(4), eTt(c.ncol), eTr(c.nrow),
-        eTb(c.ncol), eTl(c.nrow);
-    std::vector<std::vector<const tn::Tensor *>> Tn(
-        c.nrow, std::vector<const tn::Tensor *>(c.ncol)),
-        op(c.nrow, std::vector<const tn::Tensor *>(c.ncol));
-    std::vector<tn::Tensor> keep;
-    keep.reserve(c.nrow * c.ncol + 2);
-    for (int row = 0; row < c.nrow; ++row) {
-      for (int col = 0; col < c.ncol; ++col) {
-        const int idx = c.idx[row][col];
-        Tn[row][col] = &a[8][idx];
-        keep.push_back(identity_op(env->ctx, a[8][idx].shape[4]));
-        op[row][col] = &keep.back();
-      }
-      eTl[row] = &a[7][c.idx[row][0]];
-      eTr[row] = &a[5][c.idx[row][c.ncol - 1]];
-    }
-    for (int col = 0; col < c.ncol; ++col) {
-      eTt[col] = &a[4][c.idx[0][col]];
-      eTb[col] = &a[6][c.idx[c.nrow - 1][col]];
-    }
-    C[0] = &a[0][c.idx[0][0]];
-    C[1] = &a[1][c.idx[0][c.ncol - 1]];
-    C[2] = &a[2][c.idx[c.nrow - 1][c.ncol - 1]];
-    C[3] = &a[3][c.idx[c.nrow - 1][0]];
-    if (op_source) {
-      const int s = c.idx[c.source_row][c.source_col];
-      keep.push_back(host_op(env->ctx, op_source, a[8][s].shape[4]));
-      op[c.source_row][c.source_col] = &keep.back();
-    }
-    if (op_target) {
-      const int t = c.idx[c.target_row][c.target_col];
-      keep.push_back(host_op(env->ctx, op_target, a[8][t].shape[4]));
-      op[c.target_row][c.target_col] = &keep.back();
-    }
-    *out = tenes::itps::core::Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn, op);
+    tn::Tensor os, ot;
+    if (op_source)
+      os = host_op(env->ctx, op_source,
+                   a[8][c.idx[c.source_row][c.source_col]].shape[4]);
+    if (op_target)
+      ot = host_op(env->ctx, op_target,
+                   a[8][c.idx[c.target_row][c.target_col]].shape[4]);
+    *out = bond_cluster_value(env, c, op_source ? &os : nullptr,
+                              op_target ? &ot : nullptr);
   });
 }
 
@@ -1045,9 +1177,50 @@ int tenes_env_simple_update(tenes_env *env, int nops, const int *source_sites,
       TN_REQUIRE(source_legs[k] >= 0 && source_legs[k] < 4, "bad source leg");
       up.push_back({source_sites[k], source_legs[k], T(ops[k])});
     }
-    tenes::itps::core::Simple_update_steps(env->arr[TENES_TN], env->lambda,
-                                           env->lat, up, nsteps,
-                                           inverse_lambda_cut);
+    if (!is_tpo(env)) {
+      tenes::itps::core::Simple_update_steps(env->arr[TENES_TN], env->lambda,
+                                             env->lat, up, nsteps,
+                                             inverse_lambda_cut);
+      return;
+    }
+    // iTPO (simple_update.cpp:141-178): the two physical legs merged
+    // (ket index the slower) and the operator applied as kron(U, conj(U))
+    auto &Tn = env->arr[TENES_TN];
+    std::vector<tn::Tensor> M(Tn.size());
+    for (size_t i = 0; i < Tn.size(); ++i) {
+      const auto &t = Tn[i];
+      M[i] = tn::reshape(t, tn::Shape{t.shape[0], t.shape[1], t.shape[2],
+                                      t.shape[3], t.shape[4] * t.shape[5]});
+    }
+    for (auto &u : up) {
+      TN_REQUIRE(u.op.rank() == 4, "iTPO evolution operators must be two-site");
+      std::vector<double> h(u.op.size());
+      tn::to_host(u.op, h.data());
+      const auto &sh = u.op.shape;
+      tn::Shape ks{sh[0] * sh[0], sh[1] * sh[1], sh[2] * sh[2], sh[3] * sh[3]};
+      std::vector<double> k((size_t)(ks[0] * ks[1] * ks[2] * ks[3]));
+      for (int64_t i0 = 0; i0 < sh[0] * sh[0]; ++i0)
+        for (int64_t i1 = 0; i1 < sh[1] * sh[1]; ++i1)
+          for (int64_t i2 = 0; i2 < sh[2] * sh[2]; ++i2)
+            for (int64_t i3 = 0; i3 < sh[3] * sh[3]; ++i3) {
+              const int64_t a0 = i0 / sh[0], b0 = i0 % sh[0];
+              const int64_t a1 = i1 / sh[1], b1 = i1 % sh[1];
+              const int64_t a2 = i2 / sh[2], b2 = i2 % sh[2];
+              const int64_t a3 = i3 / sh[3], b3 = i3 % sh[3];
+              const double ua = h[((a0 * sh[1] + a1) * sh[2] + a2) * sh[3] + a3];
+              const double ub = h[((b0 * sh[1] + b1) * sh[2] + b2) * sh[3] + b3];
+              k[((i0 * ks[1] + i1) * ks[2] + i2) * ks[3] + i3] = ua * ub;
+            }
+      u.op = tn::from_host(*env->ctx->c, ks, k.data());
+    }
+    tenes::itps::core::Simple_update_steps(M, env->lambda, env->lat, up,
+                                           nsteps, inverse_lambda_cut);
+    for (size_t i = 0; i < Tn.size(); ++i) {
+      const auto &t = M[i];
+      const int64_t d = Tn[i].shape[4];
+      Tn[i] = tn::reshape(t, tn::Shape{t.shape[0], t.shape[1], t.shape[2],
+                                       t.shape[3], d, d});
+    }
   });
 }
 

@@ -11,6 +11,7 @@ namespace itps {
 namespace core {
 
 using tn::Axes;
+using tn::bra;
 using tn::conj;
 using tn::tensordot;
 using tn::trace;
@@ -27,7 +28,7 @@ double Contract_one_site_iTPS_CTM(const tensor &C1, const tensor &C2,
                 tensordot(C1, eT1, Axes{1}, Axes{0}), Axes{0}, Axes{1});
   tensor lower = tensordot(
       tensordot(tensordot(C4, eT4, Axes{1}, Axes{0}), eT3, Axes{0}, Axes{1}),
-      conj(Tn1), Axes{2, 5}, Axes{0, 3});
+      bra(Tn1), Axes{2, 5}, Axes{0, 3});
   tensor X = tensordot(tensordot(upper, lower, Axes{0, 2, 3, 5},
                                  Axes{2, 5, 0, 4}),
                        Tn1, Axes{0, 1, 2, 3}, Axes{2, 1, 0, 3});
@@ -44,7 +45,7 @@ double Contract_two_sites_horizontal_iTPS_CTM(
       Tn2,
       tensordot(tensordot(tensordot(C3, eT4, Axes{1}, Axes{0}), eT3, Axes{0},
                           Axes{1}),
-                tensordot(conj(Tn2), op2, Axes{4}, Axes{1}), Axes{2, 5},
+                tensordot(bra(Tn2), op2, Axes{4}, Axes{1}), Axes{2, 5},
                 Axes{3, 2}),
       Axes{2, 3, 4}, Axes{3, 1, 6});
   tensor R = tensordot(
@@ -55,7 +56,7 @@ double Contract_two_sites_horizontal_iTPS_CTM(
                        tensordot(C4, eT6, Axes{1}, Axes{0}), Axes{0}, Axes{1});
   tensor X = tensordot(
       Tn1,
-      tensordot(L, tensordot(conj(Tn1), R, Axes{2, 3}, Axes{2, 5}),
+      tensordot(L, tensordot(bra(Tn1), R, Axes{2, 3}, Axes{2, 5}),
                 Axes{0, 2, 3, 5}, Axes{3, 1, 5, 0}),
       Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
   return trace(X, op1, Axes{0, 1}, Axes{0, 1});
@@ -71,7 +72,7 @@ double Contract_two_sites_vertical_iTPS_CTM(
       Tn2,
       tensordot(tensordot(tensordot(C4, eT5, Axes{1}, Axes{0}), eT4, Axes{0},
                           Axes{1}),
-                tensordot(conj(Tn2), op2, Axes{4}, Axes{1}), Axes{2, 5},
+                tensordot(bra(Tn2), op2, Axes{4}, Axes{1}), Axes{2, 5},
                 Axes{0, 3}),
       Axes{0, 3, 4}, Axes{1, 3, 6});
   tensor B = tensordot(
@@ -82,7 +83,7 @@ double Contract_two_sites_vertical_iTPS_CTM(
                        tensordot(C1, eT1, Axes{1}, Axes{0}), Axes{0}, Axes{1});
   tensor X = tensordot(
       Tn1,
-      tensordot(T, tensordot(conj(Tn1), B, Axes{0, 3}, Axes{5, 2}),
+      tensordot(T, tensordot(bra(Tn1), B, Axes{0, 3}, Axes{5, 2}),
                 Axes{0, 2, 3, 5}, Axes{3, 1, 5, 0}),
       Axes{0, 1, 2, 3}, Axes{4, 1, 0, 3});
   return trace(X, op1, Axes{0, 1}, Axes{0, 1});
@@ -97,7 +98,7 @@ double Contract_two_sites_horizontal_op12_iTPS_CTM(
   tensor left = tensordot(
       eT1,
       tensordot(Tn1,
-                tensordot(conj(Tn1),
+                tensordot(bra(Tn1),
                           tensordot(eT5,
                                     tensordot(C1,
                                               tensordot(C4, eT6, Axes{1},
@@ -110,7 +111,7 @@ double Contract_two_sites_horizontal_op12_iTPS_CTM(
   tensor right = tensordot(
       eT2,
       tensordot(Tn2,
-                tensordot(conj(Tn2),
+                tensordot(bra(Tn2),
                           tensordot(eT4,
                                     tensordot(C2,
                                               tensordot(C3, eT3, Axes{0},
@@ -133,7 +134,7 @@ double Contract_two_sites_vertical_op12_iTPS_CTM(
   tensor top = tensordot(
       eT2,
       tensordot(Tn1,
-                tensordot(conj(Tn1),
+                tensordot(bra(Tn1),
                           tensordot(eT6,
                                     tensordot(C1,
                                               tensordot(C2, eT1, Axes{0},
@@ -146,7 +147,7 @@ double Contract_two_sites_vertical_op12_iTPS_CTM(
   tensor bottom = tensordot(
       eT3,
       tensordot(Tn2,
-                tensordot(conj(Tn2),
+                tensordot(bra(Tn2),
                           tensordot(eT5,
                                     tensordot(C3,
                                               tensordot(C4, eT4, Axes{0},
@@ -180,7 +181,7 @@ double contract_1x1(const std::vector<const tensor *> &C,
       Axes{1}, Axes{0});
   tensor X = tensordot(
       T,
-      tensordot(top, tensordot(conj(T), rest, Axes{2, 3}, Axes{2, 5}),
+      tensordot(top, tensordot(bra(T), rest, Axes{2, 3}, Axes{2, 5}),
                 Axes{0, 2, 3, 5}, Axes{3, 1, 5, 0}),
       Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
   return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
@@ -204,7 +205,7 @@ double contract_1x2(const std::vector<const tensor *> &C,
   tensor site1 = tensordot(
       *eTt[1],
       tensordot(T1,
-                tensordot(tensordot(conj(T1), *op[0][1], Axes{4}, Axes{1}),
+                tensordot(tensordot(bra(T1), *op[0][1], Axes{4}, Axes{1}),
                           right_env, Axes{2, 3}, Axes{5, 2}),
                 Axes{2, 3, 4}, Axes{6, 4, 2}),
       Axes{1, 2, 3}, Axes{5, 1, 3});
@@ -214,7 +215,7 @@ double contract_1x2(const std::vector<const tensor *> &C,
                          Axes{0}, Axes{0});
   tensor X = tensordot(
       T0,
-      tensordot(top, tensordot(conj(T0), lower, Axes{2, 3}, Axes{5, 2}),
+      tensordot(top, tensordot(bra(T0), lower, Axes{2, 3}, Axes{5, 2}),
                 Axes{0, 2, 3, 5}, Axes{5, 1, 3, 0}),
       Axes{0, 1, 2, 3}, Axes{1, 0, 4, 3});
   return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
@@ -238,7 +239,7 @@ double contract_2x1(const std::vector<const tensor *> &C,
   tensor site1 = tensordot(
       *eTr[1],
       tensordot(T1,
-                tensordot(tensordot(conj(T1), *op[1][0], Axes{4}, Axes{1}),
+                tensordot(tensordot(bra(T1), *op[1][0], Axes{4}, Axes{1}),
                           bottom_env, Axes{0, 3}, Axes{2, 5}),
                 Axes{0, 3, 4}, Axes{4, 6, 2}),
       Axes{1, 2, 3}, Axes{5, 1, 3});
@@ -248,7 +249,7 @@ double contract_2x1(const std::vector<const tensor *> &C,
                          Axes{0}, Axes{0});
   tensor X = tensordot(
       T0,
-      tensordot(top, tensordot(conj(T0), right, Axes{2, 3}, Axes{2, 4}),
+      tensordot(top, tensordot(bra(T0), right, Axes{2, 3}, Axes{2, 4}),
                 Axes{0, 2, 3, 5}, Axes{3, 1, 6, 0}),
       Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
   return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
@@ -266,21 +267,21 @@ double contract_2x2(const std::vector<const tensor *> &C,
                &T11 = *Tn[1][1];
   const tensor tr = tensordot(
       T01,
-      tensordot(tensordot(conj(T01), *op[0][1], Axes{4}, Axes{1}),
+      tensordot(tensordot(bra(T01), *op[0][1], Axes{4}, Axes{1}),
                 tensordot(*eTt[1], tensordot(*C[1], *eTr[0], Axes{1}, Axes{0}),
                           Axes{1}, Axes{0}),
                 Axes{1, 2}, Axes{2, 5}),
       Axes{1, 2, 4}, Axes{4, 6, 2});
   const tensor bl = tensordot(
       T10,
-      tensordot(tensordot(conj(T10), *op[1][0], Axes{4}, Axes{1}),
+      tensordot(tensordot(bra(T10), *op[1][0], Axes{4}, Axes{1}),
                 tensordot(*eTb[0], tensordot(*C[3], *eTl[1], Axes{1}, Axes{0}),
                           Axes{1}, Axes{0}),
                 Axes{0, 3}, Axes{5, 2}),
       Axes{0, 3, 4}, Axes{6, 4, 2});
   const tensor br = tensordot(
       T11,
-      tensordot(tensordot(conj(T11), *op[1][1], Axes{4}, Axes{1}),
+      tensordot(tensordot(bra(T11), *op[1][1], Axes{4}, Axes{1}),
                 tensordot(*eTr[1], tensordot(*C[2], *eTb[1], Axes{1}, Axes{0}),
                           Axes{1}, Axes{0}),
                 Axes{2, 3}, Axes{2, 5}),
@@ -291,7 +292,7 @@ double contract_2x2(const std::vector<const tensor *> &C,
                                Axes{0}, Axes{0});
   const tensor X = tensordot(
       T00,
-      tensordot(top, tensordot(conj(T00), rest, Axes{2, 3}, Axes{1, 4}),
+      tensordot(top, tensordot(bra(T00), rest, Axes{2, 3}, Axes{1, 4}),
                 Axes{0, 2, 3, 5}, Axes{4, 1, 6, 0}),
       Axes{0, 1, 2, 3}, Axes{1, 0, 3, 4});
   return trace(*op[0][0], X, Axes{0, 1}, Axes{0, 1});
@@ -391,10 +392,10 @@ double contract_cluster_sweep(const std::vector<const tensor *> &C,
     return Labeled{*Tn[r][c],
                    {hk(r, c), vk(r, c), hk(r, c + 1), vk(r + 1, c), ph(r, c)}};
   };
-  auto bra = [&](int r, int c) {
+  auto bra_of = [&](int r, int c) {
     TN_REQUIRE((int)op[r].size() == nc && op[r][c] && Tn[r][c],
                "cluster sweep: missing site tensor or operator");
-    return Labeled{tensordot(conj(*Tn[r][c]), *op[r][c], Axes{4}, Axes{1}),
+    return Labeled{tensordot(bra(*Tn[r][c]), *op[r][c], Axes{4}, Axes{1}),
                    {hb(r, c), vb(r, c), hb(r, c + 1), vb(r + 1, c), ph(r, c)}};
   };
 
@@ -406,7 +407,7 @@ double contract_cluster_sweep(const std::vector<const tensor *> &C,
       B = dot(B, et(c));
       for (int r = 0; r < nr; ++r) {
         B = dot(B, ket(r, c));
-        B = dot(B, bra(r, c));
+        B = dot(B, bra_of(r, c));
       }
       B = dot(B, eb(c));
     }
@@ -422,7 +423,7 @@ double contract_cluster_sweep(const std::vector<const tensor *> &C,
     B = dot(B, el(r));
     for (int c = 0; c < nc; ++c) {
       B = dot(B, ket(r, c));
-      B = dot(B, bra(r, c));
+      B = dot(B, bra_of(r, c));
     }
     B = dot(B, er(r));
   }
@@ -482,7 +483,7 @@ void StartCorrelation_iTPS_CTM(tensor &A, const tensor &C1, const tensor &C4,
                                const tensor &eT4, const tensor &Tn1,
                                const tensor &op) {
   const tensor inner = tensordot(
-      tensordot(conj(Tn1), op, Axes{4}, Axes{1}),
+      tensordot(bra(Tn1), op, Axes{4}, Axes{1}),
       tensordot(eT3, tensordot(C1, tensordot(C4, eT4, Axes{1}, Axes{0}),
                                Axes{0}, Axes{1}),
                 Axes{1}, Axes{1}),
@@ -494,7 +495,7 @@ void StartCorrelation_iTPS_CTM(tensor &A, const tensor &C1, const tensor &C4,
 // contract_itps_ctm/correlation.cpp:60-78
 void Transfer_iTPS_CTM(tensor &A, const tensor &eT1, const tensor &eT3,
                        const tensor &Tn1) {
-  const tensor inner = tensordot(conj(Tn1), tensordot(A, eT3, Axes{1}, Axes{1}),
+  const tensor inner = tensordot(bra(Tn1), tensordot(A, eT3, Axes{1}, Axes{1}),
                                  Axes{0, 3}, Axes{2, 5});
   A = tn::tensordot_t(eT1, tensordot(Tn1, inner, Axes{0, 3, 4}, Axes{4, 6, 2}),
                       Axes{0, 2, 3}, Axes{4, 0, 2}, Axes{0, 3, 1, 2});
@@ -506,7 +507,7 @@ double FinishCorrelation_iTPS_CTM(const tensor &A, const tensor &C2,
                                   const tensor &eT2, const tensor &eT3,
                                   const tensor &Tn1, const tensor &op) {
   const tensor right = tensordot(
-      conj(Tn1),
+      bra(Tn1),
       tensordot(eT3, tensordot(C2, tensordot(C3, eT2, Axes{0}, Axes{1}),
                                Axes{1}, Axes{1}),
                 Axes{0}, Axes{1}),

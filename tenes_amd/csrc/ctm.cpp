@@ -145,7 +145,7 @@ tensor enlarged_LT(const tensor &C1, const tensor &eT1, const tensor &eT8,
       Tn1,
       tn::tensordot(tn::tensordot(tn::tensordot(C1, eT1, Axes{1}, Axes{0}),
                                   eT8, Axes{0}, Axes{1}),
-                    tn::conj(Tn1), Axes{2, 5}, Axes{1, 0}),
+                    tn::bra(Tn1), Axes{2, 5}, Axes{1, 0}),
       Axes{0, 1, 4}, Axes{3, 1, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
 // ctm.cpp:222-227: LB = Tn4*(((C4*eT7)*eT6)*Tn4c), reference legs (t41,
@@ -156,7 +156,7 @@ tensor enlarged_LB(const tensor &C4, const tensor &eT7, const tensor &eT6,
       Tn4,
       tn::tensordot(tn::tensordot(tn::tensordot(C4, eT7, Axes{1}, Axes{0}),
                                   eT6, Axes{0}, Axes{1}),
-                    tn::conj(Tn4), Axes{2, 5}, Axes{0, 3}),
+                    tn::bra(Tn4), Axes{2, 5}, Axes{0, 3}),
       Axes{0, 3, 4}, Axes{1, 3, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
 
@@ -169,7 +169,7 @@ struct ProjectorJob {
   tensor PU, PL;
   tensor Ablk, Bblk;
   Axes axA, axB;
-  int64_t r0 = 0, r1 = 0, c0 = 0, c1 = 0;
+  int64_t r0 = 0, r1 = 0, rb = 0, c0 = 0, c1 = 0, cb = 0;  // (t, e, tc) legs
   int kmax = 0;
   tn::SvdPending sv;
   // warm-start entry to update with the Ritz-value range once known
@@ -187,11 +187,12 @@ constexpr int kSyncSlot = 61;  // pinned slot of the synchronous entry points
 using ProjKey = std::array<int64_t, 4>;
 
 void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
-                      int64_t r1, int64_t c0, int64_t c1, int chi_cap,
-                      const tensor &Ablk, const Axes &axA, const tensor &Bblk,
-                      const Axes &axB, const PEPS_Parameters &pp, bool rsvd,
-                      int slot, const ProjKey *key, ProjectorJob &job) {
-  const int64_t nr = r0 * r1 * r0, nc = c0 * c1 * c0;
+                      int64_t r1, int64_t rb, int64_t c0, int64_t c1,
+                      int64_t cb, int chi_cap, const tensor &Ablk,
+                      const Axes &axA, const tensor &Bblk, const Axes &axB,
+                      const PEPS_Parameters &pp, bool rsvd, int slot,
+                      const ProjKey *key, ProjectorJob &job) {
+  const int64_t nr = r0 * r1 * rb, nc = c0 * c1 * cb;
   job.kmax = (int)std::min<int64_t>(std::min<int64_t>(nr, nc), chi_cap);
   tn::SvdOptions opts = svd_opts(pp, rsvd);
   const bool warm = key != nullptr && pp.svd_warm_start && !rsvd;
@@ -219,8 +220,10 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
   job.axB = axB;
   job.r0 = r0;
   job.r1 = r1;
+  job.rb = rb;
   job.c0 = c0;
   job.c1 = c1;
+  job.cb = cb;
 }
 
 void projectors_finish(tn::Context &ctx, ProjectorJob &job,
@@ -244,8 +247,8 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
   tensor Vc = tn::slice(sv.V, 1, 0, cut);
   tn::multiply_vector_dev(Uc, sc.data(), 1);
   tn::multiply_vector_dev(Vc, sc.data(), 1);
-  Uc = tn::reshape(Uc, Shape{job.r0, job.r1, job.r0, cut});
-  Vc = tn::reshape(Vc, Shape{job.c0, job.c1, job.c0, cut});
+  Uc = tn::reshape(Uc, Shape{job.r0, job.r1, job.rb, cut});
+  Vc = tn::reshape(Vc, Shape{job.c0, job.c1, job.cb, cut});
   job.PU = tn::tensordot_t(job.Bblk, Vc, job.axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
   job.PL = tn::tensordot_t(job.Ablk, Uc, job.axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
   job.Ablk = tensor();
@@ -275,12 +278,14 @@ void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
   tensor LT = enlarged_LT(C1, eT1, eT8, Tn1);
   tensor LB = enlarged_LB(C4, eT7, eT6, Tn4);
   const bool rsvd = (t12 != 1 && t34 != 1) && pp.Use_RSVD;
+  // bra bond dimensions (equal to the ket's, or 1 in single-layer mode)
+  const int64_t b12 = LT.shape[2], b41 = LT.shape[5], b34 = LB.shape[5];
   int cap = pp.CHI;
   if (rsvd)
     cap = (int)std::min<int64_t>(
-        std::min<int64_t>(std::min<int64_t>(pp.CHI, e78 * t41 * t41),
-                          e12 * t12 * t12),
-        e56 * t34 * t34);
+        std::min<int64_t>(std::min<int64_t>(pp.CHI, e78 * t41 * b41),
+                          e12 * t12 * b12),
+        e56 * t34 * b34);
   // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273); in
   // mixed mode 1 the matrix the SVD truncates stays FP64
   tensor R;
@@ -288,7 +293,8 @@ void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
     tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
     R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
   }
-  projectors_begin(ctx, R, t12, e12, t34, e56, cap, LT, Axes{0, 1, 2}, LB,
+  projectors_begin(ctx, R, t12, e12, b12, t34, e56, b34, cap, LT,
+                   Axes{0, 1, 2}, LB,
                    Axes{3, 4, 5}, pp, rsvd, slot, key, job);
 }
 
@@ -316,16 +322,18 @@ void updown_blocks_begin(
       Tn2,
       tn::tensordot(tn::tensordot(tn::tensordot(C2, eT3, Axes{1}, Axes{0}),
                                   eT2, Axes{0}, Axes{1}),
-                    tn::conj(Tn2), Axes{2, 5}, Axes{2, 1}),
+                    tn::bra(Tn2), Axes{2, 5}, Axes{2, 1}),
       Axes{1, 2, 4}, Axes{3, 1, 6});
   // RB legs (t34, t23, e56, e34, tc34, tc23)  (ctm.cpp:378-383)
   tensor RB = tn::tensordot(
       Tn3,
       tn::tensordot(tn::tensordot(tn::tensordot(C3, eT5, Axes{1}, Axes{0}),
                                   eT4, Axes{0}, Axes{1}),
-                    tn::conj(Tn3), Axes{2, 5}, Axes{3, 2}),
+                    tn::bra(Tn3), Axes{2, 5}, Axes{3, 2}),
       Axes{2, 3, 4}, Axes{3, 1, 6});
   tensor LB = enlarged_LB(C4, eT7, eT6, Tn4);
+  TN_REQUIRE(!ctx.single_layer(),
+             "up-down projectors of the single-layer network are not supported");
   const bool rsvd = t23 != 1 && pp.Use_RSVD;
   int cap = pp.CHI;
   if (rsvd) cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * t23);
@@ -337,8 +345,8 @@ void updown_blocks_begin(
     tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
     M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
   }
-  projectors_begin(ctx, M, t23, e34, t23, e34, cap, R1, Axes{0, 1, 2}, R2,
-                   Axes{0, 1, 2}, pp, rsvd, slot, key, job);
+  projectors_begin(ctx, M, t23, e34, t23, t23, e34, t23, cap, R1,
+                   Axes{0, 1, 2}, R2, Axes{0, 1, 2}, pp, rsvd, slot, key, job);
 }
 
 // The L projectors of a move are independent chains of small kernels, each
@@ -429,7 +437,7 @@ void Calc_Next_CTM(const tensor &C1, const tensor &C4, const tensor &eT1,
 void Calc_Next_eT(const tensor &eT8, const tensor &Tn1, const tensor &PU,
                   const tensor &PL, tensor &eT_out) {
   tensor X = tn::tensordot(eT8, PL, Axes{1}, Axes{0});
-  tensor Y = tn::tensordot(tn::conj(Tn1), X, Axes{0, 1}, Axes{2, 4});
+  tensor Y = tn::tensordot(tn::bra(Tn1), X, Axes{0, 1}, Axes{2, 4});
   tensor Z = tn::tensordot(Tn1, Y, Axes{0, 1, 4}, Axes{4, 5, 2});
   eT_out = tn::tensordot_t(Z, PU, Axes{1, 3, 4}, Axes{1, 2, 0},
                            Axes{3, 2, 0, 1});
@@ -736,6 +744,19 @@ tensor fit_chi(const tensor &T, int64_t chi) {
 }  // namespace
 
 // ctm.cpp:984-1193
+// The bra copy of T in the initial corner and edge contractions: conj(T),
+// or in single-layer mode a tensor of ones over the contracted legs (unit
+// over the others), which sums T over those legs (the uniform vectors of
+// ctm_single.cpp:751-995).
+tensor init_bra(const tensor &T, const Axes &contracted) {
+  if (!T.ctx()->single_layer()) return tn::conj(T);
+  Shape s(5, 1);
+  for (int k : contracted) s[k] = T.shape[k];
+  tensor B = tn::empty(*T.ctx(), s);
+  tn::kern::fill(B.data(), B.size(), 1.0, T.ctx()->stream());
+  return B;
+}
+
 void Initialize_CTM(std::vector<tensor> &C1, std::vector<tensor> &C2,
                     std::vector<tensor> &C3, std::vector<tensor> &C4,
                     std::vector<tensor> &eTt, std::vector<tensor> &eTr,
@@ -744,6 +765,10 @@ void Initialize_CTM(std::vector<tensor> &C1, std::vector<tensor> &C2,
                     const SquareLattice &lat) {
   const int N = lat.N_UNIT;
   const int64_t chi = pp.CHI;
+  // the single-layer initial environment keeps the site tensors' bond
+  // dimensions (ctm_single.cpp:751-995 has no extend/slice to chi)
+  const bool single = Tn[0].ctx()->single_layer();
+  auto fit = [&](const tensor &x) { return single ? x : fit_chi(x, chi); };
   C1.resize(N);
   C2.resize(N);
   C3.resize(N);
@@ -756,78 +781,78 @@ void Initialize_CTM(std::vector<tensor> &C1, std::vector<tensor> &C2,
     int num = lat.top(lat.left(i));
     {
       const auto &T = Tn[num];
-      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{0, 1, 4}, Axes{0, 1, 4},
+      const tensor B = init_bra(T, Axes{0, 1, 4});
+      tensor c = tn::tensordot_t(T, B, Axes{0, 1, 4}, Axes{0, 1, 4},
                                  Axes{1, 3, 0, 2});
-      C1[i] = fit_chi(tn::reshape(c, Shape{T.shape[3] * T.shape[3],
-                                           T.shape[2] * T.shape[2]}),
-                      chi);
+      C1[i] = fit(tn::reshape(c, Shape{T.shape[3] * B.shape[3],
+                                       T.shape[2] * B.shape[2]}));
     }
     num = lat.top(lat.right(i));
     {
       const auto &T = Tn[num];
-      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{1, 2, 4}, Axes{1, 2, 4},
+      const tensor B = init_bra(T, Axes{1, 2, 4});
+      tensor c = tn::tensordot_t(T, B, Axes{1, 2, 4}, Axes{1, 2, 4},
                                  Axes{0, 2, 1, 3});
-      C2[i] = fit_chi(tn::reshape(c, Shape{T.shape[0] * T.shape[0],
-                                           T.shape[3] * T.shape[3]}),
-                      chi);
+      C2[i] = fit(tn::reshape(c, Shape{T.shape[0] * B.shape[0],
+                                       T.shape[3] * B.shape[3]}));
     }
     num = lat.bottom(lat.right(i));
     {
       const auto &T = Tn[num];
-      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{2, 3, 4}, Axes{2, 3, 4},
+      const tensor B = init_bra(T, Axes{2, 3, 4});
+      tensor c = tn::tensordot_t(T, B, Axes{2, 3, 4}, Axes{2, 3, 4},
                                  Axes{1, 3, 0, 2});
-      C3[i] = fit_chi(tn::reshape(c, Shape{T.shape[1] * T.shape[1],
-                                           T.shape[0] * T.shape[0]}),
-                      chi);
+      C3[i] = fit(tn::reshape(c, Shape{T.shape[1] * B.shape[1],
+                                       T.shape[0] * B.shape[0]}));
     }
     num = lat.bottom(lat.left(i));
     {
       const auto &T = Tn[num];
-      tensor c = tn::tensordot_t(T, tn::conj(T), Axes{0, 3, 4}, Axes{0, 3, 4},
+      const tensor B = init_bra(T, Axes{0, 3, 4});
+      tensor c = tn::tensordot_t(T, B, Axes{0, 3, 4}, Axes{0, 3, 4},
                                  Axes{1, 3, 0, 2});
-      C4[i] = fit_chi(tn::reshape(c, Shape{T.shape[2] * T.shape[2],
-                                           T.shape[1] * T.shape[1]}),
-                      chi);
+      C4[i] = fit(tn::reshape(c, Shape{T.shape[2] * B.shape[2],
+                                       T.shape[1] * B.shape[1]}));
     }
     num = lat.top(i);
     {
       const auto &T = Tn[num];
-      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{1, 4}, Axes{1, 4},
+      const tensor B = init_bra(T, Axes{1, 4});
+      tensor e = tn::tensordot_t(T, B, Axes{1, 4}, Axes{1, 4},
                                  Axes{0, 3, 1, 4, 2, 5});
-      eTt[i] = fit_chi(tn::reshape(e, Shape{T.shape[0] * T.shape[0],
-                                            T.shape[2] * T.shape[2],
-                                            T.shape[3], T.shape[3]}),
-                       chi);
+      eTt[i] = fit(tn::reshape(e, Shape{T.shape[0] * B.shape[0],
+                                        T.shape[2] * B.shape[2],
+                                        T.shape[3], B.shape[3]}));
     }
     num = lat.right(i);
     {
       const auto &T = Tn[num];
-      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{2, 4}, Axes{2, 4},
+      const tensor B = init_bra(T, Axes{2, 4});
+      tensor e = tn::tensordot_t(T, B, Axes{2, 4}, Axes{2, 4},
                                  Axes{1, 4, 2, 5, 0, 3});
-      eTr[i] = fit_chi(tn::reshape(e, Shape{T.shape[1] * T.shape[1],
-                                            T.shape[3] * T.shape[3],
-                                            T.shape[0], T.shape[0]}),
-                       chi);
+      eTr[i] = fit(tn::reshape(e, Shape{T.shape[1] * B.shape[1],
+                                        T.shape[3] * B.shape[3],
+                                        T.shape[0], B.shape[0]}));
     }
     num = lat.bottom(i);
     {
       const auto &T = Tn[num];
-      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{3, 4}, Axes{3, 4},
+      const tensor B = init_bra(T, Axes{3, 4});
+      tensor e = tn::tensordot_t(T, B, Axes{3, 4}, Axes{3, 4},
                                  Axes{2, 5, 0, 3, 1, 4});
-      eTb[i] = fit_chi(tn::reshape(e, Shape{T.shape[2] * T.shape[2],
-                                            T.shape[0] * T.shape[0],
-                                            T.shape[1], T.shape[1]}),
-                       chi);
+      eTb[i] = fit(tn::reshape(e, Shape{T.shape[2] * B.shape[2],
+                                        T.shape[0] * B.shape[0],
+                                        T.shape[1], B.shape[1]}));
     }
     num = lat.left(i);
     {
       const auto &T = Tn[num];
-      tensor e = tn::tensordot_t(T, tn::conj(T), Axes{0, 4}, Axes{0, 4},
+      const tensor B = init_bra(T, Axes{0, 4});
+      tensor e = tn::tensordot_t(T, B, Axes{0, 4}, Axes{0, 4},
                                  Axes{2, 5, 0, 3, 1, 4});
-      eTl[i] = fit_chi(tn::reshape(e, Shape{T.shape[3] * T.shape[3],
-                                            T.shape[1] * T.shape[1],
-                                            T.shape[2], T.shape[2]}),
-                       chi);
+      eTl[i] = fit(tn::reshape(e, Shape{T.shape[3] * B.shape[3],
+                                        T.shape[1] * B.shape[1],
+                                        T.shape[2], B.shape[2]}));
     }
   }
 }

@@ -645,6 +645,16 @@ Tensor extend(const Tensor &t, const Shape &shape) {
 
 Tensor conj(const Tensor &t) { return t; }
 
+Tensor bra(const Tensor &t) {
+  Context &ctx = *t.ctx();
+  if (!ctx.single_layer()) return conj(t);
+  TN_REQUIRE(t.rank() == 5 && t.shape[4] == 1,
+             "single-layer site tensors must be (l, u, r, d, 1)");
+  Tensor u = make(ctx, Shape{1, 1, 1, 1, 1});
+  kern::fill(u.data(), 1, 1.0, ctx.stream());
+  return u;
+}
+
 void trace_dev(const Tensor &A, const Tensor &B, const Axes &axA,
                const Axes &axB, double *out) {
   TN_REQUIRE(A.rank() == (int)axA.size() && B.rank() == (int)axB.size() &&

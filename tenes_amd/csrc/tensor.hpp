@@ -172,6 +172,13 @@ class Context {
   // HBM, operands rounded to FP32, FP32 accumulation).
   void set_fp32_contractions(bool on) { fp32_ = on; }
   bool fp32_contractions() const { return fp32_; }
+  // Single-layer networks: while on, bra() of a site tensor is the unit
+  // tensor of shape (1,1,1,1,1), so the double-layer contractions of the
+  // CTM moves and of the environment contractions become those of the
+  // single-layer network of the ket tensors alone (TPOs traced over their
+  // physical legs; edges (chi, chi, D, 1)).
+  void set_single_layer(bool on) { single_layer_ = on; }
+  bool single_layer() const { return single_layer_; }
   void profile_begin(hipEvent_t *a);
   void profile_end(hipEvent_t a, const char *name, double flops);
   std::map<std::string, KernelStat> kernel_stats();  // flushes pending
@@ -197,6 +204,7 @@ class Context {
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
   bool fp32_ = false;
+  bool single_layer_ = false;
   bool profiling_ = false;
   struct Pending {
     hipEvent_t a, b;
@@ -220,6 +228,21 @@ class ContractionPrecision {
   ~ContractionPrecision() { ctx_.set_fp32_contractions(old_); }
   ContractionPrecision(const ContractionPrecision &) = delete;
   ContractionPrecision &operator=(const ContractionPrecision &) = delete;
+
+ private:
+  Context &ctx_;
+  bool old_;
+};
+
+// Sets the single-layer mode of a context for a scope.
+class SingleLayer {
+ public:
+  SingleLayer(Context &ctx, bool on) : ctx_(ctx), old_(ctx.single_layer()) {
+    ctx.set_single_layer(on);
+  }
+  ~SingleLayer() { ctx_.set_single_layer(old_); }
+  SingleLayer(const SingleLayer &) = delete;
+  SingleLayer &operator=(const SingleLayer &) = delete;
 
  private:
   Context &ctx_;
@@ -250,6 +273,10 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
 Tensor slice(const Tensor &t, int axis, int64_t begin, int64_t end);
 Tensor extend(const Tensor &t, const Shape &shape);  // zero padding
 Tensor conj(const Tensor &t);                         // real: identity
+// The bra copy of a site tensor (l, u, r, d, p) in a double-layer network:
+// conj(t), or the unit (1,1,1,1,1) tensor in single-layer mode (t must then
+// have a physical dimension of one).
+Tensor bra(const Tensor &t);
 // Full contraction of A and B over all legs (A leg axA[k] with B leg axB[k])
 // to a scalar (mptensor trace(A, B, axA, axB)).  Synchronous.
 double trace(const Tensor &A, const Tensor &B, const Axes &axA,

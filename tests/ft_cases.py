@@ -84,21 +84,29 @@ def differences(case, records):
 
 
 # Largest |ours - TeNeS| accepted, by case, beta and table.  FT_Kitaev: the
-# oracle lands on TeNeS to rounding (1e-14) except for the correlation
-# lengths along y, whose subleading transfer-matrix eigenvalues are 1e-5 of
-# the leading one and so carry the environment's last digits (2.6e-5
-# relative in |t1/t0| at worst).  FT_TFI_square: at beta = 1 to 2.4e-10; at
-# beta = 2 the CTM stops at its 10-iteration limit unconverged and at
-# beta >= 3 the chi = 10 cut falls inside nearly degenerate multiplets of
-# the projector spectrum, so two FP64 runs part at the 1e-7 level (TeNeS's
-# own test accepts rtol 1e-3, atol 1e-4, test/fulltest.py.in:141-142).
-def tolerance(name, beta, table):
+# oracle and the GPU land on TeNeS to rounding (1e-14) except for the
+# correlation lengths along y, whose subleading transfer-matrix eigenvalues
+# are 1e-5 of the leading one and so carry the environment's last digits
+# (2.6e-5 relative in |t1/t0| at worst).  FT_TFI_square: at beta = 1 to
+# 2.4e-10.  Between beta = 1.4 and 1.5 the 3rd and 4th singular values of
+# the simple update's bond matrix cross (the kept bond spectrum jumps from
+# (0.99, 0.46, 0.15) to (0.99, 0.22, 0.12)), so FP64 implementations with
+# different SVD rounding leave the crossing apart (TeNeS, the oracle and the
+# GPU by 1e-6 to 2e-5 in the bond spectrum at beta = 2), and at beta = 2 the
+# CTM also stops at its 10-iteration limit unconverged; the differences then
+# decay (1e-6 at beta = 3 to 5).  TeNeS's own test accepts rtol 1e-3, atol
+# 1e-4 (test/fulltest.py.in:141-142).
+def tolerance(name, beta, table, gpu=False):
     if name == "FT_Kitaev":
         return 2e-4 if table == "correlation_length" else 1e-12
     if beta <= 1.0:
         return 1e-8 if table == "correlation_length" else 1e-9
     if beta <= 2.0:
+        if gpu:
+            return 3e-3 if table == "correlation_length" else 3e-4
         return 5e-4 if table == "correlation_length" else 3e-5
+    if gpu:
+        return 5e-5 if table == "correlation_length" else 1e-5
     return 1e-5 if table == "correlation_length" else 1e-6
 
 
@@ -118,3 +126,24 @@ def oracle_records(case):
     Tn = FT.initial_tensors_density(lat, case["pdims"], case["vdims"])
     return FT.finite_temperature(Tn, lat, case["evolution"], case["nsteps"], case["tau"], case["interval"],
                                  measure)
+
+
+def gpu_records(ctx, case):
+    """The same pipeline through the GPU library (tenes_amd.finite_temperature)."""
+    import tenes_amd as ta
+    from tenes_amd import finite_temperature as FTG
+
+    fx = case["fx"]
+    LX, LY = fx["tensor"]["L_sub"]
+    skew = fx["tensor"].get("skew", 0)
+    env = ta.Environment(ctx, LX, LY, skew)
+    params = ta.peps_parameters(CHI=case["chi"], Max_CTM_Iteration=case["ctm_iter"])
+
+    def measure(e, beta):
+        r = FTG.measure(e, LX, LY, skew, case["onesite"], case["twosite"], case["site_ops"], case["r_max"],
+                        case["pairs"], params, beta)
+        r["density"] = FTG.density(r["onesite"], r["twosite"], case["pdims"], case["onames"], case["tnames"])
+        return r
+
+    return FTG.run(env, case["pdims"], case["vdims"], case["evolution"], case["nsteps"], case["tau"],
+                   case["interval"], measure)
