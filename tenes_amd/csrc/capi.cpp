@@ -581,9 +581,7 @@ int tenes_calc_ctm_environment_density(tenes_env *env,
     std::vector<tn::Tensor> A(a[8].size());
     for (size_t i = 0; i < A.size(); ++i)
       A[i] = tpo_ket(a[8][i], identity_op(env->ctx, a[8][i].shape[4]));
-    tenes::itps::PEPS_Parameters pp = to_params(p);
-    TN_REQUIRE(pp.CTM_Projector_corner,
-               "the single-layer CTM supports corner projectors only");
+    const tenes::itps::PEPS_Parameters pp = to_params(p);
     tn::SingleLayer single(*env->ctx->c, true);
     double sm = 0.0;
     const int n = tenes::itps::core::Calc_CTM_Environment(

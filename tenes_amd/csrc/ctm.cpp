@@ -332,20 +332,25 @@ void updown_blocks_begin(
                     tn::bra(Tn3), Axes{2, 5}, Axes{3, 2}),
       Axes{2, 3, 4}, Axes{3, 1, 6});
   tensor LB = enlarged_LB(C4, eT7, eT6, Tn4);
+  // The single-layer version (ctm_single.cpp:267-436) truncates its full SVD
+  // with a loop over the first e78 singular values only (ctm_single.cpp:400),
+  // which keeps at most e78 directions and zero-pads the projectors to the
+  // cut; that behaviour is not reproduced here.
   TN_REQUIRE(!ctx.single_layer(),
              "up-down projectors of the single-layer network are not supported");
   const bool rsvd = t23 != 1 && pp.Use_RSVD;
-  int cap = pp.CHI;
-  if (rsvd) cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * t23);
   // R1 = RT*LT, R2 = RB*LB, legs (t23, e34, tc23, t41, e78, tc41)
   tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 1, 2});
   tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{3, 4, 5});
+  const int64_t b23 = R1.shape[2];
+  int cap = pp.CHI;
+  if (rsvd) cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * b23);
   tensor M;
   {
     tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
     M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
   }
-  projectors_begin(ctx, M, t23, e34, t23, t23, e34, t23, cap, R1,
+  projectors_begin(ctx, M, t23, e34, b23, t23, e34, R2.shape[2], cap, R1,
                    Axes{0, 1, 2}, R2, Axes{0, 1, 2}, pp, rsvd, slot, key, job);
 }
 

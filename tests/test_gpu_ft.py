@@ -95,3 +95,20 @@ def test_gpu_single_layer_ctm_vs_oracle(gpu_ctx):
     mine = {(x[0], x[1]): x[2] for x in rec["correlation_length"]}
     for d, f, x, _, _ in xi:
         assert abs(mine[(d, f)] - x) < 1e-9 * abs(x), (d, f, mine[(d, f)], x)
+
+
+def test_gpu_single_layer_updown_projectors_refused(gpu_ctx):
+    """CTM_Projector_corner = 0 on an iTPO environment is an argument error
+    (TeNeS's single-layer up-down truncation, ctm_single.cpp:400, is not
+    reproduced), and the environment stays usable afterwards."""
+    from tenes_amd import finite_temperature as FTG
+
+    case = F.setup("FT_TFI_square")
+    Tn, _ = _oracle_tpo(case, 50)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    with pytest.raises(ta.TenesError, match="up-down projectors"):
+        genv.calc_ctm_environment_density(ta.peps_parameters(CHI=10, Max_CTM_Iteration=2,
+                                                             CTM_Projector_corner=0))
+    n, _ = genv.calc_ctm_environment_density(ta.peps_parameters(CHI=10, Max_CTM_Iteration=2))
+    assert n >= 1 and np.isfinite(genv.contract_one_site(0))
