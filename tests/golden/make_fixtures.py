@@ -159,7 +159,7 @@ def main():
     fixtures = {}
     for name, prefix in (("TE_TFI", "TE_"), ("AntiferroHeisenberg_real", ""), ("J1J2_AFH", ""),
                          ("Honeycomb", ""), ("Honeycomb_skew", ""),
-                         ("FT_TFI_square", "FT_"), ("FT_Kitaev", "FT_")):
+                         ("FT_TFI_square", "FT_"), ("FT_Kitaev", "FT_"), ("RSVD", "")):
         fx = input_fixture(name)
         fx["outputs"] = outputs(f"output_{name}", prefix)
         fixtures[name] = fx

@@ -272,6 +272,46 @@ def test_ctm_environment_updown(gpu_ctx):
     assert np.max(np.abs(gmz - mz)) < 1e-10
 
 
+def test_ctm_environment_rsvd_updown(gpu_ctx):
+    """The Use_RSVD branch with up-down projectors (RSVD.toml's CTM settings:
+    chi = 5, projector_corner = false; ctm.cpp:401-439), whose oracle is pinned
+    to TeNeS's output_RSVD: on real simple-update tensors of that input the
+    GPU run equals the oracle's to 1e-10 (the RSVD cut min(CHI, e34 t23^2),
+    the 4-block projectors, energies and corner spectra)."""
+    import golden_io as G
+    from oracle import simple_update as SU
+
+    fx = G.load("RSVD")
+    LX, LY, skew, pdims, vdims, dirs, noises = G.unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=True)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    Tn, lam = SU.simple_update_steps(Tn, lam, lat, G.evolution(fx, True), 100)
+    p = O.PEPSParameters(CHI=5, Use_RSVD=True, CTM_Projector_corner=False)
+    env, count, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    gcount, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=5, Use_RSVD=1, CTM_Projector_corner=0))
+    assert gcount == count
+    for i in range(4):
+        for k in ("C1", "C2", "C3", "C4"):
+            a = O.normalized_spectrum(getattr(env, k)[i])
+            b = O.normalized_spectrum(genv.get(k, i).numpy())
+            assert a.shape == b.shape and np.max(np.abs(a - b)) < 1e-10, (k, i)
+    one, _, two, _ = G.operators(fx, 4, True)
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    for x in one:
+        got = genv.contract_one_site(x["site"], x["op"]) / genv.contract_one_site(x["site"])
+        assert abs(got - o[x["group"]][x["site"]]) < 1e-10, x["name"]
+    for x in two:
+        if "op" not in x:
+            continue
+        got = genv.contract_two_sites_op12(x["source"], x["dx"], x["dy"], x["op"]) / \
+            genv.contract_cluster(x["source"], x["dx"], x["dy"])
+        assert abs(got - w[x["group"]][(x["source"], x["dx"], x["dy"])]) < 1e-10
+
+
 def test_simple_update_bond_gpu(gpu_ctx):
     """Simple_update_bond: bond weights (gauge invariant) and the two-site
     wavefunction contracted over the updated bond against the oracle."""

@@ -250,3 +250,52 @@ def test_honeycomb_end_to_end_goldens(name, tol):
         f = r.split()
         key = (int(f[0]), int(f[1]), int(f[2]), int(f[3]), int(f[4]))
         assert abs(cor[key] - complex(float(f[5]), float(f[6]))) < tol, (key, cor[key], f)
+
+
+def test_rsvd_updown_end_to_end_goldens():
+    """test/data/RSVD.toml (complex tensors, 2x2, D=2, chi=5, use_rsvd = true,
+    projector_corner = false): SU 100 + FU 1 + CTM with the up-down
+    projectors of the RSVD branch (ctm.cpp:401-439; the oracle takes the
+    exact SVD with that branch's cut) against output_RSVD/: the energy,
+    the one- and two-site densities, the correlation lengths and the
+    correlation functions to 1e-12.  At these sizes mptensor's randomized SVD
+    returns the exact leading triplets, which pins the branch's cut and
+    projector formulas."""
+    from oracle import full_update as FU, simple_update as SU
+    from oracle.lattice import SquareLattice
+
+    fx = G.load("RSVD")
+    LX, LY, skew, pdims, vdims, dirs, noises = G.unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=False)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    Tn, lam = SU.simple_update_steps(Tn, lam, lat, G.evolution(fx, False),
+                                     fx["parameter"]["simple_update"]["num_step"])
+    c = fx["parameter"]["ctm"]
+    p = ctm.PEPSParameters(CHI=c["dimension"], Use_RSVD=c["use_rsvd"],
+                           CTM_Projector_corner=c["projector_corner"])
+    Tn = FU.full_update(Tn, lat, G.evolution(fx, False, "evolution_full"),
+                        fx["parameter"]["full_update"]["num_step"], p, FU.FUParameters())
+    env, count, _ = ctm.Calc_CTM_Environment(Tn, p, lat)
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, False)
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    dens = itps.density(o, w, lat, pdims, onames, tnames)
+    gold = G.density_table(fx["outputs"]["density"], timed=False)
+    assert len(gold) >= 8
+    for k, v in gold.items():
+        assert abs(dens[k] - v) < 1e-12, (k, dens[k], v)
+    for r, (d, f, xi, es, ev) in zip(fx["outputs"]["correlation_length"],
+                                     itps.correlation_lengths(env, Tn, lat, TM.TMParameters())):
+        ref = [float(x) for x in r.split()]
+        assert (ref[0], ref[1]) == (d, f)
+        assert abs(xi - ref[2]) < 1e-10 * abs(ref[2])
+    site_ops = {(x["site"], x["group"]): x["op"] for x in one}
+    cor = itps.measure_correlation(env, Tn, lat, site_ops, fx["correlation"]["r_max"],
+                                   [tuple(x) for x in fx["correlation"]["operators"]])
+    rows = fx["outputs"]["correlation"]
+    assert len(rows) == len(cor)
+    for r in rows:
+        f = r.split()
+        key = (int(f[0]), int(f[1]), int(f[2]), int(f[3]), int(f[4]))
+        assert abs(cor[key] - complex(float(f[5]), float(f[6]))) < 1e-12, (key, cor[key], f)
