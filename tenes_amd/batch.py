@@ -115,6 +115,9 @@ def main(argv=None):
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--su-steps", type=int, default=300)
+    ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
+                    help="also run the batch with contraction_fp32 = 1 or 2 and report each "
+                         "problem's energy error against the FP64 batch")
     args = ap.parse_args(argv)
     # read by the HIP runtime when the first context initialises it
     os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
@@ -157,6 +160,16 @@ def main(argv=None):
         "energies": energies,
         "max_energy_diff_batched_vs_one_context": float(np.max(np.abs(np.array(energies) - np.array(e1)))),
     }
+    if args.mixed:
+        # the same batch with the moves' contractions on the FP32 MFMA
+        run_ctm(envs, args.warmup, True, contraction_fp32=args.mixed)
+        dtm = run_ctm(envs, args.sweeps, False, contraction_fp32=args.mixed)
+        em = [energy(e, H) for e, H in zip(envs, hams)]
+        err = [abs(a - b) / abs(b) for a, b in zip(em, energies)]
+        log(f"mixed mode {args.mixed}: {args.problems} x {args.sweeps} sweeps in {dtm:.2f} s -> "
+            f"{args.problems * args.sweeps / dtm:.1f} sweeps/s; energy rel err vs FP64 max {max(err):.2e}")
+        out["mixed"] = {"contraction_fp32": args.mixed, "value": args.problems * args.sweeps / dtm,
+                        "energy_rel_err": err, "max_energy_rel_err": max(err)}
     print(json.dumps(out), flush=True)
 
 
