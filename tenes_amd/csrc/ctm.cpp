@@ -7,8 +7,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <limits>
 #include <map>
+#include <memory>
 
 #include "kernels.hpp"
 
@@ -354,45 +356,6 @@ void updown_blocks_begin(
                    Axes{0, 1, 2}, R2, Axes{0, 1, 2}, pp, rsvd, slot, key, job);
 }
 
-// The L projectors of a move are independent chains of small kernels, each
-// with one host synchronisation; with L > 1 they run overlapped on the lanes
-// of the forked context.  begin(i, slot, job) issues projector i.
-template <class Begin>
-void run_projectors(tn::Context &ctx, int L, Begin begin,
-                    const PEPS_Parameters &pp, std::vector<tensor> &PUs,
-                    std::vector<tensor> &PLs) {
-  std::vector<ProjectorJob> jobs(L);
-  const int nl = std::min(L, ctx.max_lanes());
-  const bool fork = nl > 1 && L <= kSyncSlot && !ctx.forked();
-  if (fork) ctx.fork(nl);
-  for (int i = 0; i < L; ++i) {
-    if (fork) ctx.use_lane(i % nl);
-    begin(i, fork ? i : kSyncSlot, jobs[i]);
-    if (!fork) projectors_finish(ctx, jobs[i], pp);
-  }
-  for (int i = 0; i < L; ++i) {
-    if (fork) ctx.use_lane(i % nl);
-    projectors_finish(ctx, jobs[i], pp);
-    PUs[i] = jobs[i].PU;
-    PLs[i] = jobs[i].PL;
-  }
-  if (fork) ctx.join();
-}
-
-// Independent per-row work of a move (the renormalisation), overlapped on
-// the lanes of the forked context when L > 1.
-template <class Body>
-void run_lanes(tn::Context &ctx, int L, Body body) {
-  const int nl = std::min(L, ctx.max_lanes());
-  const bool fork = nl > 1 && !ctx.forked();
-  if (fork) ctx.fork(nl);
-  for (int i = 0; i < L; ++i) {
-    if (fork) ctx.use_lane(i % nl);
-    body(i);
-  }
-  if (fork) ctx.join();
-}
-
 }  // namespace
 
 // ctm.cpp:174-318
@@ -449,36 +412,93 @@ void Calc_Next_eT(const tensor &eT8, const tensor &Tn1, const tensor &PU,
   tn::normalize_max_abs(eT_out);
 }
 
-// ctm.cpp:562-616
-void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
-               const std::vector<tensor> &C3, std::vector<tensor> &C4,
-               const std::vector<tensor> &eTt, const std::vector<tensor> &eTr,
-               const std::vector<tensor> &eTb, std::vector<tensor> &eTl,
-               const std::vector<tensor> &Tn, int ix,
-               const PEPS_Parameters &pp, const SquareLattice &lat) {
-  const int L = lat.LY_noskew;
-  tn::Context &ctx = *Tn[0].ctx();
+namespace {
+
+// One directional move split at its projector synchronisation: begin(row,
+// slot, job) issues the projector of a row; once all are done, snapshot()
+// keeps the arrays the renormalisation reads, and update(row, PUs, PLs)
+// issues one row's renormalisation (Calc_Next_CTM and two Calc_Next_eT).
+struct MoveSpec {
+  int L = 0;
+  std::function<void(int, int, ProjectorJob &)> begin;
+  std::function<void()> snapshot;
+  std::function<void(int, const std::vector<tensor> &,
+                     const std::vector<tensor> &)>
+      update;
+};
+
+// Runs one move, or two moves that read and write disjoint arrays, with all
+// their projector chains overlapped on the lanes of the forked context.
+void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
+               const std::vector<MoveSpec *> &moves) {
   tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
-  std::vector<tensor> PUs(L), PLs(L);
-  run_projectors(
-      ctx, L,
-      [&](int iy, int slot, ProjectorJob &job) {
-        const ProjKey key{environment_id(C1), 0, ix, iy};
-        const int i = lat.index(ix, iy);
-        const int j = lat.right(i);
-        const int k = lat.bottom(j);
-        const int l = lat.left(k);
-        if (pp.CTM_Projector_corner)
-          left_block_begin(C1[i], C4[l], eTt[i], eTb[l], eTl[l], eTl[i], Tn[i],
-                           Tn[l], pp, slot, &key, job);
-        else
-          updown_blocks_begin(C1[i], C2[j], C3[k], C4[l], eTt[i], eTt[j],
-                              eTr[j], eTr[k], eTb[k], eTb[l], eTl[l], eTl[i],
-                              Tn[i], Tn[j], Tn[k], Tn[l], pp, slot, &key, job);
-      },
-      pp, PUs, PLs);
-  const std::vector<tensor> C1_bak = C1, C4_bak = C4, eTl_bak = eTl;
-  run_lanes(ctx, L, [&](int iy) {
+  int total = 0;
+  for (auto *m : moves) total += m->L;
+  const int nl = std::min(total, ctx.max_lanes());
+  const bool fork = nl > 1 && total <= kSyncSlot && !ctx.forked();
+  std::vector<std::vector<ProjectorJob>> jobs(moves.size());
+  std::vector<std::vector<tensor>> PUs(moves.size()), PLs(moves.size());
+  if (fork) ctx.fork(nl);
+  int q = 0;
+  for (size_t m = 0; m < moves.size(); ++m) {
+    jobs[m].resize(moves[m]->L);
+    for (int i = 0; i < moves[m]->L; ++i, ++q) {
+      if (fork) ctx.use_lane(q % nl);
+      moves[m]->begin(i, fork ? q : kSyncSlot, jobs[m][i]);
+      if (!fork) projectors_finish(ctx, jobs[m][i], pp);
+    }
+  }
+  q = 0;
+  for (size_t m = 0; m < moves.size(); ++m) {
+    PUs[m].resize(moves[m]->L);
+    PLs[m].resize(moves[m]->L);
+    for (int i = 0; i < moves[m]->L; ++i, ++q) {
+      if (fork) ctx.use_lane(q % nl);
+      projectors_finish(ctx, jobs[m][i], pp);
+      PUs[m][i] = jobs[m][i].PU;
+      PLs[m][i] = jobs[m][i].PL;
+    }
+  }
+  // a row's renormalisation uses the projectors of the neighbouring rows
+  if (fork) ctx.join();
+  for (auto *m : moves) m->snapshot();
+  if (fork) ctx.fork(nl);
+  q = 0;
+  for (size_t m = 0; m < moves.size(); ++m)
+    for (int i = 0; i < moves[m]->L; ++i, ++q) {
+      if (fork) ctx.use_lane(q % nl);
+      moves[m]->update(i, PUs[m], PLs[m]);
+    }
+  if (fork) ctx.join();
+}
+
+using Arr = std::vector<tensor>;
+
+// ctm.cpp:562-616
+MoveSpec left_spec(Arr &C1, const Arr &C2, const Arr &C3, Arr &C4,
+                   const Arr &eTt, const Arr &eTr, const Arr &eTb, Arr &eTl,
+                   const Arr &Tn, int ix, const PEPS_Parameters &pp,
+                   const SquareLattice &lat) {
+  MoveSpec m;
+  const int L = m.L = lat.LY_noskew;
+  auto bak = std::make_shared<std::array<Arr, 3>>();
+  m.begin = [&, ix](int iy, int slot, ProjectorJob &job) {
+    const ProjKey key{environment_id(C1), 0, ix, iy};
+    const int i = lat.index(ix, iy);
+    const int j = lat.right(i);
+    const int k = lat.bottom(j);
+    const int l = lat.left(k);
+    if (pp.CTM_Projector_corner)
+      left_block_begin(C1[i], C4[l], eTt[i], eTb[l], eTl[l], eTl[i], Tn[i],
+                       Tn[l], pp, slot, &key, job);
+    else
+      updown_blocks_begin(C1[i], C2[j], C3[k], C4[l], eTt[i], eTt[j], eTr[j],
+                          eTr[k], eTb[k], eTb[l], eTl[l], eTl[i], Tn[i],
+                          Tn[j], Tn[k], Tn[l], pp, slot, &key, job);
+  };
+  m.snapshot = [&, bak] { *bak = {C1, C4, eTl}; };
+  m.update = [&, ix, L, bak](int iy, const Arr &PUs, const Arr &PLs) {
+    const Arr &C1_bak = (*bak)[0], &C4_bak = (*bak)[1], &eTl_bak = (*bak)[2];
     const int i = lat.index(ix, iy);
     const int j = lat.right(i);
     const int k = lat.bottom(j);
@@ -489,43 +509,39 @@ void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
                   PLs[iy_down], C1[j], C4[k]);
     Calc_Next_eT(eTl_bak[i], Tn[i], PUs[iy], PLs[iy_up], eTl[j]);
     Calc_Next_eT(eTl_bak[l], Tn[l], PUs[iy_down], PLs[iy], eTl[k]);
-  });
+  };
+  return m;
 }
 
 // ctm.cpp:618-681
-void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
-                std::vector<tensor> &C3, const std::vector<tensor> &C4,
-                const std::vector<tensor> &eTt, std::vector<tensor> &eTr,
-                const std::vector<tensor> &eTb, const std::vector<tensor> &eTl,
-                const std::vector<tensor> &Tn, int ix,
-                const PEPS_Parameters &pp, const SquareLattice &lat) {
-  const int L = lat.LY_noskew;
+MoveSpec right_spec(const Arr &C1, Arr &C2, Arr &C3, const Arr &C4,
+                    const Arr &eTt, Arr &eTr, const Arr &eTb, const Arr &eTl,
+                    const Arr &Tn, int ix, const PEPS_Parameters &pp,
+                    const SquareLattice &lat) {
+  MoveSpec m;
+  const int L = m.L = lat.LY_noskew;
   const Axes rot{2, 3, 0, 1, 4};
-  tn::Context &ctx = *Tn[0].ctx();
-  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
-  std::vector<tensor> PUs(L), PLs(L);
-  run_projectors(
-      ctx, L,
-      [&](int iy, int slot, ProjectorJob &job) {
-        const ProjKey key{environment_id(C1), 1, ix, iy};
-        const int k = lat.index(ix, iy);
-        const int l = lat.left(k);
-        const int i = lat.top(l);
-        const int j = lat.right(i);
-        if (pp.CTM_Projector_corner)
-          left_block_begin(C3[k], C2[j], eTb[k], eTt[j], eTr[j], eTr[k],
-                           tn::transpose(Tn[k], rot), tn::transpose(Tn[j], rot),
-                           pp, slot, &key, job);
-        else
-          updown_blocks_begin(
-              C3[k], C4[l], C1[i], C2[j], eTb[k], eTb[l], eTl[l], eTl[i],
-              eTt[i], eTt[j], eTr[j], eTr[k], tn::transpose(Tn[k], rot),
-              tn::transpose(Tn[l], rot), tn::transpose(Tn[i], rot),
-              tn::transpose(Tn[j], rot), pp, slot, &key, job);
-      },
-      pp, PUs, PLs);
-  const std::vector<tensor> C2_bak = C2, C3_bak = C3, eTr_bak = eTr;
-  run_lanes(ctx, L, [&](int iy) {
+  auto bak = std::make_shared<std::array<Arr, 3>>();
+  m.begin = [&, ix, rot](int iy, int slot, ProjectorJob &job) {
+    const ProjKey key{environment_id(C1), 1, ix, iy};
+    const int k = lat.index(ix, iy);
+    const int l = lat.left(k);
+    const int i = lat.top(l);
+    const int j = lat.right(i);
+    if (pp.CTM_Projector_corner)
+      left_block_begin(C3[k], C2[j], eTb[k], eTt[j], eTr[j], eTr[k],
+                       tn::transpose(Tn[k], rot), tn::transpose(Tn[j], rot),
+                       pp, slot, &key, job);
+    else
+      updown_blocks_begin(
+          C3[k], C4[l], C1[i], C2[j], eTb[k], eTb[l], eTl[l], eTl[i], eTt[i],
+          eTt[j], eTr[j], eTr[k], tn::transpose(Tn[k], rot),
+          tn::transpose(Tn[l], rot), tn::transpose(Tn[i], rot),
+          tn::transpose(Tn[j], rot), pp, slot, &key, job);
+  };
+  m.snapshot = [&, bak] { *bak = {C2, C3, eTr}; };
+  m.update = [&, ix, L, rot, bak](int iy, const Arr &PUs, const Arr &PLs) {
+    const Arr &C2_bak = (*bak)[0], &C3_bak = (*bak)[1], &eTr_bak = (*bak)[2];
     const int k = lat.index(ix, iy);
     const int l = lat.left(k);
     const int i = lat.top(l);
@@ -538,43 +554,39 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
                  PLs[iy_down], eTr[l]);
     Calc_Next_eT(eTr_bak[j], tn::transpose(Tn[j], rot), PUs[iy_up], PLs[iy],
                  eTr[i]);
-  });
+  };
+  return m;
 }
 
 // ctm.cpp:683-746
-void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
-              const std::vector<tensor> &C3, const std::vector<tensor> &C4,
-              std::vector<tensor> &eTt, const std::vector<tensor> &eTr,
-              const std::vector<tensor> &eTb, const std::vector<tensor> &eTl,
-              const std::vector<tensor> &Tn, int iy, const PEPS_Parameters &pp,
-              const SquareLattice &lat) {
-  const int L = lat.LX_noskew;
+MoveSpec top_spec(Arr &C1, Arr &C2, const Arr &C3, const Arr &C4, Arr &eTt,
+                  const Arr &eTr, const Arr &eTb, const Arr &eTl,
+                  const Arr &Tn, int iy, const PEPS_Parameters &pp,
+                  const SquareLattice &lat) {
+  MoveSpec m;
+  const int L = m.L = lat.LX_noskew;
   const Axes rot{1, 2, 3, 0, 4};
-  tn::Context &ctx = *Tn[0].ctx();
-  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
-  std::vector<tensor> PUs(L), PLs(L);
-  run_projectors(
-      ctx, L,
-      [&](int ix, int slot, ProjectorJob &job) {
-        const ProjKey key{environment_id(C1), 2, iy, ix};
-        const int j = lat.index(ix, iy);
-        const int k = lat.bottom(j);
-        const int l = lat.left(k);
-        const int i = lat.top(l);
-        if (pp.CTM_Projector_corner)
-          left_block_begin(C2[j], C1[i], eTr[j], eTl[i], eTt[i], eTt[j],
-                           tn::transpose(Tn[j], rot), tn::transpose(Tn[i], rot),
-                           pp, slot, &key, job);
-        else
-          updown_blocks_begin(
-              C2[j], C3[k], C4[l], C1[i], eTr[j], eTr[k], eTb[k], eTb[l],
-              eTl[l], eTl[i], eTt[i], eTt[j], tn::transpose(Tn[j], rot),
-              tn::transpose(Tn[k], rot), tn::transpose(Tn[l], rot),
-              tn::transpose(Tn[i], rot), pp, slot, &key, job);
-      },
-      pp, PUs, PLs);
-  const std::vector<tensor> C1_bak = C1, C2_bak = C2, eTt_bak = eTt;
-  run_lanes(ctx, L, [&](int ix) {
+  auto bak = std::make_shared<std::array<Arr, 3>>();
+  m.begin = [&, iy, rot](int ix, int slot, ProjectorJob &job) {
+    const ProjKey key{environment_id(C1), 2, iy, ix};
+    const int j = lat.index(ix, iy);
+    const int k = lat.bottom(j);
+    const int l = lat.left(k);
+    const int i = lat.top(l);
+    if (pp.CTM_Projector_corner)
+      left_block_begin(C2[j], C1[i], eTr[j], eTl[i], eTt[i], eTt[j],
+                       tn::transpose(Tn[j], rot), tn::transpose(Tn[i], rot),
+                       pp, slot, &key, job);
+    else
+      updown_blocks_begin(
+          C2[j], C3[k], C4[l], C1[i], eTr[j], eTr[k], eTb[k], eTb[l], eTl[l],
+          eTl[i], eTt[i], eTt[j], tn::transpose(Tn[j], rot),
+          tn::transpose(Tn[k], rot), tn::transpose(Tn[l], rot),
+          tn::transpose(Tn[i], rot), pp, slot, &key, job);
+  };
+  m.snapshot = [&, bak] { *bak = {C1, C2, eTt}; };
+  m.update = [&, iy, L, rot, bak](int ix, const Arr &PUs, const Arr &PLs) {
+    const Arr &C1_bak = (*bak)[0], &C2_bak = (*bak)[1], &eTt_bak = (*bak)[2];
     const int j = lat.index(ix, iy);
     const int k = lat.bottom(j);
     const int l = lat.left(k);
@@ -587,43 +599,39 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
                  PLs[ix_right], eTt[k]);
     Calc_Next_eT(eTt_bak[i], tn::transpose(Tn[i], rot), PUs[ix_left], PLs[ix],
                  eTt[l]);
-  });
+  };
+  return m;
 }
 
 // ctm.cpp:748-813
-void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
-                 std::vector<tensor> &C3, std::vector<tensor> &C4,
-                 const std::vector<tensor> &eTt,
-                 const std::vector<tensor> &eTr, std::vector<tensor> &eTb,
-                 const std::vector<tensor> &eTl, const std::vector<tensor> &Tn,
-                 int iy, const PEPS_Parameters &pp, const SquareLattice &lat) {
-  const int L = lat.LX_noskew;
+MoveSpec bottom_spec(const Arr &C1, const Arr &C2, Arr &C3, Arr &C4,
+                     const Arr &eTt, const Arr &eTr, Arr &eTb, const Arr &eTl,
+                     const Arr &Tn, int iy, const PEPS_Parameters &pp,
+                     const SquareLattice &lat) {
+  MoveSpec m;
+  const int L = m.L = lat.LX_noskew;
   const Axes rot{3, 0, 1, 2, 4};
-  tn::Context &ctx = *Tn[0].ctx();
-  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
-  std::vector<tensor> PUs(L), PLs(L);
-  run_projectors(
-      ctx, L,
-      [&](int ix, int slot, ProjectorJob &job) {
-        const ProjKey key{environment_id(C1), 3, iy, ix};
-        const int l = lat.index(ix, iy);
-        const int i = lat.top(l);
-        const int j = lat.right(i);
-        const int k = lat.bottom(j);
-        if (pp.CTM_Projector_corner)
-          left_block_begin(C4[l], C3[k], eTl[l], eTr[k], eTb[k], eTb[l],
-                           tn::transpose(Tn[l], rot), tn::transpose(Tn[k], rot),
-                           pp, slot, &key, job);
-        else
-          updown_blocks_begin(
-              C4[l], C1[i], C2[j], C3[k], eTl[l], eTl[i], eTt[i], eTt[j],
-              eTr[j], eTr[k], eTb[k], eTb[l], tn::transpose(Tn[l], rot),
-              tn::transpose(Tn[i], rot), tn::transpose(Tn[j], rot),
-              tn::transpose(Tn[k], rot), pp, slot, &key, job);
-      },
-      pp, PUs, PLs);
-  const std::vector<tensor> C3_bak = C3, C4_bak = C4, eTb_bak = eTb;
-  run_lanes(ctx, L, [&](int ix) {
+  auto bak = std::make_shared<std::array<Arr, 3>>();
+  m.begin = [&, iy, rot](int ix, int slot, ProjectorJob &job) {
+    const ProjKey key{environment_id(C1), 3, iy, ix};
+    const int l = lat.index(ix, iy);
+    const int i = lat.top(l);
+    const int j = lat.right(i);
+    const int k = lat.bottom(j);
+    if (pp.CTM_Projector_corner)
+      left_block_begin(C4[l], C3[k], eTl[l], eTr[k], eTb[k], eTb[l],
+                       tn::transpose(Tn[l], rot), tn::transpose(Tn[k], rot),
+                       pp, slot, &key, job);
+    else
+      updown_blocks_begin(
+          C4[l], C1[i], C2[j], C3[k], eTl[l], eTl[i], eTt[i], eTt[j], eTr[j],
+          eTr[k], eTb[k], eTb[l], tn::transpose(Tn[l], rot),
+          tn::transpose(Tn[i], rot), tn::transpose(Tn[j], rot),
+          tn::transpose(Tn[k], rot), pp, slot, &key, job);
+  };
+  m.snapshot = [&, bak] { *bak = {C3, C4, eTb}; };
+  m.update = [&, iy, L, rot, bak](int ix, const Arr &PUs, const Arr &PLs) {
+    const Arr &C3_bak = (*bak)[0], &C4_bak = (*bak)[1], &eTb_bak = (*bak)[2];
     const int l = lat.index(ix, iy);
     const int i = lat.top(l);
     const int j = lat.right(i);
@@ -636,7 +644,54 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
                  eTb[i]);
     Calc_Next_eT(eTb_bak[k], tn::transpose(Tn[k], rot), PUs[ix_right],
                  PLs[ix], eTb[j]);
-  });
+  };
+  return m;
+}
+
+}  // namespace
+
+// ctm.cpp:562-616
+void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
+               const std::vector<tensor> &C3, std::vector<tensor> &C4,
+               const std::vector<tensor> &eTt, const std::vector<tensor> &eTr,
+               const std::vector<tensor> &eTb, std::vector<tensor> &eTl,
+               const std::vector<tensor> &Tn, int ix,
+               const PEPS_Parameters &pp, const SquareLattice &lat) {
+  MoveSpec m = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, pp, lat);
+  run_moves(*Tn[0].ctx(), pp, {&m});
+}
+
+// ctm.cpp:618-681
+void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
+                std::vector<tensor> &C3, const std::vector<tensor> &C4,
+                const std::vector<tensor> &eTt, std::vector<tensor> &eTr,
+                const std::vector<tensor> &eTb, const std::vector<tensor> &eTl,
+                const std::vector<tensor> &Tn, int ix,
+                const PEPS_Parameters &pp, const SquareLattice &lat) {
+  MoveSpec m = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, pp, lat);
+  run_moves(*Tn[0].ctx(), pp, {&m});
+}
+
+// ctm.cpp:683-746
+void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
+              const std::vector<tensor> &C3, const std::vector<tensor> &C4,
+              std::vector<tensor> &eTt, const std::vector<tensor> &eTr,
+              const std::vector<tensor> &eTb, const std::vector<tensor> &eTl,
+              const std::vector<tensor> &Tn, int iy, const PEPS_Parameters &pp,
+              const SquareLattice &lat) {
+  MoveSpec m = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, pp, lat);
+  run_moves(*Tn[0].ctx(), pp, {&m});
+}
+
+// ctm.cpp:748-813
+void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
+                 std::vector<tensor> &C3, std::vector<tensor> &C4,
+                 const std::vector<tensor> &eTt,
+                 const std::vector<tensor> &eTr, std::vector<tensor> &eTb,
+                 const std::vector<tensor> &eTl, const std::vector<tensor> &Tn,
+                 int iy, const PEPS_Parameters &pp, const SquareLattice &lat) {
+  MoveSpec m = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, pp, lat);
+  run_moves(*Tn[0].ctx(), pp, {&m});
 }
 
 namespace {
@@ -883,16 +938,38 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
   const int LX = lat.LX_noskew, LY = lat.LY_noskew;
   while (!convergence && count < pp.Max_CTM_Iteration) {
     const PEPS_Parameters &ps = pp;
-    for (int ix = 0; ix < LX; ++ix)
-      Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, ps, lat);
-    for (int ix = 0; ix > -LX; --ix)
-      Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX, ps,
-                 lat);
-    for (int iy = 0; iy > -LY; --iy)
-      Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY, ps,
-               lat);
-    for (int iy = 0; iy < LY; ++iy)
-      Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, ps, lat);
+    if (pp.CTM_Projector_corner) {
+      // With corner projectors the left moves read and write only C1, C4,
+      // eTl (and read eTt, eTb, Tn), the right moves only C2, C3, eTr: the
+      // two chains of ctm.cpp:1205-1214 are independent and run side by
+      // side, step s of one with step s of the other; likewise the top
+      // (C1, C2, eTt) and bottom (C3, C4, eTb) chains of ctm.cpp:1215-1224.
+      // Each chain keeps the reference's order, so the result is the same.
+      tn::Context &ctx = *Tn[0].ctx();
+      for (int s = 0; s < LX; ++s) {
+        MoveSpec a = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
+        MoveSpec b = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
+                                (-s + 1 + LX) % LX, ps, lat);
+        run_moves(ctx, ps, {&a, &b});
+      }
+      for (int s = 0; s < LY; ++s) {
+        MoveSpec a = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
+                              (-s + 1 + LY) % LY, ps, lat);
+        MoveSpec b = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
+        run_moves(ctx, ps, {&a, &b});
+      }
+    } else {
+      for (int ix = 0; ix < LX; ++ix)
+        Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, ps, lat);
+      for (int ix = 0; ix > -LX; --ix)
+        Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX,
+                   ps, lat);
+      for (int iy = 0; iy > -LY; --iy)
+        Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY,
+                 ps, lat);
+      for (int iy = 0; iy < LY; ++iy)
+        Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, ps, lat);
+    }
     convergence = Check_Convergence_CTM(C1, C2, C3, C4, C1_old, C2_old,
                                         C3_old, C4_old, pp, lat, sig_max);
     count += 1;
