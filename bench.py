@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--no-check", action="store_true",
                     help="skip the untimed accuracy run against the oracle fixture")
     ap.add_argument("--power-iters", type=int, default=None)
+    ap.add_argument("--warm-iters", type=int, default=None,
+                    help="range-finder iterations of a warm-started projector SVD (library default 8)")
     ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
                     help="contractions of the moves on the FP32 MFMA: 1 all but the projector matrix "
                          "R = LT LB, 2 R as well (projector SVDs and the rest FP64)")
@@ -186,6 +188,8 @@ def main():
     kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)
     if args.power_iters is not None:
         kw["svd_power_iters"] = args.power_iters
+    if args.warm_iters is not None:
+        kw["svd_warm_power_iters"] = args.warm_iters
     if args.mixed:
         kw["contraction_fp32"] = args.mixed
     t0 = time.perf_counter()
