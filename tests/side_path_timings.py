@@ -6,7 +6,10 @@ numpy oracle on the same inputs (the bench of the sweep itself is bench.py):
   * correlation functions along +x, r = 1..10, D=5 chi=100 (one left site,
     two right operators);
   * energy and gradient of the bond Hamiltonian with the environment fixed,
-    D=4 chi=64 random tensors.
+    D=4 chi=64 random tensors;
+  * the finite-temperature pipelines of TeNeS's FT_TFI_square and FT_Kitaev
+    inputs (TPO simple update, single-layer CTM and all measurements at every
+    measured temperature), end to end.
 
     python tests/side_path_timings.py
 
@@ -110,6 +113,20 @@ def main():
     tg, _ = _best(lambda: e4.bond_energy(bonds, grad=True))
     tc, _ = _best(lambda: A.energy_and_grad(h4, T4, lat, bonds), reps=1)
     out["energy_gradient_d4_chi64"] = {"gpu_s": tg, "cpu_oracle_s": tc}
+
+    # finite temperature, end to end
+    sys.path.insert(0, os.path.join(root, "tests"))
+    import ft_cases
+
+    for name in ("FT_TFI_square", "FT_Kitaev"):
+        case = ft_cases.setup(name)
+        t0 = time.perf_counter()
+        ft_cases.gpu_records(ctx, case)
+        tg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ft_cases.oracle_records(case)
+        tc = time.perf_counter() - t0
+        out[f"finite_temperature_{name}"] = {"gpu_s": tg, "cpu_oracle_s": tc}
     print(json.dumps(out), flush=True)
 
 
