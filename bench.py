@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--no-check", action="store_true",
                     help="skip the untimed accuracy run against the oracle fixture")
     ap.add_argument("--power-iters", type=int, default=None)
+    ap.add_argument("--autotune", type=int, default=0, choices=(0, 1),
+                    help="GEMM autotuning of the context (tuned during the warm-up sweeps)")
     ap.add_argument("--warm-iters", type=int, default=None,
                     help="range-finder iterations of a warm-started projector SVD (library default 8)")
     ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
@@ -185,6 +187,8 @@ def main():
     import tenes_amd as ta
 
     ctx = ta.Context(0, seed=2024)
+    if args.autotune:
+        ctx.set_autotune(True)
     fx = load_fixture(args.D, args.chi, args.su_steps)
     env, H, Tn = build_inputs(ta, ctx, args.D, args.su_steps, 0.01, fx)
     kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)

@@ -67,6 +67,14 @@ int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n);
  * CTM moves set it themselves from tenes_peps_parameters.contraction_fp32.
  * No counterpart in the reference. */
 int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on);
+/* GEMM autotuning of a context (on != 0): the first contraction of every
+ * plan (operand shapes and legs) with at least 1e8 flops times each candidate
+ * launch shape of the gather-GEMM (64 or 128 tiles, split-K counts) twice on
+ * the context's stream, synchronising, and keeps the fastest for that plan;
+ * later contractions of the plan launch it directly.  Results change only at
+ * rounding (split-K sums in another order).  Off by default.  No
+ * counterpart in the reference. */
+int tenes_ctx_set_autotune(tenes_ctx *ctx, int on);
 /* Truncated SVDs (tenes_svd_top and the projectors) since the last reset and
  * the Jacobi rounds they took (a round orthogonalises p/2 disjoint column
  * pairs of the p x p projected matrix; a sweep is p - 1 rounds). */

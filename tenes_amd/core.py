@@ -51,6 +51,10 @@ class Context:
         """Side streams a CTM move may overlap its rows on (1: none)."""
         self.check(self.lib.tenes_ctx_set_max_lanes(self.h, int(n)))
 
+    def set_autotune(self, on: bool):
+        """GEMM autotuning (tenes_ctx_set_autotune)."""
+        self.check(self.lib.tenes_ctx_set_autotune(self.h, 1 if on else 0))
+
     def set_fp32_contractions(self, on: bool):
         """Contractions from now on run on the FP32 MFMA (FP64 data)."""
         self.check(self.lib.tenes_ctx_set_fp32_contractions(self.h, 1 if on else 0))

@@ -238,7 +238,7 @@ double bond_cluster_value(tenes_env *env, const Cluster &c,
 
 extern "C" {
 
-int tenes_abi_version(void) { return 203; }
+int tenes_abi_version(void) { return 204; }
 
 int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
   return guard(nullptr, [&] {
@@ -290,6 +290,11 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx) {
 int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on) {
   if (!ctx) return TENES_ERR_ARGUMENT;
   return guard(ctx, [&] { ctx->c->set_fp32_contractions(on != 0); });
+}
+
+int tenes_ctx_set_autotune(tenes_ctx *ctx, int on) {
+  if (!ctx) return TENES_ERR_ARGUMENT;
+  return guard(ctx, [&] { ctx->c->set_autotune(on != 0); });
 }
 
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {

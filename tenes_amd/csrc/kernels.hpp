@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace tn {
 namespace kern {
@@ -40,11 +41,20 @@ struct GemmArgs {
   bool fp32 = false;
 };
 
+// A GEMM launch shape: square output tile (64 or 128) and the number of K
+// splits; bm = 0 selects the built-in heuristic.
+struct GemmPlan {
+  int bm = 0;
+  int splits = 0;
+};
+// Launch shapes worth timing for (M, N, K), the heuristic's first.
+std::vector<GemmPlan> gemm_candidates(int M, int N, int K);
 // Scratch needed for split-K on the given shape (bytes, 0 if none).
-size_t gemm_workspace_bytes(int M, int N, int K);
+size_t gemm_workspace_bytes(int M, int N, int K,
+                            const GemmPlan *plan = nullptr);
 // Returns the name of the kernel instantiation launched (static string).
 const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
-                     hipStream_t s);
+                     hipStream_t s, const GemmPlan *plan = nullptr);
 
 // ---------------------------------------------------------------------------
 // Elementwise / reduction helpers

@@ -338,20 +338,57 @@ const char *launch_cfg(const KArgs &ka, dim3 grid, hipStream_t s) {
   return launch_k<CT, BM, BN, false, false>(ka, grid, s);
 }
 
+// The heuristic plan, or the given launch shape with its K range rounded
+// to whole tiles (at most one split per K tile).
+Plan plan_for(int M, int N, int K, const GemmPlan *gp) {
+  if (gp == nullptr || gp->bm == 0 || K <= 0) return choose(M, N, K);
+  Plan p;
+  p.BM = p.BN = gp->bm == 128 ? 128 : 64;
+  const int maxs = std::max(1, ceil_div(K, BK));
+  p.splits = std::min(std::max(gp->splits, 1), maxs);
+  p.kps = ceil_div(ceil_div(K, p.splits), BK) * BK;
+  p.splits = ceil_div(K, p.kps);
+  return p;
+}
+
 }  // namespace
 
-size_t gemm_workspace_bytes(int M, int N, int K) {
-  Plan p = choose(M, N, K);
+std::vector<GemmPlan> gemm_candidates(int M, int N, int K) {
+  std::vector<GemmPlan> out;
+  const Plan h = choose(M, N, K);
+  auto add = [&](int bm, int splits) {
+    if (bm == 128 && (M < 128 || N < 128)) return;
+    GemmPlan g{bm, splits};
+    const Plan r = plan_for(M, N, K, &g);
+    for (const auto &o : out) {
+      const Plan e = plan_for(M, N, K, &o);
+      if (e.BM == r.BM && e.splits == r.splits) return;
+    }
+    out.push_back(g);
+  };
+  add(h.BM, h.splits);
+  add(64, 1);
+  add(128, 1);
+  if (K >= 8 * BK) {
+    add(64, std::max(2, h.splits * 2));
+    add(64, std::max(2, h.splits / 2));
+    add(128, std::max(2, h.splits));
+  }
+  return out;
+}
+
+size_t gemm_workspace_bytes(int M, int N, int K, const GemmPlan *plan) {
+  Plan p = plan_for(M, N, K, plan);
   if (p.splits <= 1) return 0;
   return (size_t)p.splits * M * N * sizeof(double);
 }
 
 const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
-                     hipStream_t s) {
+                     hipStream_t s, const GemmPlan *plan) {
   TN_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, "gemm: negative size");
   if (g.M == 0 || g.N == 0) return "gemm_f64_empty";
   // K == 0 gives C = beta*C through a launch with an empty K range.
-  Plan p = choose(g.M, g.N, g.K);
+  Plan p = plan_for(g.M, g.N, g.K, plan);
   KArgs ka;
   ka.g = g;
   ka.partial = nullptr;

@@ -578,20 +578,49 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
   g.b_k_stride = p.b_k_stride;
   g.b_n_stride = p.b_n_stride;
   g.fp32 = ctx.fp32_contractions();
-  const size_t ws = kern::gemm_workspace_bytes(p.M, p.N, p.K);
-  void *w = ws ? ctx.workspace(ws) : nullptr;
   const double flops = 2.0 * p.M * (double)p.N * p.K;
+  // autotuning (C = A B overwrites C, so every candidate leaves a valid
+  // result): time each launch shape twice on this stream, keep the fastest
+  if (ctx.autotune() && beta == 0.0 && p.tuned_bm == 0 &&
+      flops >= Context::kAutotuneFlops) {
+    hipEvent_t e0, e1;
+    TN_HIP_CHECK(hipEventCreate(&e0));
+    TN_HIP_CHECK(hipEventCreate(&e1));
+    float best = 0.0f;
+    for (const kern::GemmPlan &c : kern::gemm_candidates(p.M, p.N, p.K)) {
+      const size_t cws = kern::gemm_workspace_bytes(p.M, p.N, p.K, &c);
+      void *cw = cws ? ctx.workspace(cws) : nullptr;
+      TN_HIP_CHECK(hipEventRecord(e0, ctx.stream()));
+      kern::gemm_f64(g, cw, cws, ctx.stream(), &c);
+      kern::gemm_f64(g, cw, cws, ctx.stream(), &c);
+      TN_HIP_CHECK(hipEventRecord(e1, ctx.stream()));
+      TN_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      TN_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (p.tuned_bm == 0 || ms < best) {
+        best = ms;
+        p.tuned_bm = c.bm;
+        p.tuned_splits = c.splits;
+      }
+    }
+    TN_HIP_CHECK(hipEventDestroy(e0));
+    TN_HIP_CHECK(hipEventDestroy(e1));
+  }
+  kern::GemmPlan tuned{p.tuned_bm, p.tuned_splits};
+  const kern::GemmPlan *plan = p.tuned_bm ? &tuned : nullptr;
+  const size_t ws = kern::gemm_workspace_bytes(p.M, p.N, p.K, plan);
+  void *w = ws ? ctx.workspace(ws) : nullptr;
   if (ctx.profiling()) {
     hipEvent_t a;
     ctx.profile_begin(&a);
-    const char *name = kern::gemm_f64(g, w, ws, ctx.stream());
+    const char *name = kern::gemm_f64(g, w, ws, ctx.stream(), plan);
     // one report line per instantiation and problem shape
     const std::string key = std::string(name) + "[" + std::to_string(p.M) +
                             "x" + std::to_string(p.N) + "x" +
                             std::to_string(p.K) + "]";
     ctx.profile_end(a, key.c_str(), flops);
   } else {
-    kern::gemm_f64(g, w, ws, ctx.stream());
+    kern::gemm_f64(g, w, ws, ctx.stream(), plan);
   }
   ctx.gemm_flops += flops;
   ctx.gemm_calls++;

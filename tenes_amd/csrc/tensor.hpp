@@ -92,6 +92,8 @@ struct ContractPlan {
   int a_m_off, a_k_off, b_k_off, b_n_off, c_m_off, c_n_off;
   bool a_mfast = false, b_nfast = false;
   int a_m_stride = 0, a_k_stride = 0, b_k_stride = 0, b_n_stride = 0;
+  // GEMM launch shape picked by the autotuner (bm = 0: not tuned yet)
+  mutable int tuned_bm = 0, tuned_splits = 0;
 };
 
 class Context {
@@ -179,6 +181,13 @@ class Context {
   // physical legs; edges (chi, chi, D, 1)).
   void set_single_layer(bool on) { single_layer_ = on; }
   bool single_layer() const { return single_layer_; }
+  // GEMM autotuning: while on, the first contraction of every plan (shapes
+  // and legs) with a GEMM of at least kAutotuneFlops times each candidate
+  // launch shape (tile size, K splits) on the stream and keeps the fastest
+  // for that plan; later calls use it.
+  void set_autotune(bool on) { autotune_ = on; }
+  bool autotune() const { return autotune_; }
+  static constexpr double kAutotuneFlops = 1e8;
   void profile_begin(hipEvent_t *a);
   void profile_end(hipEvent_t a, const char *name, double flops);
   std::map<std::string, KernelStat> kernel_stats();  // flushes pending
@@ -205,6 +214,7 @@ class Context {
   uint64_t rng_counter_ = 0;
   bool fp32_ = false;
   bool single_layer_ = false;
+  bool autotune_ = false;
   bool profiling_ = false;
   struct Pending {
     hipEvent_t a, b;

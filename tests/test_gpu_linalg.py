@@ -134,3 +134,59 @@ def test_svd_rank_deficient(gpu_ctx):
     s_ref = np.linalg.svd(A, compute_uv=False)[:20]
     assert np.max(np.abs(s[:5] - s_ref[:5])) / s_ref[0] < 1e-12
     assert np.all(s[5:] < 1e-12 * s[0])
+
+
+def test_gemm_autotune():
+    """tenes_ctx_set_autotune: the first contraction of a plan times every
+    candidate launch shape and keeps the fastest; the tuning call and the
+    tuned calls give the product to FP64 rounding (1e-12 relative), for a
+    square GEMM, a long-K one (split-K candidates) and a permuted one."""
+    import tenes_amd as ta
+
+    ctx = ta.Context(0, seed=5)
+    ctx.set_autotune(True)
+    rng = np.random.default_rng(9)
+    cases = [((600, 520), (520, 610), [1], [0]),
+             ((96, 4000), (4000, 80), [1], [0]),
+             ((40, 30, 50), (50, 30, 70), [1, 2], [1, 0])]
+    for sa, sb, axa, axb in cases:
+        A = rng.standard_normal(sa)
+        B = rng.standard_normal(sb)
+        ref = np.tensordot(A, B, (axa, axb))
+        ta_, tb_ = ctx.tensor(A), ctx.tensor(B)
+        for _ in range(3):
+            C = ta.tensordot(ta_, tb_, axa, axb).numpy()
+            assert np.max(np.abs(C - ref)) < 1e-12 * np.max(np.abs(ref)), (sa, sb)
+    ctx.close()
+
+
+def test_autotuned_ctm_matches_oracle():
+    """A CTM run with autotuning on (fresh context): energy and Sz against the
+    oracle to 1e-10, as without it."""
+    import tenes_amd as ta
+    from oracle import ctm as O, itps, models
+    from oracle.lattice import SquareLattice
+    from ctm_cases import su_tensors
+
+    Tn, ham = su_tensors("heisenberg", 3, 60)
+    Tn = list(Tn)
+    lat = SquareLattice(2, 2)
+    p = O.PEPSParameters(CHI=20, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0)
+    env, _, _ = O.Calc_CTM_Environment(Tn, p, lat)
+    ctx = ta.Context(0, seed=6)
+    ctx.set_autotune(True)
+    genv = ta.Environment(ctx, 2, 2)
+    genv.set_site_tensors(Tn)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=20, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0))
+    E = 0.0
+    gE = 0.0
+    for s, leg in models.square_bonds(lat):
+        dx, dy = (1, 0) if leg == 2 else (0, 1)
+        gE += genv.contract_two_sites_op12(s, dx, dy, ham) / genv.contract_cluster(s, dx, dy)
+    two = [{"name": "b", "group": 0, "source": s, "dx": 1 if leg == 2 else 0, "dy": 0 if leg == 2 else 1,
+            "op": ham} for s, leg in models.square_bonds(lat)]
+    w, _ = itps.measure_twosite(env, Tn, lat, two, [])
+    E = np.real(sum(w[0].values()))
+    assert abs(gE - E) < 1e-10 * abs(E), (gE, E)
+    del genv
+    ctx.close()
