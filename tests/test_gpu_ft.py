@@ -95,6 +95,17 @@ def test_gpu_single_layer_ctm_vs_oracle(gpu_ctx):
     mine = {(x[0], x[1]): x[2] for x in rec["correlation_length"]}
     for d, f, x, _, _ in xi:
         assert abs(mine[(d, f)] - x) < 1e-9 * abs(x), (d, f, mine[(d, f)], x)
+    # multisite products of the density network (Contract_density_CTM,
+    # contract_density_ctm/ctm.cpp:45-85): a 2x2 plaquette and a 3x3 cluster
+    ms = [{"group": 0, "source": 0, "dx": [1, 1, 0], "dy": [0, 1, 1], "ops": [0, 1, 0, 1], "coeff": 1.0},
+          {"group": 1, "source": 1, "dx": [2, -1], "dy": [1, -1], "ops": [0, 0, 1], "coeff": 1.0}]
+    ref = itps.measure_multisite(env, Tn, lat, case["site_ops"], ms, K=KD)
+    for m in ms:
+        ops = [case["site_ops"][(lat.other(m["source"], *(([0] + m["dx"])[n], ([0] + m["dy"])[n])), g)]
+               for n, g in enumerate(m["ops"])]
+        got = genv.contract_multisite(m["source"], m["dx"], m["dy"], [np.real(o) for o in ops])
+        key = (m["group"], m["source"], *[v for pair in zip(m["dx"], m["dy"]) for v in pair])
+        assert abs(got - ref[key].real) < 1e-10, (key, got, ref[key])
 
 
 def test_gpu_single_layer_updown_projectors_refused(gpu_ctx):
