@@ -744,6 +744,7 @@ int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
     TN_REQUIRE(std::abs(dx) + std::abs(dy) == 1,
                "op12 contraction needs a nearest-neighbour bond");
     TN_REQUIRE(op12 && out, "null pointer");
+    TN_REQUIRE(source >= 0 && source < env->lat.N_UNIT, "bad source site");
     auto &a = env->arr;
     Cluster c = cluster_of(env->lat, source, dx, dy);
     if (is_tpo(env)) {
@@ -1010,6 +1011,7 @@ int tenes_env_contract_cluster(tenes_env *env, int source, int dx, int dy,
   return guard(env->ctx, [&] {
     check_env(env);
     TN_REQUIRE(out != nullptr, "null output");
+    TN_REQUIRE(source >= 0 && source < env->lat.N_UNIT, "bad source site");
     auto &a = env->arr;
     Cluster c = cluster_of(env->lat, source, dx, dy);
     tn::Tensor os, ot;

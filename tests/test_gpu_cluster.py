@@ -80,3 +80,13 @@ def test_cluster_beyond_4x4_is_an_error(case):
     lat, env, Tn, genv, site_ops = case
     with pytest.raises(ta.TenesError, match="not implemented"):
         genv.contract_cluster(0, 4, 0, site_ops[0], site_ops[1])
+
+
+def test_bad_source_site_is_an_error(case):
+    lat, env, Tn, genv, site_ops = case
+    with pytest.raises(ta.TenesError, match="bad source site"):
+        genv.contract_cluster(4, 1, 0)
+    with pytest.raises(ta.TenesError, match="bad source site"):
+        genv.contract_two_sites_op12(-1, 1, 0, np.eye(4).reshape(2, 2, 2, 2))
+    with pytest.raises(ta.TenesError, match="bad multisite operator"):
+        genv.contract_multisite(7, [1], [0], [np.eye(2), np.eye(2)])
