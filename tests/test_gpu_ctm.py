@@ -643,3 +643,35 @@ def test_tfi_plain_reference_config(gpu_ctx):
     o, _ = itps.measure_onesite(env, Tn, lat, one)
     gmx = [genv.contract_one_site(i, models.Sx) / genv.contract_one_site(i) for i in range(4)]
     assert np.max(np.abs(np.array(gmx) - np.real([o[0][i] for i in range(4)]))) < 1e-10
+
+
+def test_paired_move_chains_equal_reference_order(gpu_ctx):
+    """Calc_CTM_Environment runs the left and right (then top and bottom)
+    move chains side by side; the same sweeps issued one move at a time in
+    the reference's order (ctm.cpp:1203-1228: left 0, 1; right 1, 0; top 1,
+    0; bottom 0, 1) give the same environment: corner spectra and energy to
+    1e-12 (D=2, chi=16: every projector matrix takes the exact Jacobi SVD)."""
+    Tn, ham = su_tensors("heisenberg", 2, 60)
+    Tn = list(Tn)
+    lat = SquareLattice(2, 2)
+    kw = dict(CHI=16, CTM_Convergence_Epsilon=0.0)
+    a = ta.Environment(gpu_ctx, 2, 2)
+    a.set_site_tensors(Tn)
+    a.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=3, **kw))
+    b = ta.Environment(gpu_ctx, 2, 2)
+    b.set_site_tensors(Tn)
+    n, _ = b.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=0, **kw))
+    assert n == 0
+    p = ta.peps_parameters(**kw)
+    for _ in range(3):
+        for d, coords in ((0, (0, 1)), (1, (1, 0)), (2, (1, 0)), (3, (0, 1))):
+            for c in coords:
+                b.move(d, c, p)
+    for i in range(4):
+        for k in ("C1", "C2", "C3", "C4"):
+            sa = O.normalized_spectrum(a.get(k, i).numpy())
+            sb = O.normalized_spectrum(b.get(k, i).numpy())
+            assert sa.shape == sb.shape and np.max(np.abs(sa - sb)) < 1e-12, (k, i)
+    Ea, _ = observables_gpu(a, lat, ham)
+    Eb, _ = observables_gpu(b, lat, ham)
+    assert abs(Ea - Eb) < 1e-12 * abs(Ea)
