@@ -1,9 +1,11 @@
 """Whole CTM runs on unit cells other than 2x2 (Calc_CTM_Environment,
 src/iTPS/core/ctm.cpp:971-1253, with the lattice of src/SquareLattice.hpp):
 1x1, 3x2, 2x3, 3x3 and skewed 3x2 / 2x1 cells, where the left/right and
-top/bottom move chains pair different columns and rows than on the 2x2 cell.
-Seeded random site tensors (D=2, unit norm) on the GPU and in the oracle, 4
-sweeps at chi=8: L2-normalised corner spectra of every site to 1e-9 and
+top/bottom move chains pair different columns and rows than on the 2x2 cell,
+with corner projectors and with the up-down projectors
+(Calc_projector_updown_blocks, ctm.cpp:320-491; moves in sequence).  Seeded
+random site tensors (D=2, unit norm) on the GPU and in the oracle, 4 sweeps
+at chi=8: L2-normalised corner spectra of every site to 1e-9 and
 nearest-neighbour Sz Sz bond values to 1e-9 relative."""
 import numpy as np
 import pytest
@@ -19,16 +21,19 @@ pytestmark = pytest.mark.gpu
 CELLS = [(1, 1, 0), (3, 2, 0), (2, 3, 0), (3, 3, 0), (3, 2, 1), (2, 1, 1)]
 
 
+@pytest.mark.parametrize("corner", [1, 0], ids=["corner", "updown"])
 @pytest.mark.parametrize("LX,LY,skew", CELLS)
-def test_ctm_on_unit_cell(gpu_ctx, LX, LY, skew):
+def test_ctm_on_unit_cell(gpu_ctx, LX, LY, skew, corner):
     lat = SquareLattice(LX, LY, skew)
     Tn = random_tensors(2, 2, seed=100 + 10 * LX + LY + skew, n=lat.N_UNIT)
-    p = O.PEPSParameters(CHI=8, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0)
+    p = O.PEPSParameters(CHI=8, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0,
+                         CTM_Projector_corner=bool(corner))
     env, count, _ = O.Calc_CTM_Environment(list(Tn), p, lat)
     genv = ta.Environment(gpu_ctx, LX, LY, skew)
     genv.set_site_tensors(Tn)
     gcount, _ = genv.calc_ctm_environment(
-        ta.peps_parameters(CHI=8, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0))
+        ta.peps_parameters(CHI=8, Max_CTM_Iteration=4, CTM_Convergence_Epsilon=0.0,
+                           CTM_Projector_corner=corner))
     assert gcount == count
     for i in range(lat.N_UNIT):
         for k in ("C1", "C2", "C3", "C4"):
