@@ -13,17 +13,26 @@ the first move); then `--steps` sweeps are timed, each = 2 left + 2 right +
 2500 x 2500 matrix, 16 Calc_Next_CTM, 32 Calc_Next_eT) + the convergence
 check (epsilon = 0: no early exit).
 
-Reported: sweeps/s; the dominant kernel (the GEMM instantiation and shape
-carrying the most flops: R = LT*LB, 2500^3) with its achieved FP64 MFMA
-TFLOP/s from HIP events recorded around every launch (roofline): alone on
-the device in one extra sweep with the projector chains serialised (the
-figure rocprofv3's kernel trace reports), and inside the timed region, where
-four chains share the device (timed_region_*); its HBM traffic per launch from the committed rocprofv3 PMC
-summary (profiles/); the relative energy error against the oracle's FP64
-(LAPACK SVD) run of the same problem and sweep count, from a separate untimed
-run; the CPU baseline = the numpy oracle (same contraction order, LAPACK
-SVD) timing one Left_move on the same environment on this host, scaled to
-sweeps/s (a sweep = 8 moves of equal cost).
+Passes, in order:
+  1. timed region (the `value`): no per-launch instrumentation;
+  2. the same number of sweeps again with HIP events around every GEMM
+     launch (recorded on the launch's own stream): per-kernel report on
+     stderr and the `timed_region_*` roofline fields;
+  3. one sweep with the projector chains serialised on one stream: the
+     dominant kernel (the GEMM instantiation and shape carrying the most
+     flops, R = LT*LB, 2500^3) runs alone on the device, as under rocprofv3's
+     kernel trace; its average launch time gives `roofline.achieved`;
+  4. accuracy: a fresh environment run for the oracle's sweep count, energy,
+     staggered magnetisation and the correlation lengths against the
+     oracle's FP64 (LAPACK SVD) run of the same problem;
+  5. mixed precision (contractions on the FP32 MFMA, modes 1 and 2): the same
+     timed region and accuracy check;
+  6. the CPU baseline: the numpy oracle (same contraction order, LAPACK SVD)
+     running one full sweep (8 moves + convergence check) on the GPU's
+     environment on this host's cores.
+
+    python bench.py [--steps K --warmup W]     # the driver's command
+    python bench.py --probe-gemm               # workload for rocprofv3 --pmc
 """
 from __future__ import annotations
 
@@ -50,6 +59,10 @@ METRIC = "CTMRG sweeps/sec at D=5 chi=100 (MFMA TFLOP/s) + rel energy err vs FP6
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# algorithmic flop of one sweep at D=5, chi=100 in the reference's
+# contraction order (projectors with the full R = LT*LB, 16 Calc_Next_CTM,
+# 32 Calc_Next_eT), 2 flop per multiply-add
+SWEEP_FLOP_D5_CHI100 = 7.448e11
 
 
 def log(msg):
@@ -128,32 +141,115 @@ def energy(env, H):
     return E / 4.0
 
 
-def cpu_baseline(env, chi, n_moves):
-    """Oracle (numpy/LAPACK, FP64) Left_move on the GPU's current
-    environment; returns (sweeps/s, seconds, threads)."""
+def host_cpu():
+    """(model name, logical CPUs available to this process, physical cores)."""
+    model, phys = "unknown", set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if "core id" in cur:
+                    phys.add((cur.get("physical id", "0"), cur["core id"]))
+                cur = {}
+                continue
+            k, v = [x.strip() for x in line.split(":", 1)]
+            cur[k] = v
+            if k == "model name" and model == "unknown":
+                model = v
+        if "core id" in cur:
+            phys.add((cur.get("physical id", "0"), cur["core id"]))
+    except OSError:
+        pass
+    logical = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, logical, len(phys) or None
+
+
+def blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+
+        return max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(env, chi):
+    """One full sweep of the oracle (numpy/LAPACK, FP64: 2 left, 2 right, 2
+    top, 2 bottom moves with 16 LAPACK SVDs of 2500 x 2500, then the
+    convergence check) on the GPU's current environment; returns
+    (sweeps/s, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import ctm as O
     from oracle.lattice import SquareLattice
 
-    try:
-        from threadpoolctl import threadpool_info
-
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:  # pragma: no cover
-        threads = os.cpu_count() or 1
     lat = SquareLattice(2, 2)
     oenv = O.Env(4)
     for k in ("C1", "C2", "C3", "C4", "eTt", "eTr", "eTb", "eTl"):
         for i in range(4):
             getattr(oenv, k)[i] = env.get(k, i).numpy()
     Tn = [env.get("Tn", i).numpy() for i in range(4)]
-    p = O.PEPSParameters(CHI=chi)
+    p = O.PEPSParameters(CHI=chi, CTM_Convergence_Epsilon=0.0)
+    old = {n: list(getattr(oenv, n)) for n in ("C1", "C2", "C3", "C4")}
     t0 = time.perf_counter()
-    for m in range(n_moves):
-        O.Left_move(oenv, Tn, m % 2, p, lat)
+    O.sweep(oenv, Tn, p, lat)
+    O.Check_Convergence_CTM(oenv, old, p, lat)
     dt = time.perf_counter() - t0
-    per_move = dt / n_moves
-    return 1.0 / (8.0 * per_move), dt, threads
+    return 1.0 / dt, dt
+
+
+def accuracy(ta, ctx, Tn, H, fx, kw, chi, tag):
+    """Fresh environment, the oracle's sweep count: relative errors of the
+    energy, the staggered magnetisation and the worst correlation length
+    (10 leading transfer-matrix eigenvalues) against the oracle's FP64 run."""
+    chk = ta.Environment(ctx, 2, 2)
+    chk.set_site_tensors(Tn)
+    chk.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=fx["sweeps"], **kw), initialize=True)
+    E = energy(chk, H)
+    e_err = abs(E - fx["energy_per_site"]) / abs(fx["energy_per_site"])
+    ms = staggered_magnetization(chk)
+    ms_err = abs(ms - fx["staggered_magnetization"]) / fx["staggered_magnetization"]
+    xi_err = 0.0
+    for c in fx["correlation"]:
+        ev = chk.transfer_matrix_eigenvalues(c["dir"], c["fixed"], ta.tm_parameters(num_eigvals=10))
+        xi = -2.0 / np.log(abs(ev[1]) / abs(ev[0]))
+        xi_err = max(xi_err, abs(xi - c["xi"]) / c["xi"])
+    log(f"{tag}: after {fx['sweeps']} sweeps E = {E:.15f} (oracle {fx['energy_per_site']:.15f}), "
+        f"rel err energy {e_err:.2e}, m_s {ms_err:.2e}, worst xi {xi_err:.2e} "
+        f"(10 leading eigenvalues, chi^2 = {chi ** 2})")
+    return e_err, ms_err, xi_err
+
+
+def timed_sweeps(ctx, env, ta, kw, steps):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    n, _ = env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=steps, **kw), initialize=False)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    assert n == steps
+    return dt
+
+
+def probe_gemm():
+    """Workload for counter collection (rocprofv3 --pmc): ten launches of the
+    dominant CTMRG GEMM shape (R = LT*LB at D=5, chi=100: 2500 x 2500 x 2500
+    FP64, row-major operands) and five launches of a plain copy kernel of
+    known size (4096 x 4096 FP64 transpose: 128 MiB read, 128 MiB written)
+    whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    import tenes_amd as ta
+
+    ctx = ta.Context(0)
+    rng = np.random.default_rng(0)
+    n = 2500
+    A = ctx.tensor(rng.standard_normal((n, n)))
+    B = ctx.tensor(rng.standard_normal((n, n)))
+    for _ in range(10):
+        ta.tensordot(A, B, [1], [0])
+    ctx.synchronize()
+    X = ctx.tensor(rng.standard_normal((4096, 4096)))
+    for _ in range(5):
+        ta.transpose(X, [1, 0])
+    ctx.synchronize()
+    print("probe: 10 GEMM 2500^3 + 5 copies of 128 MiB done", flush=True)
 
 
 def main():
@@ -164,18 +260,20 @@ def main():
     ap.add_argument("--D", type=int, default=5)
     ap.add_argument("--chi", type=int, default=100)
     ap.add_argument("--su-steps", type=int, default=500)
-    ap.add_argument("--cpu-moves", type=int, default=1, help="oracle moves timed for the CPU baseline")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true",
-                    help="skip the untimed accuracy run against the oracle fixture")
+                    help="skip the untimed accuracy runs against the oracle fixture")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
     ap.add_argument("--power-iters", type=int, default=None)
     ap.add_argument("--autotune", type=int, default=0, choices=(0, 1),
                     help="GEMM autotuning of the context (tuned during the warm-up sweeps)")
     ap.add_argument("--warm-iters", type=int, default=None,
                     help="range-finder iterations of a warm-started projector SVD (library default 8)")
     ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
-                    help="contractions of the moves on the FP32 MFMA: 1 all but the projector matrix "
-                         "R = LT LB, 2 R as well (projector SVDs and the rest FP64)")
+                    help="headline in mixed precision: contractions of the moves on the FP32 MFMA, 1 all but "
+                         "the projector matrix R = LT LB, 2 R as well (projector SVDs and the rest FP64)")
+    ap.add_argument("--probe-gemm", action="store_true",
+                    help="run only the GEMM + copy workload for rocprofv3 counter passes")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -183,6 +281,9 @@ def main():
         print(f"bench.py measures one GPU only (the CTMRG hot path of one problem runs on a single "
               f"MI355X here); --gpus {args.gpus} / WORLD_SIZE {world} is not supported.", flush=True)
         sys.exit(2)
+    if args.probe_gemm:
+        probe_gemm()
+        return
 
     import tenes_amd as ta
 
@@ -196,89 +297,93 @@ def main():
         kw["svd_power_iters"] = args.power_iters
     if args.warm_iters is not None:
         kw["svd_warm_power_iters"] = args.warm_iters
-    if args.mixed:
-        kw["contraction_fp32"] = args.mixed
+    kw_head = dict(kw, contraction_fp32=args.mixed) if args.mixed else kw
     t0 = time.perf_counter()
-    env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.warmup, **kw), initialize=True)
+    env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.warmup, **kw_head), initialize=True)
     ctx.synchronize()
     log(f"warmup: {args.warmup} sweeps in {time.perf_counter() - t0:.2f} s")
 
+    # 1. timed region, no instrumentation
     ctx.reset_stats()
-    ctx.reset_kernel_report()
-    ctx.set_profiling(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    n, _ = env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.steps, **kw), initialize=False)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    assert n == args.steps
+    dt = timed_sweeps(ctx, env, ta, kw_head, args.steps)
     flops, ncalls = ctx.stats()
     nsvd, nrounds = ctx.svd_stats()
-    rep = ctx.kernel_report()
     sweeps_per_s = args.steps / dt
-    log(f"timed: {args.steps} sweeps in {dt:.3f} s -> {sweeps_per_s:.3f} sweeps/s; "
-        f"{ncalls} GEMMs, {flops / dt / 1e12:.2f} TFLOP/s overall; {nsvd} truncated SVDs, "
+    log(f"timed: {args.steps} sweeps in {dt:.3f} s -> {sweeps_per_s:.3f} sweeps/s "
+        f"({SWEEP_FLOP_D5_CHI100 * sweeps_per_s / 1e12:.2f} TFLOP/s algorithmic); {ncalls} GEMMs "
+        f"({flops / dt / 1e12:.2f} TFLOP/s issued); {nsvd} truncated SVDs, "
         f"{nrounds / max(nsvd, 1):.0f} Jacobi rounds each")
-    # dominant kernel: the GEMM (instantiation and shape) carrying the most
-    # algorithmic work of the timed region
+
+    # 2. the same sweeps with per-launch events (GEMM report)
+    ctx.reset_kernel_report()
+    ctx.set_profiling(True)
+    dt_ins = timed_sweeps(ctx, env, ta, kw_head, args.steps)
+    ctx.set_profiling(False)
+    rep = ctx.kernel_report()
     dom_name, (dom_n, dom_ms, dom_fl) = max(rep.items(), key=lambda kv: kv[1][2])
-    gemm_ms = sum(v[1] for v in rep.values())
+    log(f"instrumented pass: {args.steps} sweeps in {dt_ins:.3f} s; GEMMs by device time:")
     for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])[:12]:
         log(f"  {k}: {v[0]} launches, {v[1]:.2f} ms, {v[2] / (v[1] * 1e-3) / 1e12 if v[1] else 0:.2f} TFLOP/s")
-    achieved = dom_fl / (dom_ms * 1e-3) / 1e12
+    timed_achieved = dom_fl / (dom_ms * 1e-3) / 1e12
     E = energy(env, H)
-    log(f"energy per site (FP64): {E:.15f}; GEMM share of timed region {gemm_ms / (dt * 1e3):.1%}")
-    # one more sweep with the projector lanes serialised (one stream): the
-    # dominant GEMM then runs alone on the device, as under rocprofv3's kernel
-    # trace, whose average launch time this one matches
+
+    # 3. one sweep with the projector chains serialised: the dominant GEMM
+    # alone on the device, as under rocprofv3's kernel trace
     ctx.set_max_lanes(1)
     ctx.reset_kernel_report()
     ctx.set_profiling(True)
-    env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=1, **kw), initialize=False)
+    env.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=1, **kw_head), initialize=False)
     ctx.synchronize()
     ctx.set_profiling(False)
     ctx.set_max_lanes(8)
     ser_n, ser_ms, ser_fl = ctx.kernel_report().get(dom_name, (0, 0.0, 0.0))
     ser_achieved = ser_fl / (ser_ms * 1e-3) / 1e12 if ser_ms else None
-    log(f"dominant GEMM {dom_name}: {dom_ms / dom_n:.3f} ms per launch in the timed region (four lanes), "
-        f"{ser_ms / max(ser_n, 1):.3f} ms serialised -> {ser_achieved or 0:.2f} TFLOP/s")
-    # accuracy against the oracle's FP64 (LAPACK SVD) run of the same problem:
-    # a fresh environment, the oracle's sweep count, untimed
+    log(f"dominant GEMM {dom_name}: {ser_ms / max(ser_n, 1):.3f} ms per launch alone on the device "
+        f"-> {ser_achieved or 0:.2f} TFLOP/s; {dom_ms / dom_n:.3f} ms between its events inside the "
+        f"overlapped sweeps")
+
+    # 4. accuracy of the headline mode
     e_err = ms_err = xi_err = None
     if fx is not None and not args.no_check:
-        chk = ta.Environment(ctx, 2, 2)
-        chk.set_site_tensors(Tn)
-        chk.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=fx["sweeps"], **kw), initialize=True)
-        E_chk = energy(chk, H)
-        e_err = abs(E_chk - fx["energy_per_site"]) / abs(fx["energy_per_site"])
-        log(f"energy after {fx['sweeps']} sweeps: GPU {E_chk:.15f}, oracle {fx['energy_per_site']:.15f}, "
-            f"rel err {e_err:.2e}")
-        ms = staggered_magnetization(chk)
-        ms_err = abs(ms - fx["staggered_magnetization"]) / fx["staggered_magnetization"]
-        # every correlation length of the fixture (both directions, both
-        # rows/columns); the worst relative error is reported
-        xi_err = 0.0
-        for c in fx["correlation"]:
-            ev = chk.transfer_matrix_eigenvalues(c["dir"], c["fixed"], ta.tm_parameters(num_eigvals=10))
-            xi = -2.0 / np.log(abs(ev[1]) / abs(ev[0]))
-            err = abs(xi - c["xi"]) / c["xi"]
-            xi_err = max(xi_err, err)
-            log(f"correlation length dir {c['dir']} row/col {c['fixed']}: {xi:.12f} (rel err {err:.2e}, "
-                f"10 leading eigenvalues, chi^2 = {args.chi ** 2})")
-        log(f"staggered magnetisation {ms:.12f} (rel err {ms_err:.2e})")
+        e_err, ms_err, xi_err = accuracy(ta, ctx, Tn, H, fx, kw_head, args.chi,
+                                         f"accuracy (mode {args.mixed})" if args.mixed else "accuracy (FP64)")
+
+    # 5. mixed precision, measured in the same run
+    mixed = []
+    if not args.no_mixed and not args.mixed:
+        for mode in (1, 2):
+            kwm = dict(kw, contraction_fp32=mode)
+            menv = ta.Environment(ctx, 2, 2)
+            menv.set_site_tensors(Tn)
+            menv.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.warmup, **kwm), initialize=True)
+            mdt = timed_sweeps(ctx, menv, ta, kwm, args.steps)
+            rec = {"mode": mode,
+                   "fp32_contractions": "all but R = LT*LB" if mode == 1 else "all, R = LT*LB included",
+                   "sweeps_per_s": args.steps / mdt, "ms_per_step": mdt / args.steps * 1e3}
+            log(f"mixed mode {mode}: {args.steps} sweeps in {mdt:.3f} s -> {args.steps / mdt:.3f} sweeps/s")
+            if fx is not None and not args.no_check:
+                a, b, c = accuracy(ta, ctx, Tn, H, fx, kwm, args.chi, f"accuracy (mixed mode {mode})")
+                rec.update(energy_rel_err_vs_fp64=a, staggered_magnetization_rel_err_vs_fp64=b,
+                           correlation_length_rel_err_vs_fp64=c)
+            mixed.append(rec)
+            del menv
+
     traffic = load_traffic(dom_name)
-    # the dominant GEMM's matrix peak: FP32 MFMA in the mixed mode
     peak = FP32_MFMA_PEAK_TFLOPS if dom_name.startswith("gemm_f32") else FP64_MFMA_PEAK_TFLOPS
 
+    # 6. CPU baseline
     cpu = None
+    model, logical, phys = host_cpu()
     if not args.no_cpu:
-        t0 = time.perf_counter()
-        cps, cdt, threads = cpu_baseline(env, args.chi, args.cpu_moves)
-        log(f"cpu baseline: {args.cpu_moves} Left_move in {cdt:.2f} s on {threads} threads -> {cps:.5f} sweeps/s")
+        cps, cdt = cpu_baseline(env, args.chi)
+        threads = blas_threads()
+        log(f"cpu baseline: 1 oracle sweep in {cdt:.2f} s on {threads} BLAS threads ({model}; "
+            f"{logical} logical CPUs, {phys} physical cores) -> {cps:.5f} sweeps/s")
         cpu = {"value": cps, "unit": "sweeps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_moves} Left_move (2 projectors with a 2500x2500 LAPACK SVD, 2 Calc_Next_CTM, "
-                         f"4 Calc_Next_eT) of the numpy oracle on the same environment; sweeps/s = 1/(8 t_move)"}
+               "sample": "1 full sweep of the numpy oracle (2 left, 2 right, 2 top, 2 bottom moves: 16 "
+                         "projectors with a 2500 x 2500 LAPACK SVD, 16 Calc_Next_CTM, 32 Calc_Next_eT; then "
+                         "the convergence check) on the GPU's environment",
+               "cpu_model": model, "logical_cpus": logical, "physical_cores": phys}
 
     out = {
         "metric": METRIC,
@@ -290,6 +395,8 @@ def main():
         "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
+        # no published number exists for this metric; the CPU
+        # ratio is reported separately below
         "vs_baseline": None,
         "dtype": (f"fp32-mfma contractions (mode {args.mixed}), f64 SVD/QR/Arnoldi" if args.mixed else "f64"),
         "data": ("synthetic: seeded simple-update Heisenberg site tensors (oracle run stored in "
@@ -299,29 +406,29 @@ def main():
                                + ("mixed precision" if args.mixed else "FP64"),
                    "D": args.D, "chi": args.chi, "unit_cell": [2, 2], "su_steps": args.su_steps,
                    "projector": "corner", "svd": "GPU truncated FP64 SVD (randomized range finder + Jacobi)"},
-        # the kernel alone on the device (the serialised sweep; what
-        # rocprofv3's kernel trace reports), then the same launches inside
-        # the timed region, where four projector chains share the device and
-        # a launch's event-to-event time includes waiting for their CUs
+        "algorithmic_tflops": SWEEP_FLOP_D5_CHI100 * sweeps_per_s / 1e12 if (args.D, args.chi) == (5, 100)
+        else None,
         "roofline": {"bound": "mfma", "kernel": dom_name,
-                     "achieved": ser_achieved if ser_achieved else achieved, "peak": peak,
+                     "achieved": ser_achieved if ser_achieved else timed_achieved, "peak": peak,
                      "unit": "TFLOP/s",
-                     "frac": (ser_achieved if ser_achieved else achieved) / peak,
+                     "frac": (ser_achieved if ser_achieved else timed_achieved) / peak,
                      "traffic": traffic,
                      "launches": ser_n if ser_n else dom_n,
                      "avg_launch_ms": ser_ms / ser_n if ser_n else dom_ms / dom_n,
-                     "measured": "HIP events around each launch, one extra sweep with the projector "
-                                 "chains serialised",
+                     "measured": "HIP events on the launch's stream around each launch, one sweep with the "
+                                 "projector chains serialised (pass 3)",
                      "timed_region_launches": dom_n,
                      "timed_region_avg_launch_ms": dom_ms / dom_n,
-                     "timed_region_achieved": achieved,
-                     "timed_region_frac": achieved / peak},
+                     "timed_region_achieved": timed_achieved,
+                     "timed_region_frac": timed_achieved / peak},
         "cpu_baseline": cpu,
+        "speedup_vs_cpu_baseline": sweeps_per_s / cpu["value"] if cpu else None,
         "energy_per_site_fp64": E,
         "energy_rel_err_vs_fp64": e_err,
         "staggered_magnetization_rel_err_vs_fp64": ms_err,
         "correlation_length_rel_err_vs_fp64": xi_err,
-        "gemm_tflops_overall": flops / dt / 1e12,
+        "mixed_precision": mixed,
+        "gemm_tflops_issued": flops / dt / 1e12,
     }
     print(json.dumps(out), flush=True)
 

@@ -62,6 +62,11 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx);
  * order on the context stream).  No counterpart in the reference, whose
  * moves loop over the rows (src/iTPS/core/ctm.cpp:577-615). */
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n);
+/* Diagnostics: *forked = 1 while the context's work is split over its side
+ * streams (inside a CTM move), *lane = the stream new work goes to (-1: the
+ * context stream).  Between library calls they are 0 and -1, also after a
+ * call that failed inside a move.  No counterpart in the reference. */
+int tenes_ctx_lane_state(tenes_ctx *ctx, int *forked, int *lane);
 /* Contractions issued from now on (tenes_tensordot and the rest) run on the
  * FP32 MFMA when on != 0 (FP64 data, FP32 operands and accumulation); the
  * CTM moves set it themselves from tenes_peps_parameters.contraction_fp32.
@@ -247,7 +252,10 @@ int tenes_env_contract_one_site(tenes_env *env, int site, const double *op,
  * (contract_itps_ctm/ctm.cpp:213-319) for the nearest-neighbour bond
  * (source, dx, dy) with |dx| + |dy| == 1, using the tensor selection of
  * twosite_obs.cpp:146-222 (operator legs swapped when source is the right
- * or bottom site). */
+ * or bottom site).  Longer bonds (|dx|, |dy| <= 3; TENES_ERR_UNSUPPORTED
+ * beyond) take twosite_obs.cpp:223-243: the operator is split by its SVD
+ * over (source legs | target legs) and each product of one-site operators
+ * is contracted on the bond's cluster (Contract_iTPS_CTM). */
 int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
                                       int dy, const double *op12,
                                       double *out);
@@ -341,12 +349,17 @@ int tenes_full_update_bond(const tenes_tensor *const *T, const double *op12,
                            tenes_tensor **Tn1_new, tenes_tensor **Tn2_new);
 /* The full-update driver, src/iTPS/full_update.cpp:154-186: a fresh CTM
  * environment, then nsteps passes over the nops two-site operators (bond
- * (source[k], source_leg[k]), operator the next (d_s d_t)^2 doubles of ops,
- * order (source, target, source', target')), each followed by the
+ * (source[k], source_leg[k]), operator k the next doubles of ops, row-major
+ * (source, target, source', target') with the dimensions op_dims[4k..4k+3];
+ * op_dims == NULL: (d_s, d_t, d_s, d_t) from the site tensors.  An operator
+ * whose output dimensions differ from its input ones (TeNeS's
+ * path-decomposed longer-range gates) changes the sites' physical legs, as
+ * Full_update_bond does), each followed by the
  * fast-full-update re-moves (full_update.cpp:114-151) or, with
  * Full_Use_FastFullUpdate = 0, a fresh environment. */
 int tenes_env_full_update(tenes_env *env, int nops, const int *source,
-                          const int *source_leg, const double *ops, int nsteps,
+                          const int *source_leg, const int *op_dims,
+                          const double *ops, int nsteps,
                           const tenes_peps_parameters *pp,
                           const tenes_fu_parameters *fp);
 
@@ -361,6 +374,9 @@ typedef struct tenes_tm_parameters {
 } tenes_tm_parameters;
 void tenes_tm_parameters_default(tenes_tm_parameters *p);
 /* TransferMatrix_ctm::eigenvalues, transfer_matrix.cpp:84-142.
+ * arnoldi_maxiter > 1 on the Arnoldi path (dimension above
+ * maxdim_dense_eigensolver and at least arnoldi_maxdim) returns
+ * TENES_ERR_UNSUPPORTED: the restarts of arnoldi.cpp:117-167 are not built.
  * eig: 2*num_eigvals doubles (re, im), sorted by |lambda| descending;
  * *n_out receives the number of eigenvalues written. */
 int tenes_env_transfer_matrix_eigenvalues(tenes_env *env, int dir,

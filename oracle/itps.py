@@ -71,9 +71,20 @@ def measure_twosite(env, Tn, lat, twosite_ops, onesite_ops, K=K):
         if key not in norms:
             norms[key] = K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
         norm = norms[key]
-        if "op" in op:
-            if nrow * ncol != 2:
-                raise NotImplementedError("explicit two-site operators only on nearest-neighbour bonds")
+        if "op" in op and nrow * ncol != 2:
+            # twosite_obs.cpp:223-243: op = sum_k s_k U_k (x) V_k from the SVD
+            # over (source legs 0, 2 | target legs 1, 3), one cluster
+            # contraction per product of one-site operators
+            o = op["op"]
+            d1, d2 = o.shape[0], o.shape[1]
+            M = o.transpose(0, 2, 1, 3).reshape(d1 * d1, d2 * d2)
+            U, s, VT = np.linalg.svd(M, full_matrices=False)
+            value = 0.0
+            for k in range(len(s)):
+                ops_[srow][scol] = U[:, k].reshape(d1, d1)
+                ops_[trow][tcol] = VT[k].reshape(d2, d2)
+                value += s[k] * K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
+        elif "op" in op:
             o = op["op"]
             if nrow == 2:
                 top, bottom = idx[0][0], idx[1][0]

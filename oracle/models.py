@@ -27,13 +27,15 @@ def bond_hamiltonian(Jz=0.0, Jx=0.0, Jy=0.0, hx=0.0, hy=0.0, hz=0.0, z=4):
 
 def evolution_twosite(ham: np.ndarray, tau: float) -> np.ndarray:
     """tenes_std.py:893-908: exp(-tau H) through eigh of H reshaped to
-    (d1 d2) x (d1 d2)."""
+    (d1 d2) x (d1 d2).  A complex tau (tau = i t) gives the real-time gate
+    exp(-i t H) of TeNeS's time-evolution mode."""
     d = ham.shape
     H = ham.reshape(d[0] * d[1], d[0] * d[1])
     w, V = np.linalg.eigh(H)
     evo = (V * np.exp(-tau * w)) @ V.conj().T
     evo = evo.reshape(d)
-    return evo.real.copy() if np.isrealobj(ham) or np.max(np.abs(evo.imag)) == 0.0 else evo
+    real = np.isrealobj(ham) and np.imag(tau) == 0.0
+    return evo.real.copy() if real or np.max(np.abs(evo.imag)) == 0.0 else evo
 
 
 def square_bonds(lat):

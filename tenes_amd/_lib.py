@@ -111,13 +111,14 @@ _SIGS = {
     "tenes_env_contract_two_sites_op12": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
     "tenes_ctx_set_fp32_contractions": (C.c_int, [C.c_void_p, C.c_int]),
+    "tenes_ctx_lane_state": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "tenes_batch_calc_ctm_environment": (
         C.c_int, [c_void_pp, C.c_int, C.POINTER(PepsParameters), C.c_int, C.c_int, c_int_p, c_dbl_p]),
     "tenes_fu_parameters_default": (None, [C.POINTER(FuParameters)]),
     "tenes_full_update_bond": (
         C.c_int, [c_void_pp, c_dbl_p, C.c_int, C.POINTER(FuParameters), c_void_pp, c_void_pp]),
     "tenes_env_full_update": (
-        C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_dbl_p, C.c_int, C.POINTER(PepsParameters),
+        C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_dbl_p, C.c_int, C.POINTER(PepsParameters),
                   C.POINTER(FuParameters)]),
     "tenes_env_contract_multisite": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p, c_int_p, c_dbl_p, c_dbl_p]),

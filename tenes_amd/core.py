@@ -51,6 +51,13 @@ class Context:
         """Side streams a CTM move may overlap its rows on (1: none)."""
         self.check(self.lib.tenes_ctx_set_max_lanes(self.h, int(n)))
 
+    def lane_state(self):
+        """(forked, lane): whether work is split over side streams and the
+        stream new work goes to (-1: the context stream)."""
+        f, l = C.c_int(), C.c_int()
+        self.check(self.lib.tenes_ctx_lane_state(self.h, C.byref(f), C.byref(l)))
+        return bool(f.value), l.value
+
     def set_autotune(self, on: bool):
         """GEMM autotuning (tenes_ctx_set_autotune)."""
         self.check(self.lib.tenes_ctx_set_autotune(self.h, 1 if on else 0))
@@ -416,15 +423,19 @@ class Environment:
 
     def full_update(self, ops, nsteps: int, ctm_params=None, fu_params=None):
         """TeNeS's full-update driver (src/iTPS/full_update.cpp:154-186):
-        ops = [(source_site, source_leg, op (d_s, d_t, d_s, d_t)), ...]."""
+        ops = [(source_site, source_leg, op), ...] with op (d_s, d_t, d_s', d_t')
+        (output dimensions may differ: path-decomposed gates)."""
         src = np.array([o[0] for o in ops], dtype=np.int32)
         leg = np.array([o[1] for o in ops], dtype=np.int32)
         flat = np.concatenate([np.ascontiguousarray(o[2], dtype=np.float64).ravel() for o in ops]) \
             if ops else np.zeros(1)
+        dims = np.array([np.shape(o[2]) for o in ops], dtype=np.int32).reshape(-1) if ops \
+            else np.zeros(4, dtype=np.int32)
         pp = ctm_params or peps_parameters()
         fp = fu_params or fu_parameters()
         self.ctx.check(self.ctx.lib.tenes_env_full_update(
-            self.h, len(ops), L.iptr(src), L.iptr(leg), L.dptr(flat), nsteps, C.byref(pp), C.byref(fp)))
+            self.h, len(ops), L.iptr(src), L.iptr(leg), L.iptr(dims), L.dptr(flat), nsteps, C.byref(pp),
+            C.byref(fp)))
 
     def contract_multisite(self, source: int, dx, dy, ops) -> float:
         """<prod O_i> / <1> of one-site operators on the source and the

@@ -238,7 +238,7 @@ double bond_cluster_value(tenes_env *env, const Cluster &c,
 
 extern "C" {
 
-int tenes_abi_version(void) { return 204; }
+int tenes_abi_version(void) { return 205; }
 
 int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
   return guard(nullptr, [&] {
@@ -301,6 +301,14 @@ int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {
   return guard(ctx, [&] {
     TN_REQUIRE(n >= 1, "max lanes must be >= 1");
     ctx->c->set_max_lanes(n);
+  });
+}
+
+int tenes_ctx_lane_state(tenes_ctx *ctx, int *forked, int *lane) {
+  if (!ctx) return TENES_ERR_ARGUMENT;
+  return guard(ctx, [&] {
+    if (forked) *forked = ctx->c->forked() ? 1 : 0;
+    if (lane) *lane = ctx->c->current_lane();
   });
 }
 
@@ -741,16 +749,20 @@ int tenes_env_contract_two_sites_op12(tenes_env *env, int source, int dx,
   if (!env) return TENES_ERR_ARGUMENT;
   return guard(env->ctx, [&] {
     check_env(env);
-    TN_REQUIRE(std::abs(dx) + std::abs(dy) == 1,
-               "op12 contraction needs a nearest-neighbour bond");
+    TN_REQUIRE(std::abs(dx) + std::abs(dy) >= 1,
+               "op12 contraction needs two distinct sites");
+    if (std::abs(dx) > 3 || std::abs(dy) > 3)
+      throw tn::Error("op12 contraction: clusters beyond 4 x 4 are not supported",
+                      TENES_ERR_UNSUPPORTED);
     TN_REQUIRE(op12 && out, "null pointer");
     TN_REQUIRE(source >= 0 && source < env->lat.N_UNIT, "bad source site");
     auto &a = env->arr;
     Cluster c = cluster_of(env->lat, source, dx, dy);
-    if (is_tpo(env)) {
+    if (is_tpo(env) || c.nrow * c.ncol != 2) {
       // the operator split into products of one-site operators by its SVD
-      // over (source legs | target legs), twosite_obs.cpp:223-243; each
-      // product is one single-layer bond cluster
+      // over (source legs | target legs), twosite_obs.cpp:223-243 (TeNeS's
+      // branch for bonds longer than nearest neighbours); each product is
+      // one cluster contraction (single-layer for an iTPO)
       const int64_t d1 = a[8][source].shape[4];
       const int64_t d2 = a[8][c.idx[c.target_row][c.target_col]].shape[4];
       std::vector<double> M((size_t)(d1 * d1 * d2 * d2));
@@ -1077,8 +1089,8 @@ int tenes_full_update_bond(const tenes_tensor *const *Ts, const double *op12,
     for (int i = 0; i < 12; ++i) TN_REQUIRE(Ts[i] != nullptr, "null tensor handle");
     const tn::Tensor &A = T(Ts[10]), &B = T(Ts[11]);
     TN_REQUIRE(A.rank() == 5 && B.rank() == 5, "site tensors must have 5 legs");
-    const int64_t n = A.shape[4] * B.shape[4] * A.shape[4] * B.shape[4];
-    std::vector<double> op(op12, op12 + n);
+    tn::host::HT op({A.shape[4], B.shape[4], A.shape[4], B.shape[4]});
+    std::copy(op12, op12 + op.size(), op.v.begin());
     tn::Tensor n1, n2;
     tenes::itps::core::Full_update_bond(
         T(Ts[0]), T(Ts[1]), T(Ts[2]), T(Ts[3]), T(Ts[4]), T(Ts[5]), T(Ts[6]),
@@ -1089,7 +1101,8 @@ int tenes_full_update_bond(const tenes_tensor *const *Ts, const double *op12,
 }
 
 int tenes_env_full_update(tenes_env *env, int nops, const int *source,
-                          const int *source_leg, const double *ops, int nsteps,
+                          const int *source_leg, const int *op_dims,
+                          const double *ops, int nsteps,
                           const tenes_peps_parameters *pp,
                           const tenes_fu_parameters *fp) {
   if (!env) return TENES_ERR_ARGUMENT;
@@ -1100,15 +1113,25 @@ int tenes_env_full_update(tenes_env *env, int nops, const int *source,
                "null pointer");
     auto &a = env->arr;
     const int N = env->lat.N_UNIT;
-    std::vector<std::vector<double>> op(nops);
+    std::vector<tn::host::HT> op(nops);
     size_t off = 0;
     for (int k = 0; k < nops; ++k) {
       TN_REQUIRE(source[k] >= 0 && source[k] < N, "bad source site");
       TN_REQUIRE(source_leg[k] >= 0 && source_leg[k] < 4, "bad source leg");
-      const int t = env->lat.neighbor(source[k], source_leg[k]);
-      const int64_t ds = a[8][source[k]].shape[4], dt = a[8][t].shape[4];
-      op[k].assign(ops + off, ops + off + ds * dt * ds * dt);
-      off += (size_t)(ds * dt * ds * dt);
+      std::vector<int64_t> dims(4);
+      if (op_dims) {
+        for (int q = 0; q < 4; ++q) {
+          TN_REQUIRE(op_dims[4 * k + q] >= 1, "bad operator dimension");
+          dims[q] = op_dims[4 * k + q];
+        }
+      } else {
+        const int t = env->lat.neighbor(source[k], source_leg[k]);
+        const int64_t ds = a[8][source[k]].shape[4], dt = a[8][t].shape[4];
+        dims = {ds, dt, ds, dt};
+      }
+      op[k] = tn::host::HT(dims);
+      std::copy(ops + off, ops + off + op[k].size(), op[k].v.begin());
+      off += (size_t)op[k].size();
     }
     if (nsteps <= 0) return;
     const auto q = to_params(pp);

@@ -438,38 +438,40 @@ void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
   const bool fork = nl > 1 && total <= kSyncSlot && !ctx.forked();
   std::vector<std::vector<ProjectorJob>> jobs(moves.size());
   std::vector<std::vector<tensor>> PUs(moves.size()), PLs(moves.size());
-  if (fork) ctx.fork(nl);
-  int q = 0;
-  for (size_t m = 0; m < moves.size(); ++m) {
-    jobs[m].resize(moves[m]->L);
-    for (int i = 0; i < moves[m]->L; ++i, ++q) {
-      if (fork) ctx.use_lane(q % nl);
-      moves[m]->begin(i, fork ? q : kSyncSlot, jobs[m][i]);
-      if (!fork) projectors_finish(ctx, jobs[m][i], pp);
+  {
+    tn::ForkScope lanes(ctx, nl, fork);
+    int q = 0;
+    for (size_t m = 0; m < moves.size(); ++m) {
+      jobs[m].resize(moves[m]->L);
+      for (int i = 0; i < moves[m]->L; ++i, ++q) {
+        if (fork) ctx.use_lane(q % nl);
+        moves[m]->begin(i, fork ? q : kSyncSlot, jobs[m][i]);
+        if (!fork) projectors_finish(ctx, jobs[m][i], pp);
+      }
     }
-  }
-  q = 0;
-  for (size_t m = 0; m < moves.size(); ++m) {
-    PUs[m].resize(moves[m]->L);
-    PLs[m].resize(moves[m]->L);
-    for (int i = 0; i < moves[m]->L; ++i, ++q) {
-      if (fork) ctx.use_lane(q % nl);
-      projectors_finish(ctx, jobs[m][i], pp);
-      PUs[m][i] = jobs[m][i].PU;
-      PLs[m][i] = jobs[m][i].PL;
+    q = 0;
+    for (size_t m = 0; m < moves.size(); ++m) {
+      PUs[m].resize(moves[m]->L);
+      PLs[m].resize(moves[m]->L);
+      for (int i = 0; i < moves[m]->L; ++i, ++q) {
+        if (fork) ctx.use_lane(q % nl);
+        projectors_finish(ctx, jobs[m][i], pp);
+        PUs[m][i] = jobs[m][i].PU;
+        PLs[m][i] = jobs[m][i].PL;
+      }
     }
+    // a row's renormalisation uses the projectors of the neighbouring rows
+    lanes.end();
   }
-  // a row's renormalisation uses the projectors of the neighbouring rows
-  if (fork) ctx.join();
   for (auto *m : moves) m->snapshot();
-  if (fork) ctx.fork(nl);
-  q = 0;
+  tn::ForkScope lanes(ctx, nl, fork);
+  int q = 0;
   for (size_t m = 0; m < moves.size(); ++m)
     for (int i = 0; i < moves[m]->L; ++i, ++q) {
       if (fork) ctx.use_lane(q % nl);
       moves[m]->update(i, PUs[m], PLs[m]);
     }
-  if (fork) ctx.join();
+  lanes.end();
 }
 
 using Arr = std::vector<tensor>;

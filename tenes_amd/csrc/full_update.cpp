@@ -79,16 +79,16 @@ void Full_update_bond_horizontal(const tensor &C1, const tensor &C2,
                                  const tensor &eT3, const tensor &eT4,
                                  const tensor &eT5, const tensor &eT6,
                                  const tensor &Tn1, const tensor &Tn2,
-                                 const std::vector<double> &op12,
+                                 const HT &op,
                                  const FU_Parameters &fp, tensor &Tn1_new,
                                  tensor &Tn2_new) {
   tn::Context &ctx = *Tn1.ctx();
   const int64_t D_connect = Tn1.shape[2];
   const int64_t d1 = Tn1.shape[4], d2 = Tn2.shape[4];
-  TN_REQUIRE((int64_t)op12.size() == d1 * d2 * d1 * d2,
-             "Full_update_bond: operator size differs from d1*d2*d1*d2");
-  HT op({d1, d2, d1, d2});
-  op.v = op12;
+  TN_REQUIRE(op.rank() == 4 && op.shape[0] == d1 && op.shape[1] == d2 &&
+                 op.size() == op.shape[0] * op.shape[1] * op.shape[2] * op.shape[3],
+             "Full_update_bond: operator legs (d1, d2, d1o, d2o) do not match "
+             "the physical dimensions of the site tensors");
 
   // QR of the site tensors: the bond and physical legs go to R
   HT Q1, R1, Q2, R2;
@@ -193,7 +193,7 @@ void Full_update_bond(const tensor &C1, const tensor &C2, const tensor &C3,
                       const tensor &C4, const tensor &eT1, const tensor &eT2,
                       const tensor &eT3, const tensor &eT4, const tensor &eT5,
                       const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
-                      const std::vector<double> &op12, int connect1,
+                      const HT &op12, int connect1,
                       const FU_Parameters &fp, tensor &Tn1_new,
                       tensor &Tn2_new) {
   TN_REQUIRE(connect1 >= 0 && connect1 < 4, "Full_update_bond: bad leg");
@@ -219,7 +219,7 @@ void full_update_step(std::vector<tensor> &C1, std::vector<tensor> &C2,
                       std::vector<tensor> &eTt, std::vector<tensor> &eTr,
                       std::vector<tensor> &eTb, std::vector<tensor> &eTl,
                       std::vector<tensor> &Tn, int source, int leg,
-                      const std::vector<double> &op12,
+                      const HT &op12,
                       const PEPS_Parameters &pp, const FU_Parameters &fp,
                       const SquareLattice &lat) {
   TN_REQUIRE(leg >= 0 && leg < 4, "full update: bad source leg");

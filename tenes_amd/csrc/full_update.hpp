@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "ctm.hpp"
+#include "small_dense.hpp"
 
 namespace tenes {
 namespace itps {
@@ -30,14 +31,16 @@ struct FU_Parameters {
 
 namespace core {
 
-// core/full_update.cpp:85-393: op12 is a host array (d1, d2, d1, d2).
+// core/full_update.cpp:85-393: op12 is a host tensor (d1, d2, d1o, d2o) with
+// d1, d2 the physical dimensions of Tn1, Tn2; the new tensors get d1o, d2o
+// (a path-decomposed gate changes the intermediate site's physical leg).
 void Full_update_bond_horizontal(const tensor &C1, const tensor &C2,
                                  const tensor &C3, const tensor &C4,
                                  const tensor &eT1, const tensor &eT2,
                                  const tensor &eT3, const tensor &eT4,
                                  const tensor &eT5, const tensor &eT6,
                                  const tensor &Tn1, const tensor &Tn2,
-                                 const std::vector<double> &op12,
+                                 const tn::host::HT &op12,
                                  const FU_Parameters &fp, tensor &Tn1_new,
                                  tensor &Tn2_new);
 
@@ -46,7 +49,7 @@ void Full_update_bond(const tensor &C1, const tensor &C2, const tensor &C3,
                       const tensor &C4, const tensor &eT1, const tensor &eT2,
                       const tensor &eT3, const tensor &eT4, const tensor &eT5,
                       const tensor &eT6, const tensor &Tn1, const tensor &Tn2,
-                      const std::vector<double> &op12, int connect1,
+                      const tn::host::HT &op12, int connect1,
                       const FU_Parameters &fp, tensor &Tn1_new,
                       tensor &Tn2_new);
 
@@ -60,7 +63,7 @@ void full_update_step(std::vector<tensor> &C1, std::vector<tensor> &C2,
                       std::vector<tensor> &eTt, std::vector<tensor> &eTr,
                       std::vector<tensor> &eTb, std::vector<tensor> &eTl,
                       std::vector<tensor> &Tn, int source, int source_leg,
-                      const std::vector<double> &op12,
+                      const tn::host::HT &op12,
                       const PEPS_Parameters &pp, const FU_Parameters &fp,
                       const SquareLattice &lat);
 

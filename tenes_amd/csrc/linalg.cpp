@@ -14,6 +14,7 @@ namespace tn {
 namespace {
 
 constexpr int kJac = 128;  // LDS-resident Jacobi limit
+constexpr int kMinOversample = 16;  // range-finder columns beyond k
 
 // Raw device buffer of ints carried by a Tensor (for lifetime management).
 struct IntBuf {
@@ -134,6 +135,14 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     X = transpose(Rf, Axes{1, 0});
   } else {
     const int p = kJac;
+    // the subspace error after q iterations is ~(s_{p+1}/s_k)^(2q+1): with
+    // fewer than 16 columns of oversampling no iteration count bounds it
+    if (k_target > p - kMinOversample)
+      throw Error("svd_top: " + std::to_string(k_target) +
+                      " leading triplets of a matrix wider than 128 need a "
+                      "range-finder block wider than 128 (at most " +
+                      std::to_string(p - kMinOversample) + " supported)",
+                  -5);
     const bool warm = opt.start && opt.start->defined() && opt.start->rank() == 2 &&
                       opt.start->shape[0] == n && opt.start->shape[1] == p;
     Tensor Om;

@@ -74,7 +74,19 @@ void Simple_update_bond(const tensor &Tn1, const tensor &Tn2,
   auto qr = [&](const tensor &T, tensor &Q, tensor &R) {
     const int64_t m = T.shape[0] * T.shape[1] * T.shape[2];
     const int64_t n = T.shape[3] * T.shape[4];
-    TN_REQUIRE(m >= n, "simple update: QR needs rows >= columns");
+    if (m < n) {
+      // fewer rows than columns (a site whose physical leg a path-decomposed
+      // gate has widened): the thin QR has k = m, and the new tensors
+      // Q Uc do not depend on which orthonormal basis of R^m Q is (Q -> Q O
+      // turns R into O^T R and the SVD factor Uc into O^T Uc), so Q = I,
+      // R = T is exact
+      std::vector<double> id((size_t)(m * m), 0.0);
+      for (int64_t i = 0; i < m; ++i) id[(size_t)(i * m + i)] = 1.0;
+      Q = tn::from_host(ctx, Shape{T.shape[0], T.shape[1], T.shape[2], m},
+                        id.data());
+      R = tn::reshape(T, Shape{m, T.shape[3], T.shape[4]});
+      return;
+    }
     tensor Rm;
     tensor Qm = tn::orthonormalize(ctx, tn::reshape(T, Shape{m, n}), &Rm);
     Q = tn::reshape(Qm, Shape{T.shape[0], T.shape[1], T.shape[2], n});

@@ -176,6 +176,22 @@ void Context::join() {
   alloc_.set_deferred(false);
 }
 
+void Context::join_noexcept() noexcept {
+  if (!forked_) return;
+  for (int i = 1; i <= nforked_; ++i) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) continue;
+    if (hipEventRecord(e, lanes_[i].s) == hipSuccess)
+      (void)hipStreamWaitEvent(lanes_[0].s, e, 0);
+    (void)hipEventDestroy(e);  // released once the recorded work completes
+  }
+  (void)hipGetLastError();
+  cur_ = 0;
+  forked_ = false;
+  nforked_ = 0;
+  alloc_.set_deferred(false);
+}
+
 void Context::forget_svd_starts(int64_t environment) {
   for (auto it = svd_start.begin(); it != svd_start.end();) {
     if (it->first[0] == environment)
@@ -412,6 +428,16 @@ const ContractPlan &Context::plan(const Shape &sa, const Shape &sb,
   build_offsets(dn, sn, t_bn);
   build_offsets(dm, cm, t_cm);
   build_offsets(dn, cn, t_cn);
+  // the kernel adds an outer and a contracted offset (and the two output
+  // offsets) in 32-bit arithmetic: every such sum must fit as well
+  auto maxv = [](const std::vector<int> &t) {
+    return t.empty() ? (int64_t)0 : (int64_t)*std::max_element(t.begin(), t.end());
+  };
+  TN_REQUIRE(maxv(t_am) + maxv(t_ak) <= INT32_MAX &&
+                 maxv(t_bk) + maxv(t_bn) <= INT32_MAX &&
+                 maxv(t_cm) + maxv(t_cn) <= INT32_MAX,
+             "tensordot: operand or result too large for 32-bit offsets " +
+                 shape_str(sa) + " x " + shape_str(sb));
 
   ContractPlan p;
   p.M = (int)t_am.size();

@@ -118,7 +118,12 @@ class Context {
   void fork(int n);
   void use_lane(int i);
   void join();
+  // join() for error paths: never throws; the main stream waits for every
+  // lane where the runtime still accepts the calls, and the context is left
+  // on its main stream with frees no longer deferred in any case.
+  void join_noexcept() noexcept;
   bool forked() const { return forked_; }
+  int current_lane() const { return cur_ - 1; }  // -1: the main stream
   // Upper bound on the lanes the CTM moves fork into (1: no overlap, every
   // kernel on the main stream in issue order).
   int max_lanes() const { return max_lanes_; }
@@ -227,6 +232,32 @@ class Context {
   std::map<std::string, KernelStat> kstats_;
   hipEvent_t get_event();
   void flush_pending();
+};
+
+// A fork of a context that cannot outlive its scope: end() joins on the
+// normal path; leaving the scope by an exception joins with
+// join_noexcept(), so an error inside the fork never leaves later work of
+// the context on a side stream or its frees deferred.
+class ForkScope {
+ public:
+  ForkScope(Context &ctx, int lanes, bool on) : ctx_(ctx), active_(on) {
+    if (active_) ctx_.fork(lanes);
+  }
+  void end() {
+    if (active_) {
+      active_ = false;
+      ctx_.join();
+    }
+  }
+  ~ForkScope() {
+    if (active_) ctx_.join_noexcept();
+  }
+  ForkScope(const ForkScope &) = delete;
+  ForkScope &operator=(const ForkScope &) = delete;
+
+ private:
+  Context &ctx_;
+  bool active_;
 };
 
 // Sets the contraction precision of a context for a scope.

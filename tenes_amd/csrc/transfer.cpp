@@ -142,7 +142,17 @@ std::vector<std::complex<double>> TransferMatrix_ctm::eigenvalues(
     return eig;
   }
   int maxvec = params.arnoldi_maxdim;
-  if ((int64_t)N < maxvec) maxvec = (int)N;
+  int maxiter = params.arnoldi_maxiter;
+  if ((int64_t)N < maxvec) {
+    maxvec = (int)N;
+    maxiter = 1;  // transfer_matrix.cpp:117-120
+  }
+  // The restarts of arnoldi.cpp:72-86, 117-167 (maxiter > 1) are not built:
+  // refused rather than run as a single pass.
+  if (maxiter > 1)
+    throw tn::Error("transfer matrix: arnoldi_maxiter > 1 (Arnoldi restarts) is "
+                    "not supported",
+                    -5);
   hipStream_t s = ctx.stream();
   tensor Qall = tn::empty(ctx, Shape{maxvec + 1, (int64_t)N});
   auto Qp = [&](int k) { return Qall.data() + (int64_t)k * N; };

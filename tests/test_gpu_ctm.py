@@ -413,18 +413,20 @@ def _fixture_or_skip(name):
         pytest.skip(f"fixture {name}.json not generated (tests/golden/make_oracle_runs.py)")
 
 
-@pytest.mark.parametrize("warm", [0, 1])
-def test_ctm_random_d4_chi64_30_sweeps(gpu_ctx, warm):
+@pytest.mark.parametrize("kw", [{}, {"svd_power_iters": 16, "svd_warm_start": 0},
+                                {"svd_power_iters": 16, "svd_warm_start": 1}],
+                         ids=["library-defaults", "cold-q16", "warm-after-q16"])
+def test_ctm_random_d4_chi64_30_sweeps(gpu_ctx, kw):
     """CTM-only case: 2x2, D=4, chi=64, unit-norm random site tensors, 30
     sweeps with epsilon = 0; L2-normalised spectra of every C1..C4 against the
-    oracle's LAPACK-SVD run (tests/golden/ctm_random_d4_chi64.json) to 1e-10.
-    The spectrum of R decays slowly for random tensors (s_128/s_64 ~ 0.38, so
-    the range finder's subspace error after q iterations is ~0.38^(2q+1)):
-    measured worst spectrum differences after 30 sweeps are 2.5e-7 (q=4),
-    3.2e-10 (q=8) and 4.4e-15 (q=16); the test runs q=16.  warm=1 adds the
-    warm-started range finder: this environment settles (corner-spectrum
-    change 7e-8 at sweep 7, 1e-15 from sweep 14), so from sweep 8 on the
-    projectors run 2 iterations from the previous sweep's Ritz basis."""
+    oracle's LAPACK-SVD run (tests/golden/ctm_random_d4_chi64.json) to 1e-10,
+    at the library's default range-finder settings and with 16 iterations
+    from a Gaussian start (cold throughout, or warm-started after the first
+    sweep).  The spectrum of R decays slowly for random tensors (s_128/s_64 ~
+    0.38, so the range finder's subspace error after q cold iterations is
+    ~0.38^(2q+1)): without the warm start, q = 4 / 8 / 16 give 2.5e-7 /
+    3.2e-10 / 4.4e-15.  The warm start accumulates iterations over the sweeps
+    (the environment settles: corner-spectrum change 7e-8 at sweep 7)."""
     from tenes_amd import inputs
 
     fx = _fixture_or_skip("ctm_random_d4_chi64")
@@ -432,8 +434,7 @@ def test_ctm_random_d4_chi64_30_sweeps(gpu_ctx, warm):
     genv = ta.Environment(gpu_ctx, 2, 2)
     genv.set_site_tensors(Tn)
     n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=fx["chi"], Max_CTM_Iteration=fx["sweeps"],
-                                                        CTM_Convergence_Epsilon=0.0, svd_power_iters=16,
-                                                        svd_warm_start=warm))
+                                                        CTM_Convergence_Epsilon=0.0, **kw))
     assert n == fx["sweeps"]
     worst = 0.0
     for i in range(4):
@@ -442,6 +443,7 @@ def test_ctm_random_d4_chi64_30_sweeps(gpu_ctx, warm):
             got = genv.corner_spectrum(k, i)
             assert got.shape == ref.shape
             worst = max(worst, float(np.max(np.abs(got - ref))))
+    print(f"D=4 chi=64 30 sweeps {kw or 'library defaults'}: worst spectrum difference {worst:.2e}")
     assert worst < 1e-10, worst
 
 

@@ -111,15 +111,20 @@ def test_gpu_single_layer_ctm_vs_oracle(gpu_ctx):
 def test_gpu_single_layer_updown_projectors_refused(gpu_ctx):
     """CTM_Projector_corner = 0 on an iTPO environment is an argument error
     (TeNeS's single-layer up-down truncation, ctm_single.cpp:400, is not
-    reproduced), and the environment stays usable afterwards."""
-    from tenes_amd import finite_temperature as FTG
-
+    reproduced), raised inside a move whose rows run on side streams; the
+    context is back on its own stream afterwards (no fork left open) and the
+    environment stays usable.  A context of its own, so that a failure here
+    cannot leak into the other tests' session context."""
+    ctx = ta.Context(0, seed=99)
     case = F.setup("FT_TFI_square")
     Tn, _ = _oracle_tpo(case, 50)
-    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv = ta.Environment(ctx, 2, 2)
     genv.set_site_tensors(Tn)
+    assert ctx.lane_state() == (False, -1)
     with pytest.raises(ta.TenesError, match="up-down projectors"):
         genv.calc_ctm_environment_density(ta.peps_parameters(CHI=10, Max_CTM_Iteration=2,
                                                              CTM_Projector_corner=0))
+    assert ctx.lane_state() == (False, -1)
     n, _ = genv.calc_ctm_environment_density(ta.peps_parameters(CHI=10, Max_CTM_Iteration=2))
+    assert ctx.lane_state() == (False, -1)
     assert n >= 1 and np.isfinite(genv.contract_one_site(0))

@@ -266,3 +266,120 @@ def test_honeycomb_skew_full_update_steps(gpu_ctx):
                 m = min(a.size, b.size)
                 assert np.max(np.abs(a[:m] - b[:m])) < 1e-12, (src, leg, k, i)
         Tn, env = Tn2, env2
+
+
+def _gpu_values(genv, lat, fx):
+    """Per-group densities and per-bond values measured on the GPU
+    environment as measure_onesite / measure_twosite / save_density do
+    (twosite_obs.cpp:146-262; explicit operators on longer bonds through the
+    SVD split of twosite_obs.cpp:223-243)."""
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, True)
+    site_op = {(o["site"], o["group"]): o for o in one}
+    dens, per_site, per_bond = {}, {}, {}
+    for g, name in onames.items():
+        tot = 0.0
+        for o in (x for x in one if x["group"] == g):
+            v = o["coeff"] * genv.contract_one_site(o["site"], o["op"]) / genv.contract_one_site(o["site"])
+            per_site[(g, o["site"])] = v
+            tot += v
+        dens[name] = tot / lat.N_UNIT
+    for g, name in tnames.items():
+        tot = 0.0
+        for op in (t for t in two if t["group"] == g):
+            s, dx, dy = op["source"], op["dx"], op["dy"]
+            norm = genv.contract_cluster(s, dx, dy)
+            if "op" in op:
+                v = genv.contract_two_sites_op12(s, dx, dy, op["op"])
+            else:
+                tgt = lat.other(s, dx, dy)
+                v = genv.contract_cluster(s, dx, dy, site_op[(s, op["ops"][0])]["op"],
+                                          site_op[(tgt, op["ops"][1])]["op"])
+            per_bond[(g, s, dx, dy)] = op["coeff"] * v / norm
+            tot += per_bond[(g, s, dx, dy)]
+        dens[name] = dens.get(name, 0.0) + tot / lat.N_UNIT
+    return dens, per_site, per_bond
+
+
+def test_j1j2_full_pipeline_on_gpu(gpu_ctx):
+    """J1J2_AFH (2x2, D=2, chi=5, J2 = 0.1): 100 simple-update steps whose
+    second-neighbour gates are path-decomposed ((2,2,2,8) then (8,2,2,2): the
+    intermediate site's physical leg is 8 between them), 1 full-update step
+    with the same gates and fast-full-update moves, the CTM and the
+    observables, diagonal bonds included, all on the GPU, against TeNeS's
+    output_J1J2_AFH.  The full-update steps of the path gates amplify rounding
+    (a 1e-15 relative input change moves the energy by ~3e-7 in the oracle,
+    tests/test_oracle_goldens.py::test_j1j2_end_to_end_goldens), so the bar
+    is 5e-6, as for the oracle."""
+    from oracle import simple_update as SU
+
+    fx = G.load("J1J2_AFH")
+    LX, LY, skew, pdims, vdims, dirs, noises = G.unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=True)
+    chi = fx["parameter"]["ctm"]["dimension"]
+    genv = ta.Environment(gpu_ctx, LX, LY, skew)
+    genv.set_site_tensors(Tn)
+    genv.simple_update(G.evolution(fx, True), fx["parameter"]["simple_update"]["num_step"])
+    genv.full_update(G.evolution(fx, True, "evolution_full"), fx["parameter"]["full_update"]["num_step"],
+                     ctm_params=ta.peps_parameters(CHI=chi))
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=chi))
+    dens, per_site, per_bond = _gpu_values(genv, lat, fx)
+    gold = G.density_table(fx["outputs"]["density"], timed=False)
+    # this output names the bond term "hamiltonian" (older TeNeS); it equals Energy
+    common = [k for k in dens if k in gold]
+    assert len(common) >= 5
+    worst = max(abs(dens[k] - gold[k].real) for k in common)
+    worst = max(worst, abs(dens["bond_hamiltonian"] - gold["Energy"].real))
+    print(f"J1J2 on the GPU: worst density difference against density.dat {worst:.2e}")
+    for k in common:
+        assert abs(dens[k] - gold[k].real) < 5e-6, (k, dens[k], gold[k])
+    assert abs(dens["bond_hamiltonian"] - gold["Energy"].real) < 5e-6
+    assert abs(dens["bond_hamiltonian"] - gold["hamiltonian"].real) < 5e-6
+    rows = {tuple(int(x) for x in r.split()[:4]): float(r.split()[4]) for r in fx["outputs"]["twosite_obs"]}
+    assert sum(abs(dx) + abs(dy) == 2 for (_, _, dx, dy) in per_bond) == 8
+    for k, v in per_bond.items():
+        assert abs(v - rows[k]) < 5e-6, (k, v, rows[k])
+
+
+def test_j1j2_path_gates_simple_update_and_observables_match_oracle(gpu_ctx):
+    """The well-conditioned parts of the J1J2 pipeline at full precision:
+    (1) 100 simple-update steps with the path-decomposed gates on the GPU
+    against the oracle's (bond weights, gauge invariant, to 1e-10; the GPU
+    QR of a widened site, 8 rows x 16 columns, takes the Q = 1 form);
+    (2) the oracle's final site tensors (after its full update) through the
+    GPU CTM and observables, diagonal J2 bonds through the SVD split, against
+    the oracle on the same tensors to 1e-10."""
+    from oracle import itps, simple_update as SU
+
+    fx = G.load("J1J2_AFH")
+    LX, LY, skew, pdims, vdims, dirs, noises = G.unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    T0 = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=True)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    ev = G.evolution(fx, True)
+    nsu = fx["parameter"]["simple_update"]["num_step"]
+    _, lam = SU.simple_update_steps(list(T0), lam, lat, ev, nsu)
+    genv = ta.Environment(gpu_ctx, LX, LY, skew)
+    genv.set_site_tensors(T0)
+    genv.simple_update(ev, nsu)
+    worst = 0.0
+    for i in range(lat.N_UNIT):
+        for leg in range(4):
+            worst = max(worst, float(np.max(np.abs(genv.get_lambda(i, leg) - lam[i][leg]))))
+    print(f"J1J2 simple update: worst bond-weight difference {worst:.2e}")
+    assert worst < 1e-10
+    fx, lat, pdims, Tn = G.pipeline_site_tensors("J1J2_AFH")
+    chi = fx["parameter"]["ctm"]["dimension"]
+    env, _, _ = O.Calc_CTM_Environment(Tn, G.ctm_parameters(fx), lat)
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, True)
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    ref = itps.density(o, w, lat, pdims, onames, tnames)
+    genv = ta.Environment(gpu_ctx, LX, LY, skew)
+    genv.set_site_tensors(Tn)
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=chi))
+    dens, per_site, per_bond = _gpu_values(genv, lat, fx)
+    for name, v in dens.items():
+        assert abs(v - ref[name].real) < 1e-10 * max(1.0, abs(ref[name])), (name, v, ref[name])
+    for (g, s, dx, dy), v in per_bond.items():
+        assert abs(v - w[g][(s, dx, dy)].real) < 1e-10, (g, s, dx, dy, v)

@@ -299,3 +299,126 @@ def test_rsvd_updown_end_to_end_goldens():
         f = r.split()
         key = (int(f[0]), int(f[1]), int(f[2]), int(f[3]), int(f[4]))
         assert abs(cor[key] - complex(float(f[5]), float(f[6]))) < 1e-12, (key, cor[key], f)
+
+
+def _j1j2_pipeline(perturb_seed=None):
+    """J1J2_AFH through the oracle: SU 100 (path-decomposed second-neighbour
+    gates), optionally a 1e-15 relative perturbation of the SU tensors, FU 1
+    (fast full update), CTM chi = 5, observables."""
+    from oracle import full_update as FU, simple_update as SU
+    from oracle.lattice import SquareLattice
+
+    fx = G.load("J1J2_AFH")
+    LX, LY, skew, pdims, vdims, dirs, noises = G.unitcell(fx)
+    lat = SquareLattice(LX, LY, skew)
+    Tn = SU.initial_tensors(lat, pdims, vdims, dirs, noises, seed=11, is_real=True)
+    lam = [[np.ones(vdims[i][l]) for l in range(4)] for i in range(lat.N_UNIT)]
+    Tn, lam = SU.simple_update_steps(Tn, lam, lat, G.evolution(fx, True), fx["parameter"]["simple_update"]["num_step"])
+    if perturb_seed is not None:
+        rng = np.random.default_rng(perturb_seed)
+        Tn = [t * (1.0 + 1e-15 * rng.standard_normal(t.shape)) for t in Tn]
+    p = G.ctm_parameters(fx)
+    Tn = FU.full_update(Tn, lat, G.evolution(fx, True, "evolution_full"), fx["parameter"]["full_update"]["num_step"],
+                        p, FU.FUParameters())
+    env, count, _ = ctm.Calc_CTM_Environment(Tn, p, lat)
+    one, onames, two, tnames = G.operators(fx, lat.N_UNIT, True)
+    o, _ = itps.measure_onesite(env, Tn, lat, one)
+    w, _ = itps.measure_twosite(env, Tn, lat, two, one)
+    return fx, lat, pdims, Tn, env, o, w, onames, tnames
+
+
+def test_j1j2_end_to_end_goldens():
+    """J1J2_AFH (real, 2x2, D=2, chi=5; J2 = 0.1 on the diagonal bonds, whose
+    gates are path-decomposed through the intermediate site, and whose bond
+    energies take the SVD-split branch of twosite_obs.cpp:223-243): SU 100 +
+    FU 1 + CTM + observables against output_J1J2_AFH/.
+
+    This pipeline is ill-conditioned: the full-update steps of the
+    path-decomposed gates turn a 1e-15 relative change of their input tensors
+    into ~3e-7 of energy (without them, or with the nearest-neighbour gates
+    only, the spread stays below 1e-15), so two FP64 implementations that
+    round differently cannot agree beyond ~1e-6.  The test measures that
+    sensitivity itself and compares densities, per-site and per-bond values at
+    5e-6 and correlation lengths at 2e-5 relative; the end-to-end goldens of
+    AntiferroHeisenberg_real and RSVD pin the same code to 1e-12."""
+    fx, lat, pdims, Tn, env, o, w, onames, tnames = _j1j2_pipeline()
+    dens = itps.density(o, w, lat, pdims, onames, tnames)
+    gold = G.density_table(fx["outputs"]["density"], timed=False)
+    n = 0
+    for k, v in gold.items():
+        if k in dens:
+            assert abs(dens[k] - v) < 5e-6, (k, dens[k], v)
+            n += 1
+    assert n >= 6
+    # per-site values (onesite_obs.dat: group site re im)
+    for r in fx["outputs"]["onesite_obs"]:
+        f = r.split()
+        g, s = int(f[0]), int(f[1])
+        if g in o:
+            assert abs(o[g][s] - float(f[2])) < 5e-6, (g, s, o[g][s], f)
+    # per-bond values, diagonal J2 bonds included (twosite_obs.dat: group
+    # source dx dy re im); the J1 and J2 bond Hamiltonians share group 0
+    rows = {}
+    for r in fx["outputs"]["twosite_obs"]:
+        f = r.split()
+        rows[(int(f[0]), int(f[1]), int(f[2]), int(f[3]))] = float(f[4])
+    diag = 0
+    for g, vals in w.items():
+        for (s, dx, dy), v in vals.items():
+            assert abs(v - rows[(g, s, dx, dy)]) < 5e-6, (g, s, dx, dy, v)
+            diag += abs(dx) + abs(dy) == 2
+    assert diag == 8
+    for r, (d, f, xi, es, ev) in zip(fx["outputs"]["correlation_length"],
+                                     itps.correlation_lengths(env, Tn, lat, TM.TMParameters())):
+        ref = [float(x) for x in r.split()]
+        assert (ref[0], ref[1]) == (d, f)
+        assert abs(xi - ref[2]) < 2e-5 * abs(ref[2]), (d, f, xi, ref[2])
+    # the conditioning that sets the tolerance
+    e0 = dens["Energy"].real
+    spread = 0.0
+    for seed in (1, 2):
+        fx, lat, pdims, _, _, o2, w2, onames, tnames = _j1j2_pipeline(seed)
+        spread = max(spread, abs(itps.density(o2, w2, lat, pdims, onames, tnames)["Energy"].real - e0))
+    print(f"J1J2 energy: golden difference {abs(e0 - gold['Energy'].real):.2e}, "
+          f"spread under 1e-15 input noise {spread:.2e}")
+    assert spread > 1e-9
+
+
+def test_svd_split_bond_operator_equals_reduced_density_matrix():
+    """twosite_obs.cpp:223-243 (a two-site operator on a bond longer than
+    nearest neighbours, split by its SVD over (source legs 0, 2 | target legs
+    1, 3) into products of one-site operators) against an independent route:
+    op12 = sum op12[a, c, a', c'] E_aa' (x) E_cc' with E_aa' the one-site basis
+    operators, i.e. the bond's reduced density matrix from d^4 cluster
+    contractions.  Non-symmetric random operator, diagonal and straight
+    second/third-neighbour bonds, random D = 2 environment; a swap of the
+    operator's leg pairing or of source and target fails."""
+    from oracle import contract as K
+    from oracle.lattice import SquareLattice
+
+    lat = SquareLattice(2, 2)
+    rng = np.random.default_rng(5)
+    Tn = [rng.uniform(-1, 1, (2, 2, 2, 2, 2)) for _ in range(4)]
+    env, _, _ = ctm.Calc_CTM_Environment(Tn, ctm.PEPSParameters(CHI=6, Max_CTM_Iteration=4,
+                                                                CTM_Convergence_Epsilon=0.0), lat)
+    op12 = rng.standard_normal((2, 2, 2, 2))
+    bonds = [(0, 1, 1), (1, -1, 1), (2, 1, -1), (3, 2, 0), (0, 0, 2), (1, 1, -2)]
+    two = [{"name": "x", "group": 0, "source": s, "dx": dx, "dy": dy, "op": op12} for s, dx, dy in bonds]
+    got, norms = itps.measure_twosite(env, Tn, lat, two, [])
+    E = np.eye(2)
+    for s, dx, dy in bonds:
+        nrow, ncol, srow, scol, trow, tcol, idx = itps.cluster(lat, s, dx, dy)
+        Tn_ = [[Tn[idx[r][c]] for c in range(ncol)] for r in range(nrow)]
+        C, eTt, eTr, eTb, eTl = itps._cluster_args(env, idx, nrow, ncol)
+        val = 0.0
+        for a, c, a2, c2 in np.ndindex(2, 2, 2, 2):
+            ops_ = [[np.eye(2) for _ in range(ncol)] for _ in range(nrow)]
+            ops_[srow][scol] = np.outer(E[a], E[a2])
+            ops_[trow][tcol] = np.outer(E[c], E[c2])
+            val += op12[a, c, a2, c2] * K.Contract_iTPS_CTM(C, eTt, eTr, eTb, eTl, Tn_, ops_)
+        norm = norms[(idx[nrow - 1][0], nrow - 1, ncol - 1)]
+        assert abs(got[0][(s, dx, dy)] - val / norm) < 1e-12 * max(1.0, abs(val / norm)), (s, dx, dy)
+        # the leg pairing matters for this operator
+        swapped = {"name": "x", "group": 0, "source": s, "dx": dx, "dy": dy, "op": op12.transpose(0, 1, 3, 2)}
+        bad, _ = itps.measure_twosite(env, Tn, lat, [swapped], [])
+        assert abs(bad[0][(s, dx, dy)] - val / norm) > 1e-6
