@@ -272,6 +272,11 @@ def main():
     ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
                     help="headline in mixed precision: contractions of the moves on the FP32 MFMA, 1 all but "
                          "the projector matrix R = LT LB, 2 R as well (projector SVDs and the rest FP64)")
+    ap.add_argument("--host-cut", action="store_true",
+                    help="decide the projector cut on the host (one synchronisation per projector, the "
+                         "reference's shapes) instead of on the device")
+    ap.add_argument("--moves-in-flight", type=int, default=None,
+                    help="bound on CTM moves issued by the host but unfinished on the device (0: none)")
     ap.add_argument("--probe-gemm", action="store_true",
                     help="run only the GEMM + copy workload for rocprofv3 counter passes")
     args = ap.parse_args()
@@ -290,9 +295,13 @@ def main():
     ctx = ta.Context(0, seed=2024)
     if args.autotune:
         ctx.set_autotune(True)
+    if args.moves_in_flight is not None:
+        ctx.set_max_moves_in_flight(args.moves_in_flight)
     fx = load_fixture(args.D, args.chi, args.su_steps)
     env, H, Tn = build_inputs(ta, ctx, args.D, args.su_steps, 0.01, fx)
     kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)
+    if args.host_cut:
+        kw["projector_cut_on_device"] = 0
     if args.power_iters is not None:
         kw["svd_power_iters"] = args.power_iters
     if args.warm_iters is not None:

@@ -62,6 +62,11 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx);
  * order on the context stream).  No counterpart in the reference, whose
  * moves loop over the rows (src/iTPS/core/ctm.cpp:577-615). */
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n);
+/* Host run-ahead bound of the CTM driver and moves: after issuing a move
+ * (or the pair of moves the driver overlaps) the host waits until at most n
+ * issued moves are unfinished on the device; 0 (default): no bound.  No
+ * counterpart in the reference. */
+int tenes_ctx_set_max_moves_in_flight(tenes_ctx *ctx, int n);
 /* Diagnostics: *forked = 1 while the context's work is split over its side
  * streams (inside a CTM move), *lane = the stream new work goes to (-1: the
  * context stream).  Between library calls they are 0 and -1, also after a
@@ -119,7 +124,6 @@ int tenes_transpose(const tenes_tensor *T, const int *axes, tenes_tensor **out);
  * *k_out receives the number returned. */
 int tenes_svd_top(const tenes_tensor *A, int k, int power_iters,
                   tenes_tensor **U, tenes_tensor **V, double *s, int *k_out);
-
 /* ---- parameters (PEPS_Parameters.hpp:29-94, CTM fields) ------------------ */
 typedef struct tenes_peps_parameters {
   int CHI;
@@ -148,6 +152,16 @@ typedef struct tenes_peps_parameters {
    * SVDs, QRs, the convergence check, Arnoldi and observables stay FP64.
    * Default 0. */
   int contraction_fp32;
+  /* Where the projector cut of ctm.cpp:275-288 is decided inside the
+   * environment driver and the moves.  0 (default): on the host, the
+   * reference's shapes, one synchronisation per projector.  1 (no
+   * counterpart in the reference): on the device -- every projector keeps
+   * min(rows, cols, CHI) columns and a dropped direction gets a zero
+   * column, so a sweep has no host round trip and fixed shapes; the
+   * environment tensors stay CHI-sized where the reference's shrink (zero
+   * rows and columns instead: the same network, the same observables).  The
+   * standalone projector entry points always cut on the host. */
+  int projector_cut_on_device;
 } tenes_peps_parameters;
 /* Defaults of PEPS_Parameters.cpp:29-77. */
 void tenes_peps_parameters_default(tenes_peps_parameters *p);

@@ -33,6 +33,7 @@ class PepsParameters(C.Structure):
         ("svd_warm_start", C.c_int),
         ("svd_warm_power_iters", C.c_int),
         ("contraction_fp32", C.c_int),
+        ("projector_cut_on_device", C.c_int),
     ]
 
 
@@ -112,6 +113,7 @@ _SIGS = {
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
     "tenes_ctx_set_fp32_contractions": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_lane_state": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
+    "tenes_ctx_set_max_moves_in_flight": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_batch_calc_ctm_environment": (
         C.c_int, [c_void_pp, C.c_int, C.POINTER(PepsParameters), C.c_int, C.c_int, c_int_p, c_dbl_p]),
     "tenes_fu_parameters_default": (None, [C.POINTER(FuParameters)]),

@@ -51,6 +51,10 @@ class Context:
         """Side streams a CTM move may overlap its rows on (1: none)."""
         self.check(self.lib.tenes_ctx_set_max_lanes(self.h, int(n)))
 
+    def set_max_moves_in_flight(self, n: int):
+        """Bound on CTM moves issued but unfinished on the device (0: none)."""
+        self.check(self.lib.tenes_ctx_set_max_moves_in_flight(self.h, int(n)))
+
     def lane_state(self):
         """(forked, lane): whether work is split over side streams and the
         stream new work goes to (-1: the context stream)."""

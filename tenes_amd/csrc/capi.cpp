@@ -87,6 +87,7 @@ PEPS_Parameters to_params(const tenes_peps_parameters *p) {
   q.svd_warm_start = p->svd_warm_start != 0;
   q.svd_warm_power_iters = p->svd_warm_power_iters;
   q.contraction_fp32 = p->contraction_fp32;
+  q.projector_cut_on_device = p->projector_cut_on_device != 0;
   TN_REQUIRE(q.CHI >= 1, "CHI must be positive");
   TN_REQUIRE(q.RSVD_Oversampling_factor >= 1.0,
              "RSVD oversampling factor must be >= 1");
@@ -238,7 +239,7 @@ double bond_cluster_value(tenes_env *env, const Cluster &c,
 
 extern "C" {
 
-int tenes_abi_version(void) { return 205; }
+int tenes_abi_version(void) { return 206; }
 
 int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
   return guard(nullptr, [&] {
@@ -301,6 +302,13 @@ int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {
   return guard(ctx, [&] {
     TN_REQUIRE(n >= 1, "max lanes must be >= 1");
     ctx->c->set_max_lanes(n);
+  });
+}
+
+int tenes_ctx_set_max_moves_in_flight(tenes_ctx *ctx, int n) {
+  return guard(ctx, [&] {
+    TN_REQUIRE(n >= 0, "moves in flight must be >= 0");
+    ctx->c->set_max_moves_in_flight(n);
   });
 }
 
@@ -458,6 +466,7 @@ void tenes_peps_parameters_default(tenes_peps_parameters *p) {
   p->svd_warm_start = q.svd_warm_start ? 1 : 0;
   p->svd_warm_power_iters = q.svd_warm_power_iters;
   p->contraction_fp32 = q.contraction_fp32;
+  p->projector_cut_on_device = q.projector_cut_on_device ? 1 : 0;
 }
 
 int tenes_calc_projector_left_block(

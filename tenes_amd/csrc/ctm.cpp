@@ -129,9 +129,8 @@ int truncate_spectrum(const std::vector<double> &s, int cut, double eps,
 }
 
 tensor identity_projector(tn::Context &ctx, int64_t e78, int64_t t41) {
-  std::vector<double> id((size_t)(e78 * e78), 0.0);
-  for (int64_t i = 0; i < e78; ++i) id[(size_t)(i * e78 + i)] = 1.0;
-  tensor I = tn::from_host(ctx, Shape{e78, e78}, id.data());
+  tensor I = tn::empty(ctx, Shape{e78, e78});
+  tn::kern::eye(I.data(), e78, ctx.stream());
   return tn::reshape(I, Shape{e78, t41, t41, e78});
 }
 
@@ -168,18 +167,21 @@ tensor enlarged_LB(const tensor &C4, const tensor &eT7, const tensor &eT6,
 // the singular values and issues the projector GEMMs.
 struct ProjectorJob {
   bool done = false;  // identity projector (t41 == 1): PU, PL already set
+  // cut decided on the device (fixed shapes, no host wait), else on the host
+  bool device_cut = false;
   tensor PU, PL;
   tensor Ablk, Bblk;
   Axes axA, axB;
   int64_t r0 = 0, r1 = 0, rb = 0, c0 = 0, c1 = 0, cb = 0;  // (t, e, tc) legs
   int kmax = 0;
+  int slot = 0;  // pinned slot of the singular values
   tn::SvdPending sv;
   // warm-start entry to update with the Ritz-value range once known
   bool warm_store = false;
   std::array<int64_t, 5> wkey{};
 };
 
-constexpr int kSyncSlot = 61;  // pinned slot of the synchronous entry points
+constexpr int kSyncSlot = tn::Context::kSlotProjectorSync;
 
 // From the SVD of the (rows x cols) matrix `Rm`, build the oblique projector
 // pair: PU = Bblk * (VT_c^dagger s^-1/2), PL = Ablk * (U_c^* s^-1/2), both
@@ -216,6 +218,7 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
     ctx.svd_start[wkey] = {job.sv.r.Vbasis, job.sv.r.sbasis, 0.0};
     job.wkey = wkey;
   }
+  job.slot = slot;
   job.Ablk = Ablk;
   job.Bblk = Bblk;
   job.axA = axA;
@@ -231,6 +234,28 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
 void projectors_finish(tn::Context &ctx, ProjectorJob &job,
                        const PEPS_Parameters &pp) {
   if (job.done) return;
+  if (job.device_cut) {
+    // fixed shape: all k = min(rows, cols, CHI) columns, the dropped ones
+    // scaled by zero (ctm.cpp:281-288 decided by projector_scale), applied
+    // to the projector GEMMs' outputs: B (V s^-1/2) = (B V) s^-1/2
+    const int nall = job.sv.nall;
+    tn::SvdResult sv = tn::svd_top_detach(job.sv);
+    ctx.pending_svd.push_back({job.wkey, job.warm_store, job.slot, nall});
+    const int64_t k = sv.s_dev.size();
+    tensor sc = tn::empty(ctx, Shape{k});
+    tn::kern::projector_scale(sv.s_dev.data(), (int)k, pp.Inverse_projector_cut,
+                              sc.data(), ctx.stream());
+    tensor Uc = tn::reshape(sv.U, Shape{job.r0, job.r1, job.rb, k});
+    tensor Vc = tn::reshape(sv.V, Shape{job.c0, job.c1, job.cb, k});
+    job.PU = tn::tensordot_t(job.Bblk, Vc, job.axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+    job.PL = tn::tensordot_t(job.Ablk, Uc, job.axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+    tn::multiply_vector_dev(job.PU, sc.data(), 3);
+    tn::multiply_vector_dev(job.PL, sc.data(), 3);
+    job.Ablk = tensor();
+    job.Bblk = tensor();
+    job.done = true;
+    return;
+  }
   tn::SvdResult sv = tn::svd_top_end(job.sv);
   if (job.warm_store && !sv.s_all.empty()) {
     auto it = ctx.svd_start.find(job.wkey);
@@ -429,13 +454,24 @@ struct MoveSpec {
 
 // Runs one move, or two moves that read and write disjoint arrays, with all
 // their projector chains overlapped on the lanes of the forked context.
+// With the cut on the device the singular values of projector q go to pinned
+// slot `slot` + q (slot advances by the number of projectors) and are read
+// at the caller's next synchronisation (collect_svd_stats).
 void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
-               const std::vector<MoveSpec *> &moves) {
+               const std::vector<MoveSpec *> &moves, int &slot) {
   tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
+  const bool dev = pp.projector_cut_on_device;
   int total = 0;
   for (auto *m : moves) total += m->L;
+  TN_REQUIRE(total < kSyncSlot, "CTM move: too many projectors in one move");
+  if (dev && slot + total > kSyncSlot) {
+    // the pinned slots of this sweep are used up: read them first
+    ctx.sync();
+    ctx.collect_svd_stats();
+    slot = 0;
+  }
   const int nl = std::min(total, ctx.max_lanes());
-  const bool fork = nl > 1 && total <= kSyncSlot && !ctx.forked();
+  const bool fork = nl > 1 && !ctx.forked();
   std::vector<std::vector<ProjectorJob>> jobs(moves.size());
   std::vector<std::vector<tensor>> PUs(moves.size()), PLs(moves.size());
   {
@@ -445,10 +481,12 @@ void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
       jobs[m].resize(moves[m]->L);
       for (int i = 0; i < moves[m]->L; ++i, ++q) {
         if (fork) ctx.use_lane(q % nl);
-        moves[m]->begin(i, fork ? q : kSyncSlot, jobs[m][i]);
+        jobs[m][i].device_cut = dev;
+        moves[m]->begin(i, dev ? slot + q : (fork ? q : kSyncSlot), jobs[m][i]);
         if (!fork) projectors_finish(ctx, jobs[m][i], pp);
       }
     }
+    if (dev) slot += total;
     q = 0;
     for (size_t m = 0; m < moves.size(); ++m) {
       PUs[m].resize(moves[m]->L);
@@ -472,6 +510,7 @@ void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
       moves[m]->update(i, PUs[m], PLs[m]);
     }
   lanes.end();
+  ctx.move_issued();
 }
 
 using Arr = std::vector<tensor>;
@@ -652,6 +691,18 @@ MoveSpec bottom_spec(const Arr &C1, const Arr &C2, Arr &C3, Arr &C4,
 
 }  // namespace
 
+// One move on its own (the public move functions): with the cut on the
+// device the move ends with a synchronisation that reads its projectors'
+// statistics.
+static void run_single_move(tn::Context &ctx, const PEPS_Parameters &pp, MoveSpec &m) {
+  int slot = 0;
+  run_moves(ctx, pp, {&m}, slot);
+  if (pp.projector_cut_on_device) {
+    ctx.sync();
+    ctx.collect_svd_stats();
+  }
+}
+
 // ctm.cpp:562-616
 void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
                const std::vector<tensor> &C3, std::vector<tensor> &C4,
@@ -660,7 +711,7 @@ void Left_move(std::vector<tensor> &C1, const std::vector<tensor> &C2,
                const std::vector<tensor> &Tn, int ix,
                const PEPS_Parameters &pp, const SquareLattice &lat) {
   MoveSpec m = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, pp, lat);
-  run_moves(*Tn[0].ctx(), pp, {&m});
+  run_single_move(*Tn[0].ctx(), pp, m);
 }
 
 // ctm.cpp:618-681
@@ -671,7 +722,7 @@ void Right_move(const std::vector<tensor> &C1, std::vector<tensor> &C2,
                 const std::vector<tensor> &Tn, int ix,
                 const PEPS_Parameters &pp, const SquareLattice &lat) {
   MoveSpec m = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, pp, lat);
-  run_moves(*Tn[0].ctx(), pp, {&m});
+  run_single_move(*Tn[0].ctx(), pp, m);
 }
 
 // ctm.cpp:683-746
@@ -682,7 +733,7 @@ void Top_move(std::vector<tensor> &C1, std::vector<tensor> &C2,
               const std::vector<tensor> &Tn, int iy, const PEPS_Parameters &pp,
               const SquareLattice &lat) {
   MoveSpec m = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, pp, lat);
-  run_moves(*Tn[0].ctx(), pp, {&m});
+  run_single_move(*Tn[0].ctx(), pp, m);
 }
 
 // ctm.cpp:748-813
@@ -693,7 +744,7 @@ void Bottom_move(const std::vector<tensor> &C1, const std::vector<tensor> &C2,
                  const std::vector<tensor> &eTl, const std::vector<tensor> &Tn,
                  int iy, const PEPS_Parameters &pp, const SquareLattice &lat) {
   MoveSpec m = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, pp, lat);
-  run_moves(*Tn[0].ctx(), pp, {&m});
+  run_single_move(*Tn[0].ctx(), pp, m);
 }
 
 namespace {
@@ -940,6 +991,7 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
   const int LX = lat.LX_noskew, LY = lat.LY_noskew;
   while (!convergence && count < pp.Max_CTM_Iteration) {
     const PEPS_Parameters &ps = pp;
+    int slot = 0;  // pinned slots of this sweep's projectors
     if (pp.CTM_Projector_corner) {
       // With corner projectors the left moves read and write only C1, C4,
       // eTl (and read eTt, eTb, Tn), the right moves only C2, C3, eTr: the
@@ -952,13 +1004,13 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
         MoveSpec a = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
         MoveSpec b = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
                                 (-s + 1 + LX) % LX, ps, lat);
-        run_moves(ctx, ps, {&a, &b});
+        run_moves(ctx, ps, {&a, &b}, slot);
       }
       for (int s = 0; s < LY; ++s) {
         MoveSpec a = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
                               (-s + 1 + LY) % LY, ps, lat);
         MoveSpec b = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
-        run_moves(ctx, ps, {&a, &b});
+        run_moves(ctx, ps, {&a, &b}, slot);
       }
     } else {
       for (int ix = 0; ix < LX; ++ix)
@@ -974,6 +1026,9 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
     }
     convergence = Check_Convergence_CTM(C1, C2, C3, C4, C1_old, C2_old,
                                         C3_old, C4_old, pp, lat, sig_max);
+    // the check synchronised the stream: the sweep's projector statistics
+    // (and warm-start ranges) are in their pinned slots
+    Tn[0].ctx()->collect_svd_stats();
     count += 1;
     C1_old = C1;
     C2_old = C2;

@@ -60,6 +60,16 @@ const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
 // Elementwise / reduction helpers
 // ---------------------------------------------------------------------------
 void fill(double *p, int64_t n, double v, hipStream_t s);
+// Pinned host slot (device address dst, 512 doubles): dst[0:n] = src[0:n]
+// (n <= 511) and the int *count (if given) in the first 4 bytes of dst[511].
+void write_slot(double *dst, const double *src, int n, const int *count,
+                hipStream_t s);
+// p (n x n, row-major) = identity
+void eye(double *p, int64_t n, hipStream_t s);
+// sc[i] = s[i]^(-1/2) while s[i] / s[0] > eps, else 0, for i < k (s on the
+// device, descending): the projector truncation without a host round trip.
+void projector_scale(const double *s_dev, int k, double eps, double *sc,
+                     hipStream_t s);
 // dst[dst_off(i)] = src[src_off(i)] for a rank<=8 row-major index space
 // `shape` (used for transpose / slice / extend copies).
 void strided_copy(const double *src, double *dst, int rank,

@@ -17,6 +17,35 @@ __global__ void fill_kernel(double *p, int64_t n, double v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// p (n x n) = identity
+__global__ void eye_kernel(double *p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n * n; i += stride) p[i] = (i / n == i % n) ? 1.0 : 0.0;
+}
+
+// dst[0:n] = src[0:n] and the int *count into the bit pattern of dst[511]:
+// one dispatch writing a pinned host slot through its device address
+// (instead of DMA copies).
+__global__ void write_slot_kernel(double *dst, const double *src, int n,
+                                  const int *count) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  if (threadIdx.x == 0 && count) {
+    const int c = *count;
+    reinterpret_cast<int *>(dst + 511)[0] = c;
+  }
+}
+
+// Projector cut on the device (the truncation of ctm.cpp:275-288 for a
+// descending spectrum): sc[i] = s[i]^(-1/2) while s[i] / s[0] > eps, else 0.
+__global__ void projector_scale_kernel(const double *s, int k, double eps,
+                                       double *sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const double s0 = s[0];
+  sc[i] = (s0 > 0.0 && s[i] / s0 > eps) ? 1.0 / sqrt(s[i]) : 0.0;
+}
+
 struct CopyGeom {
   int rank;
   int64_t shape[8];
@@ -149,6 +178,28 @@ inline int grid_for(int64_t n) {
 }
 
 }  // namespace
+
+void write_slot(double *dst, const double *src, int n, const int *count,
+                hipStream_t s) {
+  TN_REQUIRE(n >= 0 && n <= 511, "write_slot: at most 511 values");
+  hipLaunchKernelGGL(write_slot_kernel, dim3(1), dim3(256), 0, s, dst, src, n,
+                     count);
+  TN_HIP_CHECK(hipGetLastError());
+}
+
+void eye(double *p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(eye_kernel, dim3(grid_for(n * n)), dim3(TPB), 0, s, p, n);
+  TN_HIP_CHECK(hipGetLastError());
+}
+
+void projector_scale(const double *s_dev, int k, double eps, double *sc,
+                     hipStream_t s) {
+  if (k <= 0) return;
+  hipLaunchKernelGGL(projector_scale_kernel, dim3((k + 255) / 256), dim3(256),
+                     0, s, s_dev, k, eps, sc);
+  TN_HIP_CHECK(hipGetLastError());
+}
 
 void fill(double *p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return;

@@ -107,7 +107,7 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
 
 SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
                          const SvdOptions &opt, int slot) {
-  TN_REQUIRE(slot >= 0 && slot < 63, "svd_top_begin: pinned slot 0..62");
+  TN_REQUIRE(slot >= 0 && slot < Context::kPinnedSlots, "svd_top_begin: bad pinned slot");
   ContractionPrecision fp64(ctx, false);  // the SVD and its QRs stay FP64
   TN_REQUIRE(A.rank() == 2, "svd_top: 2-D input expected");
   const int m = (int)A.shape[0], n = (int)A.shape[1];
@@ -220,13 +220,10 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
   }
   r.s_dev = slice(jo.sigma_sorted, 0, 0, k);
-  // pinned slot: all <= 128 singular values, then the Jacobi round count
-  double *host = ctx.pinned(slot);
-  TN_HIP_CHECK(hipMemcpyAsync(host, jo.sigma_sorted.data(),
-                              (size_t)cols * sizeof(double),
-                              hipMemcpyDeviceToHost, s));
-  TN_HIP_CHECK(hipMemcpyAsync(host + 511, jo.nsteps->ptr, sizeof(int),
-                              hipMemcpyDeviceToHost, s));
+  // pinned slot: all <= 128 singular values, then the Jacobi round count,
+  // written by one kernel through the slot's device address
+  kern::write_slot(ctx.pinned_device(slot), jo.sigma_sorted.data(), cols,
+                   jo.nsteps->ptr, s);
   pd.ctx = &ctx;
   pd.stream = s;
   pd.slot = slot;
@@ -249,9 +246,16 @@ SvdResult svd_top_end(SvdPending &pd) {
   return std::move(pd.r);
 }
 
+SvdResult svd_top_detach(SvdPending &pd) {
+  TN_REQUIRE(pd.ctx != nullptr, "svd_top_detach: no pending SVD");
+  pd.ctx->svd_calls++;
+  pd.ctx = nullptr;
+  return std::move(pd.r);
+}
+
 SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
                   const SvdOptions &opt) {
-  SvdPending pd = svd_top_begin(ctx, A, k_target, opt, 62);  // slot of synchronous calls
+  SvdPending pd = svd_top_begin(ctx, A, k_target, opt, Context::kSlotSvdSync);
   return svd_top_end(pd);
 }
 

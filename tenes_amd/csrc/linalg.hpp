@@ -52,7 +52,7 @@ SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
 // The same in two halves, for overlapping independent SVDs on the lanes of a
 // forked Context: svd_top_begin issues all device work on the current stream
 // (U, V and s_dev included) and an asynchronous copy of s into pinned host
-// slot `slot` (0..62, distinct for SVDs in flight at the same time);
+// slot `slot` (distinct for SVDs in flight at the same time);
 // svd_top_end waits for that stream and fills s.
 struct SvdPending {
   SvdResult r;
@@ -65,6 +65,10 @@ struct SvdPending {
 SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
                          const SvdOptions &opt, int slot);
 SvdResult svd_top_end(SvdPending &pending);
+// The device results of a pending SVD without waiting for it (s and s_all
+// stay empty; the values are in the pinned slot once the stream has passed
+// the copy).
+SvdResult svd_top_detach(SvdPending &pending);
 
 // Singular values (descending, all min(m,n) of them) of a batch of equal
 // shaped matrices, all of them <= 128 x 128.  Result: device tensor

@@ -118,6 +118,7 @@ Context::~Context() {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : event_pool_) (void)hipEventDestroy(e);
+  for (auto e : inflight_) (void)hipEventDestroy(e);
   for (auto &kv : plans_) (void)hipFree(kv.second.d_tab);
   for (size_t i = 0; i < lanes_.size(); ++i) {
     if (lanes_[i].ws) alloc_.free(lanes_[i].ws, lanes_[i].ws_bytes);
@@ -192,6 +193,18 @@ void Context::join_noexcept() noexcept {
   alloc_.set_deferred(false);
 }
 
+void Context::move_issued() {
+  if (max_moves_ <= 0 || forked_) return;
+  hipEvent_t e = get_event();
+  TN_HIP_CHECK(hipEventRecord(e, lanes_[0].s));
+  inflight_.push_back(e);
+  while ((int)inflight_.size() > max_moves_) {
+    TN_HIP_CHECK(hipEventSynchronize(inflight_.front()));
+    event_pool_.push_back(inflight_.front());
+    inflight_.erase(inflight_.begin());
+  }
+}
+
 void Context::forget_svd_starts(int64_t environment) {
   for (auto it = svd_start.begin(); it != svd_start.end();) {
     if (it->first[0] == environment)
@@ -216,9 +229,34 @@ double *Context::scalars(int n) {
 }
 
 double *Context::pinned(int slot) {
-  TN_REQUIRE(slot >= 0 && slot < 64, "pinned slot out of range");
-  if (!pinned_) TN_HIP_CHECK(hipHostMalloc(&pinned_, 64 * 512 * sizeof(double)));
+  TN_REQUIRE(slot >= 0 && slot < kPinnedSlots, "pinned slot out of range");
+  if (!pinned_)
+    TN_HIP_CHECK(hipHostMalloc(&pinned_, (size_t)kPinnedSlots * 512 * sizeof(double)));
   return pinned_ + (size_t)slot * 512;
+}
+
+double *Context::pinned_device(int slot) {
+  pinned(slot);  // allocates on first use
+  if (!pinned_dev_) {
+    void *d = nullptr;
+    TN_HIP_CHECK(hipHostGetDevicePointer(&d, pinned_, 0));
+    pinned_dev_ = static_cast<double *>(d);
+  }
+  return pinned_dev_ + (size_t)slot * 512;
+}
+
+void Context::collect_svd_stats() {
+  for (const auto &p : pending_svd) {
+    const double *h = pinned(p.slot);
+    int rounds = 0;
+    std::memcpy(&rounds, h + 511, sizeof(int));
+    jacobi_rounds += rounds;
+    if (!p.warm) continue;
+    auto it = svd_start.find(p.wkey);
+    if (it != svd_start.end() && p.nall > 0)
+      it->second.range = h[p.nall - 1] > 0.0 ? h[0] / h[p.nall - 1] : 0.0;
+  }
+  pending_svd.clear();
 }
 
 void Context::sync() {

@@ -128,13 +128,38 @@ class Context {
   // kernel on the main stream in issue order).
   int max_lanes() const { return max_lanes_; }
   void set_max_lanes(int n) { max_lanes_ = n < 1 ? 1 : (n > 16 ? 16 : n); }
+  // Host run-ahead bound of the CTM driver: after issuing a move (or a pair
+  // of moves) the host waits until at most n issued moves are unfinished on
+  // the device (0: no bound).
+  int max_moves_in_flight() const { return max_moves_; }
+  void set_max_moves_in_flight(int n) { max_moves_ = n < 0 ? 0 : n; }
+  void move_issued();
 
   // Scratch of the current stream (grows on demand, stream ordered).
   void *workspace(size_t bytes);
   double *scalars(int n);  // small device scalar area
   // Pinned host buffer for asynchronous device-to-host copies of small
-  // results (slot < 64, at most 512 doubles each).
+  // results (slot < kPinnedSlots, at most 512 doubles each).  The last two
+  // slots belong to the synchronous SVD and projector entry points.
+  static constexpr int kPinnedSlots = 256;
+  static constexpr int kSlotProjectorSync = kPinnedSlots - 2;
+  static constexpr int kSlotSvdSync = kPinnedSlots - 1;
   double *pinned(int slot);
+  // The same slot's address for device code (kernels write it directly).
+  double *pinned_device(int slot);
+  // Projector SVDs whose singular values were left in pinned slots without
+  // waiting for them (projector cut on the device): after the next
+  // synchronisation of the stream collect_svd_stats() reads them, sets the
+  // Ritz-value range of the warm-start entry (when `warm`) and adds the
+  // Jacobi rounds to the statistics.
+  struct PendingSvdStat {
+    std::array<int64_t, 5> wkey;
+    bool warm;
+    int slot;
+    int nall;
+  };
+  std::vector<PendingSvdStat> pending_svd;
+  void collect_svd_stats();
 
   const ContractPlan &plan(const Shape &sa, const Shape &sb, const Axes &axa,
                            const Axes &axb, const Axes &perm);
@@ -212,9 +237,12 @@ class Context {
   int cur_ = 0;
   int nforked_ = 0;
   int max_lanes_ = 8;
+  int max_moves_ = 0;
+  std::vector<hipEvent_t> inflight_;  // end of each issued move, oldest first
   bool forked_ = false;
   hipEvent_t fork_event_ = nullptr;
   double *pinned_ = nullptr;
+  double *pinned_dev_ = nullptr;
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
   bool fp32_ = false;

@@ -46,3 +46,12 @@ def random_tensors(D: int, d: int = 2, seed: int = 7, n: int = 4):
 
 def spin_ops():
     return {"Sz": models.Sz, "Sx": models.Sx}
+
+
+def spectra_diff(a, b):
+    """Largest difference of two descending spectra, the shorter padded with
+    zeros (with the projector cut on the device the GPU environment keeps CHI
+    columns where the reference's shrinks: the extra singular values are 0)."""
+    a, b = np.asarray(a), np.asarray(b)
+    n = max(a.size, b.size)
+    return float(np.max(np.abs(np.pad(a, (0, n - a.size)) - np.pad(b, (0, n - b.size)))))

@@ -14,7 +14,7 @@ import tenes_amd as ta
 from oracle import contract as K, ctm as O, itps, transfer as TMo
 from oracle.lattice import SquareLattice
 
-from ctm_cases import random_tensors, su_tensors
+from ctm_cases import random_tensors, spectra_diff, su_tensors
 from oracle import models
 
 pytestmark = pytest.mark.gpu
@@ -209,14 +209,17 @@ def test_ctm_environment_observables(gpu_ctx, model, D, chi, sweeps):
         for k in ("C1", "C2", "C3", "C4"):
             ref = O.normalized_spectrum(getattr(env, k)[i])
             got = genv.corner_spectrum(k, i)
-            m = min(ref.size, got.size)
-            assert np.max(np.abs(got[:m] - ref[:m])) < 1e-10
+            assert spectra_diff(got, ref) < 1e-10
 
 
-def test_product_state_closed_form(gpu_ctx):
+@pytest.mark.parametrize("device_cut", [1, 0])
+def test_product_state_closed_form(gpu_ctx, device_cut):
     """Embedded product state (nonzero only at virtual index 0): the
     environment is exact at any D and chi, <S> and the bond energy take their
-    product-state values (D=5, chi=100 shapes)."""
+    product-state values (D=5, chi=100 shapes).  Every projector keeps one
+    direction (s_1/s_0 = 0): with the cut on the host the environment shrinks
+    to the reference's chi_eff = 1; with the cut on the device it stays
+    100 x 100 with zeros outside that direction, and the observables agree."""
     v = np.array([np.cos(0.3), np.sin(0.3)])
     Tn = []
     for _ in range(4):
@@ -225,7 +228,8 @@ def test_product_state_closed_form(gpu_ctx):
         Tn.append(T)
     genv = ta.Environment(gpu_ctx, 2, 2)
     genv.set_site_tensors(Tn)
-    genv.calc_ctm_environment(ta.peps_parameters(CHI=100, Max_CTM_Iteration=3, CTM_Convergence_Epsilon=0.0))
+    genv.calc_ctm_environment(ta.peps_parameters(CHI=100, Max_CTM_Iteration=3, CTM_Convergence_Epsilon=0.0,
+                                                 projector_cut_on_device=device_cut))
     ham = models.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
     E, mz = observables_gpu(genv, SquareLattice(2, 2), ham)
     sz = v @ models.Sz @ v
@@ -233,6 +237,40 @@ def test_product_state_closed_form(gpu_ctx):
     assert abs(mz[0] - sz) < 1e-13
     # S.S on |v v>: <Sz>^2 + <Sx>^2 (+ <Sy>^2 = 0 for a real v), 2 bonds/site
     assert abs(E - 2 * (sz * sz + sx * sx)) < 1e-13
+    C = genv.get("C1", 0).numpy()
+    assert C.shape == ((100, 100) if device_cut else (1, 1))
+    s = np.linalg.svd(C, compute_uv=False)
+    assert s[0] > 0 and np.all(s[1:] <= 1e-14 * s[0])
+
+
+@pytest.mark.parametrize("model,D,chi,sweeps,floor", [("heisenberg", 2, 10, 6, 0.0), ("heisenberg", 3, 20, 4, 0.0),
+                                                      ("tfi", 3, 20, 4, 1e-5)])
+def test_device_and_host_cut_agree(gpu_ctx, model, D, chi, sweeps, floor):
+    """The projector cut decided on the device (fixed CHI shapes, no host
+    round trip) and on the host (the reference's shapes): energies, <Sz> and
+    corner spectra agree to 1e-10, both with the oracle.  The TFI D=3, chi=20
+    environment has projector singular values at s/s0 ~ 7e-12, next to the
+    1e-12 cut, where any two FP64 SVDs may keep or drop the direction
+    (test_projector_left_block): a dropped one carries ~3e-6 of a corner
+    spectrum and nothing measurable of the observables, so there the spectra
+    are compared above 1e-5."""
+    Tn, lat, env, ham = oracle_env(model, D, chi, 60, sweeps)
+    E, mz = observables_oracle(env, Tn, lat, ham)
+    for dev in (1, 0):
+        genv = ta.Environment(gpu_ctx, 2, 2)
+        genv.set_site_tensors(Tn)
+        n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=chi, Max_CTM_Iteration=sweeps,
+                                                            CTM_Convergence_Epsilon=0.0,
+                                                            projector_cut_on_device=dev))
+        assert n == sweeps
+        gE, gmz = observables_gpu(genv, lat, ham)
+        assert abs(gE - E) < 1e-10 * abs(E), (dev, gE, E)
+        assert np.max(np.abs(gmz - mz)) < 1e-10, dev
+        for i in range(4):
+            for k in ("C1", "C2", "C3", "C4"):
+                a, b = genv.corner_spectrum(k, i), O.normalized_spectrum(getattr(env, k)[i])
+                a, b = a[a > floor], b[b > floor]
+                assert spectra_diff(a, b) < 1e-10, (dev, k, i)
 
 
 def test_random_tensors_spectra(gpu_ctx):
@@ -297,7 +335,7 @@ def test_ctm_environment_rsvd_updown(gpu_ctx):
         for k in ("C1", "C2", "C3", "C4"):
             a = O.normalized_spectrum(getattr(env, k)[i])
             b = O.normalized_spectrum(genv.get(k, i).numpy())
-            assert a.shape == b.shape and np.max(np.abs(a - b)) < 1e-10, (k, i)
+            assert spectra_diff(a, b) < 1e-10, (k, i)
     one, _, two, _ = G.operators(fx, 4, True)
     o, _ = itps.measure_onesite(env, Tn, lat, one)
     w, _ = itps.measure_twosite(env, Tn, lat, two, one)
@@ -471,8 +509,7 @@ def test_ctm_warm_start_rank_growth(gpu_ctx):
             for kind in ("C1", "C2", "C3", "C4"):
                 ref = O.normalized_spectrum(getattr(env, kind)[i])
                 got = genv.corner_spectrum(kind, i)
-                assert got.shape == ref.shape
-                worst = max(worst, float(np.max(np.abs(got - ref))))
+                worst = max(worst, spectra_diff(got, ref))
         assert worst < 1e-10, (split, worst)
 
 

@@ -14,7 +14,7 @@ import tenes_amd as ta
 from oracle import ctm as O, itps, models
 from oracle.lattice import SquareLattice
 
-from ctm_cases import random_tensors
+from ctm_cases import random_tensors, spectra_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,7 @@ def test_ctm_on_unit_cell(gpu_ctx, LX, LY, skew, corner):
         for k in ("C1", "C2", "C3", "C4"):
             a = O.normalized_spectrum(getattr(env, k)[i])
             b = O.normalized_spectrum(genv.get(k, i).numpy())
-            assert a.shape == b.shape and np.max(np.abs(a - b)) < 1e-9, (k, i)
+            assert spectra_diff(a, b) < 1e-9, (k, i)
     SzSz = np.kron(models.Sz, models.Sz).reshape(2, 2, 2, 2)
     one = [{"name": "Sz", "group": 0, "site": i, "op": models.Sz} for i in range(lat.N_UNIT)]
     two = [{"name": "SzSz", "group": 0, "source": s, "dx": dx, "dy": dy, "op": SzSz}
