@@ -124,6 +124,20 @@ int tenes_transpose(const tenes_tensor *T, const int *axes, tenes_tensor **out);
  * *k_out receives the number returned. */
 int tenes_svd_top(const tenes_tensor *A, int k, int power_iters,
                   tenes_tensor **U, tenes_tensor **V, double *s, int *k_out);
+/* tenes_svd_top for a matrix of at most 128 x 128 (its exact one-sided
+ * Jacobi path) that allocates no device memory.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  A: the m x n
+ * row-major device matrix; U (m x k'), V (n x k') and s (k') are
+ * caller-owned device buffers, k' = min(k, m, n) returned in *k_out.  Work
+ * is issued on the context's stream without synchronising.  The scratch is
+ * a fixed per-device array of 4166400 bytes set up when the first context
+ * is created on the device; m or n above 128 returns TENES_ERR_UNSUPPORTED.
+ * Calls that overlap on one device (several streams or contexts) share the
+ * array and race: order them on one stream. */
+int tenes_svd_top_noalloc(tenes_ctx *ctx, const double *A, int m, int n,
+                          int k, double *U, double *V, double *s, int *k_out);
+
 /* ---- parameters (PEPS_Parameters.hpp:29-94, CTM fields) ------------------ */
 typedef struct tenes_peps_parameters {
   int CHI;

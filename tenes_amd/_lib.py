@@ -114,6 +114,9 @@ _SIGS = {
     "tenes_ctx_set_fp32_contractions": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_lane_state": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "tenes_ctx_set_max_moves_in_flight": (C.c_int, [C.c_void_p, C.c_int]),
+    "tenes_svd_top_noalloc": (
+        C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                  c_int_p]),
     "tenes_batch_calc_ctm_environment": (
         C.c_int, [c_void_pp, C.c_int, C.POINTER(PepsParameters), C.c_int, C.c_int, c_int_p, c_dbl_p]),
     "tenes_fu_parameters_default": (None, [C.POINTER(FuParameters)]),

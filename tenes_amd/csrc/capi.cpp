@@ -16,6 +16,7 @@
 #include "ctm.hpp"
 #include "full_update.hpp"
 #include "linalg.hpp"
+#include "noalloc.hpp"
 #include "simple_update.hpp"
 #include "small_dense.hpp"
 #include "transfer.hpp"
@@ -247,6 +248,9 @@ int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
     auto *c = new tenes_ctx;
     try {
       c->c.reset(new tn::Context(device, static_cast<hipStream_t>(hip_stream)));
+      // the allocation-free entry point's scratch address on this device
+      // (the context constructor made the device current)
+      tn::noalloc::init_device(device);
     } catch (...) {
       delete c;
       throw;
@@ -447,6 +451,16 @@ int tenes_svd_top(const tenes_tensor *A, int k, int power_iters,
     if (V) *V = wrap(ctx, r.V);
     if (s) std::memcpy(s, r.s.data(), r.s.size() * sizeof(double));
     if (k_out) *k_out = (int)r.s.size();
+  });
+}
+
+int tenes_svd_top_noalloc(tenes_ctx *ctx, const double *A, int m, int n,
+                          int k, double *U, double *V, double *s,
+                          int *k_out) {
+  if (!ctx) return TENES_ERR_ARGUMENT;
+  return guard(ctx, [&] {
+    tn::noalloc::svd_small(ctx->c->device(), ctx->c->stream(), A, m, n, k, U,
+                           V, s, k_out);
   });
 }
 
