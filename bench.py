@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--host-cut", action="store_true",
                     help="decide the projector cut on the host (one synchronisation per projector, the "
                          "reference's shapes) instead of on the device")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="capture a sweep into a hipGraph and replay it (the projector cut on the device)")
     ap.add_argument("--moves-in-flight", type=int, default=None,
                     help="bound on CTM moves issued by the host but unfinished on the device (0: none)")
     ap.add_argument("--probe-gemm", action="store_true",
@@ -302,6 +304,8 @@ def main():
     kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)
     if args.host_cut:
         kw["projector_cut_on_device"] = 0
+    if args.graph:
+        kw["graph_sweeps"] = 1
     if args.power_iters is not None:
         kw["svd_power_iters"] = args.power_iters
     if args.warm_iters is not None:
@@ -318,6 +322,8 @@ def main():
     flops, ncalls = ctx.stats()
     nsvd, nrounds = ctx.svd_stats()
     sweeps_per_s = args.steps / dt
+    if args.graph:
+        log(f"sweep graphs (captures, replays) so far: {ctx.graph_stats()}")
     log(f"timed: {args.steps} sweeps in {dt:.3f} s -> {sweeps_per_s:.3f} sweeps/s "
         f"({SWEEP_FLOP_D5_CHI100 * sweeps_per_s / 1e12:.2f} TFLOP/s algorithmic); {ncalls} GEMMs "
         f"({flops / dt / 1e12:.2f} TFLOP/s issued); {nsvd} truncated SVDs, "

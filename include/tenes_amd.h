@@ -62,6 +62,9 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx);
  * order on the context stream).  No counterpart in the reference, whose
  * moves loop over the rows (src/iTPS/core/ctm.cpp:577-615). */
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n);
+/* Sweep graphs captured and replayed through this context so far
+ * (tenes_peps_parameters.graph_sweeps).  No counterpart in the reference. */
+int tenes_ctx_graph_stats(tenes_ctx *ctx, int64_t *captures, int64_t *replays);
 /* Host run-ahead bound of the CTM driver and moves: after issuing a move
  * (or the pair of moves the driver overlaps) the host waits until at most n
  * issued moves are unfinished on the device; 0 (default): no bound.  No
@@ -176,6 +179,15 @@ typedef struct tenes_peps_parameters {
    * rows and columns instead: the same network, the same observables).  The
    * standalone projector entry points always cut on the host. */
   int projector_cut_on_device;
+  /* Sweeps of tenes_calc_ctm_environment with corner projectors captured
+   * into a hipGraph and replayed (no counterpart in the reference; implies
+   * the projector cut on the device).  The first sweep in which every
+   * projector has its warm-start basis is captured; later sweeps replay it
+   * while the environment's buffers, shapes and range-finder decisions stay
+   * the same (else it is captured again) and while the environment alone
+   * holds its tensors (a replay overwrites them in place); other sweeps run
+   * as usual.  Default 0. */
+  int graph_sweeps;
 } tenes_peps_parameters;
 /* Defaults of PEPS_Parameters.cpp:29-77. */
 void tenes_peps_parameters_default(tenes_peps_parameters *p);

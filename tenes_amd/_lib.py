@@ -34,6 +34,7 @@ class PepsParameters(C.Structure):
         ("svd_warm_power_iters", C.c_int),
         ("contraction_fp32", C.c_int),
         ("projector_cut_on_device", C.c_int),
+        ("graph_sweeps", C.c_int),
     ]
 
 
@@ -114,6 +115,7 @@ _SIGS = {
     "tenes_ctx_set_fp32_contractions": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_lane_state": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "tenes_ctx_set_max_moves_in_flight": (C.c_int, [C.c_void_p, C.c_int]),
+    "tenes_ctx_graph_stats": (C.c_int, [C.c_void_p, c_i64_p, c_i64_p]),
     "tenes_svd_top_noalloc": (
         C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                   c_int_p]),

@@ -55,6 +55,12 @@ class Context:
         """Bound on CTM moves issued but unfinished on the device (0: none)."""
         self.check(self.lib.tenes_ctx_set_max_moves_in_flight(self.h, int(n)))
 
+    def graph_stats(self):
+        """(captures, replays) of CTM sweep graphs through this context."""
+        c, r = C.c_int64(), C.c_int64()
+        self.check(self.lib.tenes_ctx_graph_stats(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
+
     def lane_state(self):
         """(forked, lane): whether work is split over side streams and the
         stream new work goes to (-1: the context stream)."""

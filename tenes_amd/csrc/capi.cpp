@@ -89,6 +89,7 @@ PEPS_Parameters to_params(const tenes_peps_parameters *p) {
   q.svd_warm_power_iters = p->svd_warm_power_iters;
   q.contraction_fp32 = p->contraction_fp32;
   q.projector_cut_on_device = p->projector_cut_on_device != 0;
+  q.graph_sweeps = p->graph_sweeps != 0;
   TN_REQUIRE(q.CHI >= 1, "CHI must be positive");
   TN_REQUIRE(q.RSVD_Oversampling_factor >= 1.0,
              "RSVD oversampling factor must be >= 1");
@@ -240,7 +241,7 @@ double bond_cluster_value(tenes_env *env, const Cluster &c,
 
 extern "C" {
 
-int tenes_abi_version(void) { return 206; }
+int tenes_abi_version(void) { return 207; }
 
 int tenes_ctx_create(int device, void *hip_stream, tenes_ctx **out) {
   return guard(nullptr, [&] {
@@ -306,6 +307,14 @@ int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {
   return guard(ctx, [&] {
     TN_REQUIRE(n >= 1, "max lanes must be >= 1");
     ctx->c->set_max_lanes(n);
+  });
+}
+
+int tenes_ctx_graph_stats(tenes_ctx *ctx, int64_t *captures,
+                          int64_t *replays) {
+  return guard(ctx, [&] {
+    if (captures) *captures = ctx->c->graph_captures;
+    if (replays) *replays = ctx->c->graph_replays;
   });
 }
 
@@ -481,6 +490,7 @@ void tenes_peps_parameters_default(tenes_peps_parameters *p) {
   p->svd_warm_power_iters = q.svd_warm_power_iters;
   p->contraction_fp32 = q.contraction_fp32;
   p->projector_cut_on_device = q.projector_cut_on_device ? 1 : 0;
+  p->graph_sweeps = q.graph_sweeps ? 1 : 0;
 }
 
 int tenes_calc_projector_left_block(

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <functional>
 #include <limits>
 #include <map>
@@ -970,6 +971,163 @@ void Initialize_CTM(std::vector<tensor> &C1, std::vector<tensor> &C2,
   }
 }
 
+namespace {
+
+// The moves of one sweep with corner projectors.  The left moves read and
+// write only C1, C4, eTl (and read eTt, eTb, Tn), the right moves only C2,
+// C3, eTr: the two chains of ctm.cpp:1205-1214 are independent and run side
+// by side, step s of one with step s of the other; likewise the top (C1, C2,
+// eTt) and bottom (C3, C4, eTb) chains of ctm.cpp:1215-1224.  Each chain
+// keeps the reference's order, so the result is the same.
+void corner_sweep(std::vector<tensor> *A[8], const std::vector<tensor> &Tn,
+                  const PEPS_Parameters &ps, const SquareLattice &lat) {
+  auto &C1 = *A[0], &C2 = *A[1], &C3 = *A[2], &C4 = *A[3];
+  auto &eTt = *A[4], &eTr = *A[5], &eTb = *A[6], &eTl = *A[7];
+  tn::Context &ctx = *Tn[0].ctx();
+  const int LX = lat.LX_noskew, LY = lat.LY_noskew;
+  int slot = 0;  // pinned slots of this sweep's projectors
+  for (int s = 0; s < LX; ++s) {
+    MoveSpec a = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
+    MoveSpec b = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
+                            (-s + 1 + LX) % LX, ps, lat);
+    run_moves(ctx, ps, {&a, &b}, slot);
+  }
+  for (int s = 0; s < LY; ++s) {
+    MoveSpec a = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
+                          (-s + 1 + LY) % LY, ps, lat);
+    MoveSpec b = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
+    run_moves(ctx, ps, {&a, &b}, slot);
+  }
+}
+
+// What a captured sweep of an environment bakes in: the buffers and shapes
+// of every tensor it reads and writes, the warm-start bases with the
+// range-finder decision each basis's Ritz-value range implies (svd_top_begin
+// skips the orthonormalisation between the A^T and A products below 1e7),
+// and the parameters that shape the kernels.
+std::vector<uintptr_t> sweep_signature(tn::Context &ctx, int64_t env,
+                                       std::vector<tensor> *A[8],
+                                       const std::vector<tensor> &Tn,
+                                       const PEPS_Parameters &pp) {
+  std::vector<uintptr_t> sig;
+  auto add = [&](const std::vector<tensor> &arr) {
+    for (const auto &t : arr) {
+      sig.push_back((uintptr_t)t.data());
+      for (auto d : t.shape) sig.push_back((uintptr_t)d);
+      sig.push_back(~(uintptr_t)0);
+    }
+  };
+  for (int k = 0; k < 8; ++k) add(*A[k]);
+  add(Tn);
+  for (const auto &kv : ctx.svd_start) {
+    if (kv.first[0] != env) continue;
+    for (auto x : kv.first) sig.push_back((uintptr_t)x);
+    sig.push_back((uintptr_t)kv.second.V.data());
+    sig.push_back((uintptr_t)kv.second.s.data());
+    sig.push_back(kv.second.range > 0.0 && kv.second.range < 1e7);
+  }
+  uint64_t cut;
+  std::memcpy(&cut, &pp.Inverse_projector_cut, sizeof(cut));
+  for (uintptr_t x : {(uintptr_t)pp.CHI, (uintptr_t)cut, (uintptr_t)pp.svd_power_iters,
+                      (uintptr_t)pp.rsvd_power_iters, (uintptr_t)pp.svd_warm_start,
+                      (uintptr_t)pp.svd_warm_power_iters, (uintptr_t)pp.contraction_fp32,
+                      (uintptr_t)pp.Use_RSVD, (uintptr_t)ctx.max_lanes()})
+    sig.push_back(x);
+  return sig;
+}
+
+// A sweep can be captured (or replayed) when every projector already has its
+// warm-start basis (or warm starts are off) and the environment exclusively
+// owns its tensors: a replay writes its results into those buffers, which no
+// other handle may see change.
+bool sweep_graph_ready(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
+                       const SquareLattice &lat, const PEPS_Parameters &pp) {
+  for (int k = 0; k < 8; ++k)
+    for (const auto &t : *A[k])
+      if (!t.defined() || t.st.use_count() != 1) return false;
+  if (!pp.svd_warm_start) return true;
+  int n = 0;
+  for (const auto &kv : ctx.svd_start) n += kv.first[0] == env;
+  return n == 4 * lat.LX_noskew * lat.LY_noskew;
+}
+
+// Captures one sweep of the environment into a graph: the sweep is issued
+// into the capture with every allocation from a private arena, then its
+// results (the eight environment arrays and the warm-start bases) are
+// copied into the buffers the sweep started from, so each replay reads and
+// writes the same addresses.  Returns false (nothing captured, nothing run)
+// when the capture fails; the arrays are restored in any case.
+bool capture_sweep(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
+                   const std::vector<tensor> &Tn, const PEPS_Parameters &ps,
+                   const SquareLattice &lat) {
+  tn::Context::SweepGraph g;
+  g.arena = ctx.allocator().arena_create();
+  std::vector<tensor> in[8];
+  for (int k = 0; k < 8; ++k) in[k] = *A[k];
+  std::map<std::array<int64_t, 5>, tn::Context::SvdStart> starts;
+  for (const auto &kv : ctx.svd_start)
+    if (kv.first[0] == env) starts.insert(kv);
+  const double flops0 = ctx.gemm_flops;
+  const int64_t calls0 = ctx.gemm_calls, svd0 = ctx.svd_calls;
+  const size_t pend0 = ctx.pending_svd.size();
+  bool ok = true;
+  try {
+    ctx.begin_capture();
+  } catch (...) {
+    ctx.allocator().arena_destroy(g.arena);
+    return false;
+  }
+  ctx.allocator().arena_activate(g.arena);
+  try {
+    corner_sweep(A, Tn, ps, lat);
+    hipStream_t s = ctx.stream();
+    auto copy_back = [&](const tensor &dst, const tensor &src) {
+      if (dst.st == src.st) return;
+      TN_REQUIRE(dst.size() == src.size(), "captured sweep: a tensor changed its size");
+      TN_HIP_CHECK(hipMemcpyAsync(dst.data(), src.data(), (size_t)dst.size() * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+    };
+    for (int k = 0; k < 8; ++k)
+      for (size_t i = 0; i < in[k].size(); ++i) copy_back(in[k][i], (*A[k])[i]);
+    for (const auto &kv : starts) {
+      const auto &now = ctx.svd_start.at(kv.first);
+      copy_back(kv.second.V, now.V);
+      copy_back(kv.second.s, now.s);
+    }
+  } catch (...) {
+    ok = false;
+  }
+  ctx.allocator().arena_activate(-1);
+  hipGraph_t graph = ctx.end_capture();
+  for (int k = 0; k < 8; ++k) *A[k] = in[k];
+  for (const auto &kv : starts) ctx.svd_start[kv.first] = kv.second;
+  std::vector<tn::Context::PendingSvdStat> pend(ctx.pending_svd.begin() + pend0,
+                                                ctx.pending_svd.end());
+  ctx.pending_svd.resize(pend0);
+  g.gemm_flops = ctx.gemm_flops - flops0;
+  g.gemm_calls = ctx.gemm_calls - calls0;
+  g.svd_calls = ctx.svd_calls - svd0;
+  ctx.gemm_flops = flops0;
+  ctx.gemm_calls = calls0;
+  ctx.svd_calls = svd0;
+  if (ok && graph) {
+    ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) (void)hipGetLastError();
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    ctx.allocator().arena_destroy(g.arena);
+    return false;
+  }
+  g.pending = std::move(pend);
+  g.signature = sweep_signature(ctx, env, A, Tn, ps);
+  ctx.sweep_graphs[env] = std::move(g);
+  ctx.graph_captures++;
+  return true;
+}
+
+}  // namespace
+
 // ctm.cpp:971-1253
 int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
                          std::vector<tensor> &C3, std::vector<tensor> &C4,
@@ -978,41 +1136,59 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
                          const std::vector<tensor> &Tn,
                          const PEPS_Parameters &pp, const SquareLattice &lat,
                          bool initialize, double *sig_max_out) {
-  // a fresh environment forgets the warm-start bases of its projectors; a
-  // continued one keeps them
+  tn::Context &ctx = *Tn[0].ctx();
+  const int64_t env = environment_id(C1);
+  // a fresh environment forgets the warm-start bases of its projectors (and
+  // its captured sweep); a continued one keeps them
   if (initialize) {
-    Tn[0].ctx()->forget_svd_starts(environment_id(C1));
+    ctx.forget_svd_starts(env);
     Initialize_CTM(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, pp, lat);
   }
+  // captured sweeps need fixed shapes: the projector cut on the device
+  PEPS_Parameters ps = pp;
+  const bool graphs = pp.graph_sweeps && pp.CTM_Projector_corner && !ctx.profiling() &&
+                      !ctx.autotune() && !ctx.single_layer();
+  if (graphs) ps.projector_cut_on_device = true;
+  std::vector<tensor> *A[8] = {&C1, &C2, &C3, &C4, &eTt, &eTr, &eTb, &eTl};
   bool convergence = false;
   int count = 0;
-  std::vector<tensor> C1_old = C1, C2_old = C2, C3_old = C3, C4_old = C4;
+  // corners of the previous sweep: handles, or copies when a replay may
+  // write the current buffers in place
+  std::vector<tensor> C_old[4];
+  auto keep_old = [&] {
+    for (int c = 0; c < 4; ++c) {
+      C_old[c] = *A[c];
+      if (graphs)
+        for (auto &t : C_old[c]) t = tn::clone(t);
+    }
+  };
+  keep_old();
   double sig_max = 0.0;
   const int LX = lat.LX_noskew, LY = lat.LY_noskew;
   while (!convergence && count < pp.Max_CTM_Iteration) {
-    const PEPS_Parameters &ps = pp;
-    int slot = 0;  // pinned slots of this sweep's projectors
-    if (pp.CTM_Projector_corner) {
-      // With corner projectors the left moves read and write only C1, C4,
-      // eTl (and read eTt, eTb, Tn), the right moves only C2, C3, eTr: the
-      // two chains of ctm.cpp:1205-1214 are independent and run side by
-      // side, step s of one with step s of the other; likewise the top
-      // (C1, C2, eTt) and bottom (C3, C4, eTb) chains of ctm.cpp:1215-1224.
-      // Each chain keeps the reference's order, so the result is the same.
-      tn::Context &ctx = *Tn[0].ctx();
-      for (int s = 0; s < LX; ++s) {
-        MoveSpec a = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
-        MoveSpec b = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
-                                (-s + 1 + LX) % LX, ps, lat);
-        run_moves(ctx, ps, {&a, &b}, slot);
+    bool done = false;
+    if (graphs && sweep_graph_ready(ctx, env, A, lat, ps)) {
+      auto it = ctx.sweep_graphs.find(env);
+      if (it == ctx.sweep_graphs.end() ||
+          it->second.signature != sweep_signature(ctx, env, A, Tn, ps)) {
+        ctx.drop_sweep_graph(env);
+        capture_sweep(ctx, env, A, Tn, ps, lat);
+        it = ctx.sweep_graphs.find(env);
       }
-      for (int s = 0; s < LY; ++s) {
-        MoveSpec a = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
-                              (-s + 1 + LY) % LY, ps, lat);
-        MoveSpec b = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
-        run_moves(ctx, ps, {&a, &b}, slot);
+      if (it != ctx.sweep_graphs.end()) {
+        const auto &g = it->second;
+        TN_HIP_CHECK(hipGraphLaunch(g.exec, ctx.stream()));
+        ctx.graph_replays++;
+        ctx.pending_svd.insert(ctx.pending_svd.end(), g.pending.begin(), g.pending.end());
+        ctx.gemm_flops += g.gemm_flops;
+        ctx.gemm_calls += g.gemm_calls;
+        ctx.svd_calls += g.svd_calls;
+        done = true;
       }
-    } else {
+    }
+    if (!done && ps.CTM_Projector_corner) {
+      corner_sweep(A, Tn, ps, lat);
+    } else if (!done) {
       for (int ix = 0; ix < LX; ++ix)
         Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, ps, lat);
       for (int ix = 0; ix > -LX; --ix)
@@ -1024,16 +1200,13 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
       for (int iy = 0; iy < LY; ++iy)
         Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, ps, lat);
     }
-    convergence = Check_Convergence_CTM(C1, C2, C3, C4, C1_old, C2_old,
-                                        C3_old, C4_old, pp, lat, sig_max);
+    convergence = Check_Convergence_CTM(C1, C2, C3, C4, C_old[0], C_old[1],
+                                        C_old[2], C_old[3], pp, lat, sig_max);
     // the check synchronised the stream: the sweep's projector statistics
     // (and warm-start ranges) are in their pinned slots
-    Tn[0].ctx()->collect_svd_stats();
+    ctx.collect_svd_stats();
     count += 1;
-    C1_old = C1;
-    C2_old = C2;
-    C3_old = C3;
-    C4_old = C4;
+    keep_old();
     if (pp.print_level >= 3)
       std::printf("CTM: count, sig_max %d %.17g\n", count, sig_max);
   }

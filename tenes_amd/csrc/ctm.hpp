@@ -60,6 +60,13 @@ struct PEPS_Parameters {
   // 20.5 on the host, with the same kernels; the fixed shapes are what a
   // captured sweep needs.
   bool projector_cut_on_device = false;
+  // Sweeps of the environment driver with corner projectors captured into a
+  // hipGraph (no counterpart in the reference): the cut goes on the device,
+  // the first sweep with every projector warm-started is captured, later
+  // sweeps replay the graph while the environment's buffers, shapes and
+  // warm-start decisions stay the same (else it is captured again), and a
+  // sweep the graph cannot cover runs as usual.
+  bool graph_sweeps = false;
 };
 
 // Identity of an environment for the warm-start bases of its projectors:

@@ -22,9 +22,31 @@ size_t Allocator::round(size_t bytes) {
   return ((bytes + q - 1) / q) * q;
 }
 
+void *Allocator::take(std::multimap<size_t, void *> &pool, size_t b) {
+  auto it = pool.lower_bound(b);
+  if (it == pool.end() || it->first > b + b / 4) return nullptr;
+  void *p = it->second;
+  pool.erase(it);
+  return p;
+}
+
 void *Allocator::alloc(size_t bytes) {
   std::lock_guard<std::mutex> lk(mu_);
   const size_t b = round(bytes);
+  if (active_ >= 0) {
+    Arena &a = arenas_[active_];
+    if (void *p = take(a.free, b)) return p;
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error("device allocation of " + std::to_string(b) +
+                      " bytes for a captured graph failed: " + hipGetErrorString(e),
+                  -4);
+    }
+    owner_[p] = {active_, b};
+    return p;
+  }
   auto it = free_.lower_bound(b);
   if (it != free_.end() && it->first <= b + b / 4) {
     void *p = it->second;
@@ -57,6 +79,15 @@ void Allocator::free(void *p, size_t bytes) {
   if (!p) return;
   std::lock_guard<std::mutex> lk(mu_);
   const size_t b = round(bytes);
+  auto ow = owner_.find(p);
+  if (ow != owner_.end()) {
+    Arena &a = arenas_[ow->second.first];
+    if (defer_)
+      a.deferred.emplace_back(b, p);
+    else
+      a.free.emplace(b, p);
+    return;
+  }
   in_use_ -= b;
   cached_ += b;
   if (defer_)
@@ -71,7 +102,54 @@ void Allocator::set_deferred(bool on) {
   if (!on) {
     for (auto &kv : deferred_) free_.emplace(kv.first, kv.second);
     deferred_.clear();
+    for (auto &ar : arenas_) {
+      for (auto &kv : ar.second.deferred) ar.second.free.emplace(kv.first, kv.second);
+      ar.second.deferred.clear();
+    }
   }
+}
+
+int Allocator::arena_create() {
+  std::lock_guard<std::mutex> lk(mu_);
+  const int id = next_arena_++;
+  arenas_[id];
+  return id;
+}
+
+void Allocator::arena_activate(int id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  TN_REQUIRE(id < 0 || arenas_.count(id), "arena_activate: no such arena");
+  active_ = id;
+}
+
+void Allocator::arena_destroy(int id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = arenas_.find(id);
+  if (it == arenas_.end()) return;
+  if (active_ == id) active_ = -1;
+  // blocks still in use return to the general pool when they are freed
+  // (accounted as in use from now on), the free ones join it now
+  size_t idle = 0;
+  for (auto &kv : it->second.free) {
+    free_.emplace(kv.first, kv.second);
+    idle += kv.first;
+  }
+  for (auto &kv : it->second.deferred) {
+    deferred_.emplace_back(kv.first, kv.second);
+    idle += kv.first;
+  }
+  size_t all = 0;
+  for (auto ow = owner_.begin(); ow != owner_.end();) {
+    if (ow->second.first == id) {
+      all += ow->second.second;
+      ow = owner_.erase(ow);
+    } else {
+      ++ow;
+    }
+  }
+  cached_ += idle;
+  in_use_ += all - idle;
+  arenas_.erase(it);
 }
 
 void Allocator::release_cached() {
@@ -85,6 +163,10 @@ void Allocator::release_cached() {
 Allocator::~Allocator() {
   for (auto &kv : free_) (void)hipFree(kv.second);
   for (auto &kv : deferred_) (void)hipFree(kv.second);
+  for (auto &ar : arenas_) {
+    for (auto &kv : ar.second.free) (void)hipFree(kv.second);
+    for (auto &kv : ar.second.deferred) (void)hipFree(kv.second);
+  }
 }
 
 Storage::~Storage() {
@@ -110,6 +192,9 @@ Context::Context(int device, hipStream_t stream) : device_(device), alloc_(devic
 
 Context::~Context() {
   for (auto &l : lanes_) (void)hipStreamSynchronize(l.s);
+  for (auto &kv : sweep_graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  sweep_graphs.clear();
   // tensors held by the context return their storage to alloc_, which is
   // destroyed before the public members: release them while it is alive
   svd_start.clear();
@@ -164,7 +249,7 @@ void Context::join() {
     // so keep it until the next synchronisation point of the main stream
     joined_events_.push_back(e);
   }
-  if (joined_events_.size() > 256) {
+  if (joined_events_.size() > 256 && !capturing_) {
     for (auto ev : joined_events_) {
       TN_HIP_CHECK(hipEventSynchronize(ev));
       event_pool_.push_back(ev);
@@ -194,7 +279,7 @@ void Context::join_noexcept() noexcept {
 }
 
 void Context::move_issued() {
-  if (max_moves_ <= 0 || forked_) return;
+  if (max_moves_ <= 0 || forked_ || capturing_) return;
   hipEvent_t e = get_event();
   TN_HIP_CHECK(hipEventRecord(e, lanes_[0].s));
   inflight_.push_back(e);
@@ -205,7 +290,38 @@ void Context::move_issued() {
   }
 }
 
+void Context::begin_capture() {
+  TN_REQUIRE(!capturing_ && !forked_, "begin_capture: already capturing or forked");
+  TN_HIP_CHECK(hipStreamBeginCapture(lanes_[0].s, hipStreamCaptureModeRelaxed));
+  capturing_ = true;
+}
+
+hipGraph_t Context::end_capture() {
+  TN_REQUIRE(capturing_, "end_capture without begin_capture");
+  capturing_ = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(lanes_[0].s, &g);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (g) (void)hipGraphDestroy(g);
+    return nullptr;
+  }
+  return g;
+}
+
+void Context::drop_sweep_graph(int64_t environment) {
+  auto it = sweep_graphs.find(environment);
+  if (it == sweep_graphs.end()) return;
+  // a replay may still run: the arena's blocks must not serve other work
+  // before it has finished
+  (void)hipStreamSynchronize(lanes_[0].s);
+  if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+  if (it->second.arena >= 0) alloc_.arena_destroy(it->second.arena);
+  sweep_graphs.erase(it);
+}
+
 void Context::forget_svd_starts(int64_t environment) {
+  drop_sweep_graph(environment);
   for (auto it = svd_start.begin(); it != svd_start.end();) {
     if (it->first[0] == environment)
       it = svd_start.erase(it);
@@ -384,6 +500,7 @@ const ContractPlan &Context::plan(const Shape &sa, const Shape &sb,
   for (auto a : perm) key << a << ",";
   auto it = plans_.find(key.str());
   if (it != plans_.end()) return it->second;
+  TN_REQUIRE(!capturing_, "tensordot: new contraction plan inside a stream capture");
 
   const int ra = (int)sa.size(), rb = (int)sb.size(), nc = (int)axa.size();
   TN_REQUIRE((int)axb.size() == nc, "tensordot: axes length mismatch");
@@ -645,7 +762,7 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
   const double flops = 2.0 * p.M * (double)p.N * p.K;
   // autotuning (C = A B overwrites C, so every candidate leaves a valid
   // result): time each launch shape twice on this stream, keep the fastest
-  if (ctx.autotune() && beta == 0.0 && p.tuned_bm == 0 &&
+  if (ctx.autotune() && !ctx.capturing() && beta == 0.0 && p.tuned_bm == 0 &&
       flops >= Context::kAutotuneFlops) {
     hipEvent_t e0, e1;
     TN_HIP_CHECK(hipEventCreate(&e0));

@@ -49,13 +49,31 @@ class Allocator {
   size_t bytes_in_use() const { return in_use_; }
   size_t bytes_cached() const { return cached_; }
 
+  // Arenas for captured graphs.  While an arena is active every allocation
+  // comes from it (its own free list, else a new device block) and a block
+  // of an arena always returns to that arena when freed, so the memory a
+  // captured graph refers to never serves other work while the graph
+  // exists.  Destroying an arena hands its blocks to the general pool (free
+  // ones at once, the others when they are freed).
+  int arena_create();
+  void arena_activate(int id);  // -1: none
+  void arena_destroy(int id);
+
  private:
+  struct Arena {
+    std::multimap<size_t, void *> free;
+    std::vector<std::pair<size_t, void *>> deferred;
+  };
   static size_t round(size_t bytes);
+  static void *take(std::multimap<size_t, void *> &pool, size_t b);
   int device_;
   std::multimap<size_t, void *> free_;
   std::vector<std::pair<size_t, void *>> deferred_;
   bool defer_ = false;
   size_t in_use_ = 0, cached_ = 0;
+  std::map<int, Arena> arenas_;
+  std::map<void *, std::pair<int, size_t>> owner_;  // block -> (arena, bytes)
+  int active_ = -1, next_arena_ = 0;
   std::mutex mu_;
 };
 
@@ -164,6 +182,27 @@ class Context {
   const ContractPlan &plan(const Shape &sa, const Shape &sb, const Axes &axa,
                            const Axes &axb, const Axes &perm);
   void sync();
+
+  // Stream capture of the main stream (relaxed mode; side streams join
+  // through the fork events).  While capturing nothing synchronises: a new
+  // contraction plan (a synchronous upload) is an error, GEMM autotuning and
+  // the event recycling of join() are skipped.  end_capture() returns the
+  // graph (nullptr if the capture was invalidated).
+  void begin_capture();
+  hipGraph_t end_capture();
+  bool capturing() const { return capturing_; }
+  // Captured CTM sweeps, one per environment (key: environment identity).
+  struct SweepGraph {
+    hipGraphExec_t exec = nullptr;
+    int arena = -1;
+    std::vector<uintptr_t> signature;
+    std::vector<PendingSvdStat> pending;
+    double gemm_flops = 0.0;
+    int64_t gemm_calls = 0, svd_calls = 0;
+  };
+  std::map<int64_t, SweepGraph> sweep_graphs;
+  int64_t graph_captures = 0, graph_replays = 0;
+  void drop_sweep_graph(int64_t environment);
   uint64_t next_rng_counter(uint64_t n) {
     uint64_t c = rng_counter_;
     rng_counter_ += n;
@@ -238,6 +277,7 @@ class Context {
   int nforked_ = 0;
   int max_lanes_ = 8;
   int max_moves_ = 0;
+  bool capturing_ = false;
   std::vector<hipEvent_t> inflight_;  // end of each issued move, oldest first
   bool forked_ = false;
   hipEvent_t fork_event_ = nullptr;
