@@ -9,14 +9,16 @@ and chi cycling in lockstep through (3, 32), (4, 48), (5, 64), site tensors
 from 300 simple-update steps (tau = 0.01) on the GPU from the ferro state.
 
     python -m tenes_amd.batch [--problems 48] [--contexts 4] [--sweeps 5]
+                              [--graph 1] [--lanes 8]
 
 prints one JSON line: aggregate sweeps/s over all problems (batched and, for
-comparison, one context), and each problem's energy per site.  Streams of
-different contexts only run side by side when the process has enough
-hardware queues: main() raises GPU_MAX_HW_QUEUES (HIP's default 4) to 16.  Measured (48 problems, 3 sweeps): one context 18.7
-sweeps/s; 4 contexts 26.3 (4 queues) and 40.2 (16 queues); 8 and 16 contexts
-do worse (30.4, 26.9 with 16 queues): the small problems are launch-bound and
-the host threads contend for the runtime's launch path.
+comparison, one context), and each problem's energy per site.  The
+environments of one context advance in lockstep (one sweep each per round,
+one batched convergence check); with --graph 1 each environment's sweep is
+captured once into a hipGraph and the replays of one round run side by side
+on the context's lanes (--lanes).  Streams only run side by side when the
+process has enough hardware queues: main() raises GPU_MAX_HW_QUEUES (HIP's
+default 4) to 16.
 """
 from __future__ import annotations
 
@@ -114,6 +116,11 @@ def main(argv=None):
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--su-steps", type=int, default=300)
+    ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
+                    help="capture each problem's sweep into a hipGraph and replay it")
+    ap.add_argument("--lanes", type=int, default=8, help="side streams per context")
+    ap.add_argument("--compare-one-context", type=int, default=1, choices=(0, 1),
+                    help="also time the problems on one context, uncaptured")
     ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
                     help="also run the batch with contraction_fp32 = 1 or 2 and report each "
                          "problem's energy error against the FP64 batch")
@@ -126,44 +133,53 @@ def main(argv=None):
         print(f"[batch] {msg}", file=sys.stderr, flush=True)
 
     ctxs = [core.Context(0, seed=100 + i) for i in range(args.contexts)]
+    for c in ctxs:
+        c.set_max_lanes(args.lanes)
     t0 = time.perf_counter()
     envs, hams = prepare(ctxs, args.problems, args.su_steps)
     log(f"{args.problems} problems, simple update ({args.su_steps} steps) in {time.perf_counter() - t0:.1f} s "
         f"on {args.contexts} contexts")
-    run_ctm(envs, args.warmup, True)
-    dt = run_ctm(envs, args.sweeps, False)
+    kw = dict(graph_sweeps=args.graph)
+    run_ctm(envs, args.warmup, True, **kw)
+    dt = run_ctm(envs, args.sweeps, False, **kw)
     rate = args.problems * args.sweeps / dt
     log(f"batched: {args.problems} x {args.sweeps} sweeps in {dt:.2f} s -> {rate:.1f} sweeps/s")
     energies = [energy(e, H) for e, H in zip(envs, hams)]
-    # the same problems from one context (sequential), same sweep count
-    single = [core.Environment(ctxs[0], 2, 2) for _ in envs]
-    for s, e in zip(single, envs):
-        s.set_site_tensors(e.site_tensors())
-        s.problem = e.problem
-    run_ctm(single, args.warmup, True)
-    dt1 = run_ctm(single, args.sweeps, False)
-    rate1 = args.problems * args.sweeps / dt1
-    log(f"one context: {args.problems} x {args.sweeps} sweeps in {dt1:.2f} s -> {rate1:.1f} sweeps/s")
-    e1 = [energy(e, H) for e, H in zip(single, hams)]
     out = {
         "metric": "aggregate CTMRG sweeps/s, batched transverse-field Ising field sweep",
         "value": rate,
         "unit": "sweeps/s",
-        "one_context_value": rate1,
         "problems": args.problems,
         "contexts": args.contexts,
+        "lanes": args.lanes,
+        "graph_sweeps": args.graph,
+        "graph_stats": [c.graph_stats() for c in ctxs],
         "sweeps": args.sweeps,
         "warmup": args.warmup,
         "dtype": "f64",
         "config": {"workload": "TFI Jz=-1, hx=3k/(n-1), 2x2, (D,chi) cycling (3,32),(4,48),(5,64), "
                                f"{args.su_steps} SU steps tau=0.01"},
         "energies": energies,
-        "max_energy_diff_batched_vs_one_context": float(np.max(np.abs(np.array(energies) - np.array(e1)))),
     }
+    if args.compare_one_context:
+        # the same problems from one context, uncaptured, same sweep count
+        single = [core.Environment(ctxs[0], 2, 2) for _ in envs]
+        for s, e in zip(single, envs):
+            s.set_site_tensors(e.site_tensors())
+            s.problem = e.problem
+        run_ctm(single, args.warmup, True)
+        dt1 = run_ctm(single, args.sweeps, False)
+        rate1 = args.problems * args.sweeps / dt1
+        log(f"one context, uncaptured: {args.problems} x {args.sweeps} sweeps in {dt1:.2f} s -> "
+            f"{rate1:.1f} sweeps/s")
+        e1 = [energy(e, H) for e, H in zip(single, hams)]
+        out["one_context_value"] = rate1
+        out["max_energy_diff_batched_vs_one_context"] = float(np.max(np.abs(np.array(energies) - np.array(e1))))
+        del single
     if args.mixed:
         # the same batch with the moves' contractions on the FP32 MFMA
-        run_ctm(envs, args.warmup, True, contraction_fp32=args.mixed)
-        dtm = run_ctm(envs, args.sweeps, False, contraction_fp32=args.mixed)
+        run_ctm(envs, args.warmup, True, contraction_fp32=args.mixed, **kw)
+        dtm = run_ctm(envs, args.sweeps, False, contraction_fp32=args.mixed, **kw)
         em = [energy(e, H) for e, H in zip(envs, hams)]
         err = [abs(a - b) / abs(b) for a, b in zip(em, energies)]
         log(f"mixed mode {args.mixed}: {args.problems} x {args.sweeps} sweeps in {dtm:.2f} s -> "

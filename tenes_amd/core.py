@@ -170,7 +170,9 @@ class Tensor:
 def batch_calc_ctm_environment(envs, params, initialize: bool = True):
     """Calc_CTM_Environment on several environments at once
     (tenes_batch_calc_ctm_environment): environments of one context run in
-    order, different contexts concurrently.  params: one PepsParameters or
+    lockstep (one sweep each per round, one batched convergence check per
+    round, captured sweeps replayed side by side), different contexts
+    concurrently.  params: one PepsParameters or
     one per environment.  Returns (sweep counts, sig_max values)."""
     envs = list(envs)
     n = len(envs)

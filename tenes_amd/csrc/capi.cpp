@@ -644,8 +644,8 @@ int tenes_batch_calc_ctm_environment(tenes_env *const *envs, int n,
                                      double *sig_max) {
   if (n < 0 || (n > 0 && (!envs || !p)) || !(nparams == 1 || nparams == n))
     return TENES_ERR_ARGUMENT;
-  // environments of one context run in order on one host thread; contexts
-  // run concurrently (each has its own streams and allocator)
+  // the environments of one context run in lockstep on one host thread;
+  // contexts run concurrently (each has its own streams and allocator)
   std::map<tenes_ctx *, std::vector<int>> groups;
   for (int e = 0; e < n; ++e) {
     if (!envs[e] || !envs[e]->ctx) return TENES_ERR_ARGUMENT;
@@ -661,18 +661,24 @@ int tenes_batch_calc_ctm_environment(tenes_env *const *envs, int n,
       tenes_ctx *ctx = work[g].first;
       rc[g] = guard(ctx, [&] {
         TN_HIP_CHECK(hipSetDevice(ctx->c->device()));
+        std::vector<tenes::itps::core::CtmEnvironment> list;
+        std::vector<tenes::itps::PEPS_Parameters> pps;
         for (int e : work[g].second) {
           tenes_env *env = envs[e];
           for (const auto &t : env->arr[TENES_TN])
             TN_REQUIRE(t.defined(), "site tensor not set");
-          double sm = 0.0;
           auto &a = env->arr;
-          const int c = tenes::itps::core::Calc_CTM_Environment(
-              a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8],
-              to_params(&p[nparams == 1 ? 0 : e]), env->lat, initialize != 0,
-              &sm);
-          if (count) count[e] = c;
-          if (sig_max) sig_max[e] = sm;
+          list.push_back({{&a[0], &a[1], &a[2], &a[3], &a[4], &a[5], &a[6], &a[7]},
+                          &a[8], &env->lat});
+          pps.push_back(to_params(&p[nparams == 1 ? 0 : e]));
+        }
+        std::vector<double> sm;
+        const auto c = tenes::itps::core::Calc_CTM_Environment_lockstep(
+            list, pps, initialize != 0, &sm);
+        for (size_t q = 0; q < c.size(); ++q) {
+          const int e = work[g].second[q];
+          if (count) count[e] = c[q];
+          if (sig_max) sig_max[e] = sm[q];
         }
       });
     });

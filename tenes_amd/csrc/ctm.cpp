@@ -457,16 +457,22 @@ struct MoveSpec {
 // their projector chains overlapped on the lanes of the forked context.
 // With the cut on the device the singular values of projector q go to pinned
 // slot `slot` + q (slot advances by the number of projectors) and are read
-// at the caller's next synchronisation (collect_svd_stats).
+// at the caller's next synchronisation (collect_svd_stats).  slot_end bounds
+// the slots: the uncaptured range (kEagerSlots; when it is used up the
+// slots are read first and counting restarts at 0) or a captured sweep's
+// window (which holds all its projectors).
 void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
-               const std::vector<MoveSpec *> &moves, int &slot) {
+               const std::vector<MoveSpec *> &moves, int &slot,
+               int slot_end = tn::Context::kEagerSlots) {
   tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
   const bool dev = pp.projector_cut_on_device;
   int total = 0;
   for (auto *m : moves) total += m->L;
-  TN_REQUIRE(total < kSyncSlot, "CTM move: too many projectors in one move");
-  if (dev && slot + total > kSyncSlot) {
-    // the pinned slots of this sweep are used up: read them first
+  TN_REQUIRE(total <= tn::Context::kEagerSlots, "CTM move: too many projectors in one move");
+  if (dev && slot + total > slot_end) {
+    TN_REQUIRE(slot_end == tn::Context::kEagerSlots,
+               "CTM move: the captured sweep's pinned slot window is too small");
+    // the pinned slots are used up: read them first
     ctx.sync();
     ctx.collect_svd_stats();
     slot = 0;
@@ -696,8 +702,9 @@ MoveSpec bottom_spec(const Arr &C1, const Arr &C2, Arr &C3, Arr &C4,
 // device the move ends with a synchronisation that reads its projectors'
 // statistics.
 static void run_single_move(tn::Context &ctx, const PEPS_Parameters &pp, MoveSpec &m) {
-  int slot = 0;
+  int slot = ctx.eager_slot;
   run_moves(ctx, pp, {&m}, slot);
+  ctx.eager_slot = slot;
   if (pp.projector_cut_on_device) {
     ctx.sync();
     ctx.collect_svd_stats();
@@ -784,60 +791,6 @@ std::vector<std::vector<double>> spectra(const std::vector<const tensor *> &Cs) 
 
 std::vector<double> normalized_spectrum(const tensor &C) {
   return spectra({&C})[0];
-}
-
-// ctm.cpp:815-969
-bool Check_Convergence_CTM(const std::vector<tensor> &C1,
-                           const std::vector<tensor> &C2,
-                           const std::vector<tensor> &C3,
-                           const std::vector<tensor> &C4,
-                           const std::vector<tensor> &C1_old,
-                           const std::vector<tensor> &C2_old,
-                           const std::vector<tensor> &C3_old,
-                           const std::vector<tensor> &C4_old,
-                           const PEPS_Parameters &pp,
-                           const SquareLattice &lat, double &sig_max,
-                           double *sig_all) {
-  const int N = lat.N_UNIT;
-  std::vector<const tensor *> all;
-  for (int i = 0; i < N; ++i) {
-    all.push_back(&C1[i]);
-    all.push_back(&C1_old[i]);
-    all.push_back(&C2[i]);
-    all.push_back(&C2_old[i]);
-    all.push_back(&C3[i]);
-    all.push_back(&C3_old[i]);
-    all.push_back(&C4[i]);
-    all.push_back(&C4_old[i]);
-  }
-  auto lam = spectra(all);
-  std::vector<double> sigs(4 * (size_t)N);
-  for (int i = 0; i < N; ++i) {
-    for (int c = 0; c < 4; ++c) {
-      const auto &ln = lam[8 * i + 2 * c];
-      const auto &lo = lam[8 * i + 2 * c + 1];
-      double sig = 0.0;
-      for (size_t j = 0; j < ln.size(); ++j) {
-        // the reference reads lam_old[j] for j < lam_new.size(); a missing
-        // entry (shorter old spectrum) counts as zero here.
-        const double o = j < lo.size() ? lo[j] : 0.0;
-        sig += (ln[j] - o) * (ln[j] - o);
-      }
-      sigs[4 * i + c] = std::sqrt(sig);
-    }
-  }
-  if (sig_all) *sig_all = *std::max_element(sigs.begin(), sigs.end());
-  // the reference's order and early exit at the first corner above epsilon
-  sig_max = 0.0;
-  for (double sig : sigs) {
-    if (sig > pp.CTM_Convergence_Epsilon) {
-      sig_max = sig;
-      return false;
-    } else if (sig > sig_max) {
-      sig_max = sig;
-    }
-  }
-  return true;
 }
 
 namespace {
@@ -979,25 +932,36 @@ namespace {
 // by side, step s of one with step s of the other; likewise the top (C1, C2,
 // eTt) and bottom (C3, C4, eTb) chains of ctm.cpp:1215-1224.  Each chain
 // keeps the reference's order, so the result is the same.
+// With window_first >= 0 the projectors' pinned slots are the window
+// starting there (a captured sweep's own, sweep_projectors(lat) long),
+// else the context's uncaptured slots.
+int sweep_projectors(const SquareLattice &lat) {
+  return 2 * lat.LX_noskew * lat.LY_noskew + 2 * lat.LY_noskew * lat.LX_noskew;
+}
+
 void corner_sweep(std::vector<tensor> *A[8], const std::vector<tensor> &Tn,
-                  const PEPS_Parameters &ps, const SquareLattice &lat) {
+                  const PEPS_Parameters &ps, const SquareLattice &lat,
+                  int window_first = -1) {
   auto &C1 = *A[0], &C2 = *A[1], &C3 = *A[2], &C4 = *A[3];
   auto &eTt = *A[4], &eTr = *A[5], &eTb = *A[6], &eTl = *A[7];
   tn::Context &ctx = *Tn[0].ctx();
   const int LX = lat.LX_noskew, LY = lat.LY_noskew;
-  int slot = 0;  // pinned slots of this sweep's projectors
+  const bool window = window_first >= 0;
+  int slot = window ? window_first : ctx.eager_slot;
+  const int slot_end = window ? window_first + sweep_projectors(lat) : tn::Context::kEagerSlots;
   for (int s = 0; s < LX; ++s) {
     MoveSpec a = left_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
     MoveSpec b = right_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
                             (-s + 1 + LX) % LX, ps, lat);
-    run_moves(ctx, ps, {&a, &b}, slot);
+    run_moves(ctx, ps, {&a, &b}, slot, slot_end);
   }
   for (int s = 0; s < LY; ++s) {
     MoveSpec a = top_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn,
                           (-s + 1 + LY) % LY, ps, lat);
     MoveSpec b = bottom_spec(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, s, ps, lat);
-    run_moves(ctx, ps, {&a, &b}, slot);
+    run_moves(ctx, ps, {&a, &b}, slot, slot_end);
   }
+  if (!window) ctx.eager_slot = slot;
 }
 
 // What a captured sweep of an environment bakes in: the buffers and shapes
@@ -1061,6 +1025,9 @@ bool capture_sweep(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
                    const std::vector<tensor> &Tn, const PEPS_Parameters &ps,
                    const SquareLattice &lat) {
   tn::Context::SweepGraph g;
+  g.slot_count = sweep_projectors(lat);
+  g.slot_first = ctx.slot_window(g.slot_count);
+  if (g.slot_first < 0) return false;  // no pinned slots left: stay uncaptured
   g.arena = ctx.allocator().arena_create();
   std::vector<tensor> in[8];
   for (int k = 0; k < 8; ++k) in[k] = *A[k];
@@ -1071,15 +1038,16 @@ bool capture_sweep(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
   const int64_t calls0 = ctx.gemm_calls, svd0 = ctx.svd_calls;
   const size_t pend0 = ctx.pending_svd.size();
   bool ok = true;
+  ctx.allocator().arena_activate(g.arena);
   try {
     ctx.begin_capture();
   } catch (...) {
-    ctx.allocator().arena_destroy(g.arena);
+    ctx.allocator().arena_activate(-1);
+    ctx.release_sweep_graph(g);
     return false;
   }
-  ctx.allocator().arena_activate(g.arena);
   try {
-    corner_sweep(A, Tn, ps, lat);
+    corner_sweep(A, Tn, ps, lat, g.slot_first);
     hipStream_t s = ctx.stream();
     auto copy_back = [&](const tensor &dst, const tensor &src) {
       if (dst.st == src.st) return;
@@ -1097,8 +1065,8 @@ bool capture_sweep(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
   } catch (...) {
     ok = false;
   }
+  hipGraph_t graph = ctx.end_capture(g.blocks);
   ctx.allocator().arena_activate(-1);
-  hipGraph_t graph = ctx.end_capture();
   for (int k = 0; k < 8; ++k) *A[k] = in[k];
   for (const auto &kv : starts) ctx.svd_start[kv.first] = kv.second;
   std::vector<tn::Context::PendingSvdStat> pend(ctx.pending_svd.begin() + pend0,
@@ -1116,7 +1084,7 @@ bool capture_sweep(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
   }
   if (graph) (void)hipGraphDestroy(graph);
   if (!ok) {
-    ctx.allocator().arena_destroy(g.arena);
+    ctx.release_sweep_graph(g);
     return false;
   }
   g.pending = std::move(pend);
@@ -1124,6 +1092,222 @@ bool capture_sweep(tn::Context &ctx, int64_t env, std::vector<tensor> *A[8],
   ctx.sweep_graphs[env] = std::move(g);
   ctx.graph_captures++;
   return true;
+}
+
+}  // namespace
+
+// Per-corner convergence of one environment from the spectra of its new and
+// previous corners (lam[8 i + 2 c] and lam[8 i + 2 c + 1] for site i, corner
+// c): the reference's order and early exit (ctm.cpp:815-969).
+static bool convergence_from_spectra(const std::vector<std::vector<double>> &lam,
+                                     size_t first, int N, double eps,
+                                     double &sig_max, double *sig_all) {
+  std::vector<double> sigs(4 * (size_t)N);
+  for (int i = 0; i < N; ++i) {
+    for (int c = 0; c < 4; ++c) {
+      const auto &ln = lam[first + 8 * i + 2 * c];
+      const auto &lo = lam[first + 8 * i + 2 * c + 1];
+      double sig = 0.0;
+      for (size_t j = 0; j < ln.size(); ++j) {
+        // the reference reads lam_old[j] for j < lam_new.size(); a missing
+        // entry (shorter old spectrum) counts as zero here.
+        const double o = j < lo.size() ? lo[j] : 0.0;
+        sig += (ln[j] - o) * (ln[j] - o);
+      }
+      sigs[4 * i + c] = std::sqrt(sig);
+    }
+  }
+  if (sig_all) *sig_all = *std::max_element(sigs.begin(), sigs.end());
+  sig_max = 0.0;
+  for (double sig : sigs) {
+    if (sig > eps) {
+      sig_max = sig;
+      return false;
+    } else if (sig > sig_max) {
+      sig_max = sig;
+    }
+  }
+  return true;
+}
+
+static void push_corner_pairs(std::vector<const tensor *> &all,
+                              std::vector<tensor> *const A[4],
+                              const std::vector<tensor> *old, int N) {
+  for (int i = 0; i < N; ++i)
+    for (int c = 0; c < 4; ++c) {
+      all.push_back(&(*A[c])[i]);
+      all.push_back(&old[c][i]);
+    }
+}
+
+// ctm.cpp:815-969
+bool Check_Convergence_CTM(const std::vector<tensor> &C1,
+                           const std::vector<tensor> &C2,
+                           const std::vector<tensor> &C3,
+                           const std::vector<tensor> &C4,
+                           const std::vector<tensor> &C1_old,
+                           const std::vector<tensor> &C2_old,
+                           const std::vector<tensor> &C3_old,
+                           const std::vector<tensor> &C4_old,
+                           const PEPS_Parameters &pp,
+                           const SquareLattice &lat, double &sig_max,
+                           double *sig_all) {
+  const int N = lat.N_UNIT;
+  std::vector<tensor> *const A[4] = {const_cast<std::vector<tensor> *>(&C1),
+                                     const_cast<std::vector<tensor> *>(&C2),
+                                     const_cast<std::vector<tensor> *>(&C3),
+                                     const_cast<std::vector<tensor> *>(&C4)};
+  const std::vector<tensor> old[4] = {C1_old, C2_old, C3_old, C4_old};
+  std::vector<const tensor *> all;
+  push_corner_pairs(all, A, old, N);
+  const auto lam = spectra(all);
+  return convergence_from_spectra(lam, 0, N, pp.CTM_Convergence_Epsilon, sig_max, sig_all);
+}
+
+namespace {
+
+// One environment's CTM run inside run_ctm (the state of the reference's
+// sweep loop, ctm.cpp:1196-1253).
+struct CtmRun {
+  std::vector<tensor> *A[8];
+  const std::vector<tensor> *Tn;
+  PEPS_Parameters pp;  // as given
+  PEPS_Parameters ps;  // as run: the cut on the device when sweeps are captured
+  const SquareLattice *lat;
+  int64_t env = 0;
+  bool graphs = false;
+  bool convergence = false;
+  int count = 0;
+  double sig_max = 0.0;
+  // corners of the previous sweep: handles, or copies when a replay may
+  // write the current buffers in place
+  std::vector<tensor> C_old[4];
+
+  bool active() const { return !convergence && count < pp.Max_CTM_Iteration; }
+  void keep_old() {
+    for (int c = 0; c < 4; ++c) {
+      C_old[c] = *A[c];
+      if (graphs)
+        for (auto &t : C_old[c]) t = tn::clone(t);
+    }
+  }
+};
+
+// The uncaptured sweep of one environment, in the reference's move order.
+void eager_sweep(CtmRun &r) {
+  auto &A = r.A;
+  const auto &Tn = *r.Tn;
+  const auto &lat = *r.lat;
+  const auto &ps = r.ps;
+  if (ps.CTM_Projector_corner) {
+    corner_sweep(A, Tn, ps, lat);
+    return;
+  }
+  auto &C1 = *A[0], &C2 = *A[1], &C3 = *A[2], &C4 = *A[3];
+  auto &eTt = *A[4], &eTr = *A[5], &eTb = *A[6], &eTl = *A[7];
+  const int LX = lat.LX_noskew, LY = lat.LY_noskew;
+  for (int ix = 0; ix < LX; ++ix)
+    Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, ps, lat);
+  for (int ix = 0; ix > -LX; --ix)
+    Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX, ps, lat);
+  for (int iy = 0; iy > -LY; --iy)
+    Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY, ps, lat);
+  for (int iy = 0; iy < LY; ++iy)
+    Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, ps, lat);
+}
+
+// The sweep loops of several environments of one context in lockstep: each
+// round issues one sweep of every environment still running (uncaptured
+// sweeps in turn, then the replays of the captured ones side by side, one
+// per lane), then checks the convergence of all of them with one batched
+// spectrum computation and one synchronisation.  Each environment's own
+// sequence of sweeps and checks is the reference's (ctm.cpp:1196-1253).
+void run_ctm(std::vector<CtmRun> &runs) {
+  if (runs.empty()) return;
+  tn::Context &ctx = *(*runs[0].Tn)[0].ctx();
+  for (;;) {
+    std::vector<CtmRun *> act;
+    for (auto &r : runs)
+      if (r.active()) act.push_back(&r);
+    if (act.empty()) break;
+    std::vector<const tn::Context::SweepGraph *> replays;
+    for (CtmRun *r : act) {
+      if (r->graphs && sweep_graph_ready(ctx, r->env, r->A, *r->lat, r->ps)) {
+        auto it = ctx.sweep_graphs.find(r->env);
+        if (it == ctx.sweep_graphs.end() ||
+            it->second.signature != sweep_signature(ctx, r->env, r->A, *r->Tn, r->ps)) {
+          ctx.drop_sweep_graph(r->env);
+          capture_sweep(ctx, r->env, r->A, *r->Tn, r->ps, *r->lat);
+          it = ctx.sweep_graphs.find(r->env);
+        }
+        if (it != ctx.sweep_graphs.end()) {
+          replays.push_back(&it->second);
+          continue;
+        }
+      }
+      eager_sweep(*r);
+    }
+    if (!replays.empty()) {
+      const int nl = std::min<int>((int)replays.size(), ctx.max_lanes());
+      tn::ForkScope lanes(ctx, nl, nl > 1);
+      for (size_t q = 0; q < replays.size(); ++q) {
+        if (nl > 1) ctx.use_lane((int)q % nl);
+        TN_HIP_CHECK(hipGraphLaunch(replays[q]->exec, ctx.stream()));
+      }
+      lanes.end();
+      for (const auto *g : replays) {
+        ctx.graph_replays++;
+        ctx.pending_svd.insert(ctx.pending_svd.end(), g->pending.begin(), g->pending.end());
+        ctx.gemm_flops += g->gemm_flops;
+        ctx.gemm_calls += g->gemm_calls;
+        ctx.svd_calls += g->svd_calls;
+      }
+    }
+    std::vector<const tensor *> all;
+    for (CtmRun *r : act) push_corner_pairs(all, r->A, r->C_old, r->lat->N_UNIT);
+    const auto lam = spectra(all);
+    // the spectra synchronised the stream: the sweeps' projector statistics
+    // (and warm-start ranges) are in their pinned slots
+    ctx.collect_svd_stats();
+    size_t first = 0;
+    for (CtmRun *r : act) {
+      r->convergence = convergence_from_spectra(lam, first, r->lat->N_UNIT,
+                                                r->pp.CTM_Convergence_Epsilon, r->sig_max, nullptr);
+      first += 8 * (size_t)r->lat->N_UNIT;
+      r->count += 1;
+      r->keep_old();
+      if (r->pp.print_level >= 3)
+        std::printf("CTM: count, sig_max %d %.17g\n", r->count, r->sig_max);
+    }
+  }
+  for (auto &r : runs)
+    if (!r.convergence && r.pp.print_level >= 2)
+      std::printf("Warning: CTM did not converge! count, sig_max = %d %.17g\n", r.count,
+                  r.sig_max);
+}
+
+CtmRun make_run(CtmEnvironment &e, const PEPS_Parameters &pp, bool initialize) {
+  tn::Context &ctx = *(*e.Tn)[0].ctx();
+  CtmRun r;
+  for (int k = 0; k < 8; ++k) r.A[k] = e.A[k];
+  r.Tn = e.Tn;
+  r.lat = e.lat;
+  r.pp = pp;
+  r.env = environment_id(*e.A[0]);
+  // a fresh environment forgets the warm-start bases of its projectors (and
+  // its captured sweep); a continued one keeps them
+  if (initialize) {
+    ctx.forget_svd_starts(r.env);
+    Initialize_CTM(*e.A[0], *e.A[1], *e.A[2], *e.A[3], *e.A[4], *e.A[5], *e.A[6], *e.A[7],
+                   *e.Tn, pp, *e.lat);
+  }
+  // captured sweeps need fixed shapes: the projector cut on the device
+  r.ps = pp;
+  r.graphs = pp.graph_sweeps && pp.CTM_Projector_corner && !ctx.profiling() &&
+             !ctx.autotune() && !ctx.single_layer();
+  if (r.graphs) r.ps.projector_cut_on_device = true;
+  r.keep_old();
+  return r;
 }
 
 }  // namespace
@@ -1136,85 +1320,32 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
                          const std::vector<tensor> &Tn,
                          const PEPS_Parameters &pp, const SquareLattice &lat,
                          bool initialize, double *sig_max_out) {
-  tn::Context &ctx = *Tn[0].ctx();
-  const int64_t env = environment_id(C1);
-  // a fresh environment forgets the warm-start bases of its projectors (and
-  // its captured sweep); a continued one keeps them
-  if (initialize) {
-    ctx.forget_svd_starts(env);
-    Initialize_CTM(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, pp, lat);
+  CtmEnvironment e{{&C1, &C2, &C3, &C4, &eTt, &eTr, &eTb, &eTl}, &Tn, &lat};
+  std::vector<CtmRun> runs;
+  runs.push_back(make_run(e, pp, initialize));
+  run_ctm(runs);
+  if (sig_max_out) *sig_max_out = runs[0].sig_max;
+  return runs[0].count;
+}
+
+std::vector<int> Calc_CTM_Environment_lockstep(std::vector<CtmEnvironment> &envs,
+                                               const std::vector<PEPS_Parameters> &pps,
+                                               bool initialize, std::vector<double> *sig_max) {
+  TN_REQUIRE(pps.size() == envs.size(), "lockstep CTM: one parameter set per environment");
+  std::vector<CtmRun> runs;
+  for (size_t i = 0; i < envs.size(); ++i) {
+    TN_REQUIRE((*envs[i].Tn)[0].ctx() == (*envs[0].Tn)[0].ctx(),
+               "lockstep CTM: the environments must share one context");
+    runs.push_back(make_run(envs[i], pps[i], initialize));
   }
-  // captured sweeps need fixed shapes: the projector cut on the device
-  PEPS_Parameters ps = pp;
-  const bool graphs = pp.graph_sweeps && pp.CTM_Projector_corner && !ctx.profiling() &&
-                      !ctx.autotune() && !ctx.single_layer();
-  if (graphs) ps.projector_cut_on_device = true;
-  std::vector<tensor> *A[8] = {&C1, &C2, &C3, &C4, &eTt, &eTr, &eTb, &eTl};
-  bool convergence = false;
-  int count = 0;
-  // corners of the previous sweep: handles, or copies when a replay may
-  // write the current buffers in place
-  std::vector<tensor> C_old[4];
-  auto keep_old = [&] {
-    for (int c = 0; c < 4; ++c) {
-      C_old[c] = *A[c];
-      if (graphs)
-        for (auto &t : C_old[c]) t = tn::clone(t);
-    }
-  };
-  keep_old();
-  double sig_max = 0.0;
-  const int LX = lat.LX_noskew, LY = lat.LY_noskew;
-  while (!convergence && count < pp.Max_CTM_Iteration) {
-    bool done = false;
-    if (graphs && sweep_graph_ready(ctx, env, A, lat, ps)) {
-      auto it = ctx.sweep_graphs.find(env);
-      if (it == ctx.sweep_graphs.end() ||
-          it->second.signature != sweep_signature(ctx, env, A, Tn, ps)) {
-        ctx.drop_sweep_graph(env);
-        capture_sweep(ctx, env, A, Tn, ps, lat);
-        it = ctx.sweep_graphs.find(env);
-      }
-      if (it != ctx.sweep_graphs.end()) {
-        const auto &g = it->second;
-        TN_HIP_CHECK(hipGraphLaunch(g.exec, ctx.stream()));
-        ctx.graph_replays++;
-        ctx.pending_svd.insert(ctx.pending_svd.end(), g.pending.begin(), g.pending.end());
-        ctx.gemm_flops += g.gemm_flops;
-        ctx.gemm_calls += g.gemm_calls;
-        ctx.svd_calls += g.svd_calls;
-        done = true;
-      }
-    }
-    if (!done && ps.CTM_Projector_corner) {
-      corner_sweep(A, Tn, ps, lat);
-    } else if (!done) {
-      for (int ix = 0; ix < LX; ++ix)
-        Left_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, ix, ps, lat);
-      for (int ix = 0; ix > -LX; --ix)
-        Right_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (ix + 1 + LX) % LX,
-                   ps, lat);
-      for (int iy = 0; iy > -LY; --iy)
-        Top_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, (iy + 1 + LY) % LY,
-                 ps, lat);
-      for (int iy = 0; iy < LY; ++iy)
-        Bottom_move(C1, C2, C3, C4, eTt, eTr, eTb, eTl, Tn, iy, ps, lat);
-    }
-    convergence = Check_Convergence_CTM(C1, C2, C3, C4, C_old[0], C_old[1],
-                                        C_old[2], C_old[3], pp, lat, sig_max);
-    // the check synchronised the stream: the sweep's projector statistics
-    // (and warm-start ranges) are in their pinned slots
-    ctx.collect_svd_stats();
-    count += 1;
-    keep_old();
-    if (pp.print_level >= 3)
-      std::printf("CTM: count, sig_max %d %.17g\n", count, sig_max);
+  run_ctm(runs);
+  std::vector<int> counts;
+  if (sig_max) sig_max->clear();
+  for (auto &r : runs) {
+    counts.push_back(r.count);
+    if (sig_max) sig_max->push_back(r.sig_max);
   }
-  if (!convergence && pp.print_level >= 2)
-    std::printf("Warning: CTM did not converge! count, sig_max = %d %.17g\n",
-                count, sig_max);
-  if (sig_max_out) *sig_max_out = sig_max;
-  return count;
+  return counts;
 }
 
 }  // namespace core

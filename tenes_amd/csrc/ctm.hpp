@@ -187,6 +187,26 @@ int Calc_CTM_Environment(std::vector<tensor> &C1, std::vector<tensor> &C2,
                          const SquareLattice &lattice, bool initialize = true,
                          double *sig_max_out = nullptr);
 
+// The arrays of one environment: C1..C4, eTt, eTr, eTb, eTl, and its site
+// tensors and lattice.
+struct CtmEnvironment {
+  std::vector<tensor> *A[8];
+  const std::vector<tensor> *Tn;
+  const SquareLattice *lat;
+};
+
+// Calc_CTM_Environment for several environments of one context at once:
+// every environment runs the reference's own sequence of sweeps and
+// convergence checks (with its own parameters), and the environments
+// advance in lockstep, one sweep each per round, so that one batched
+// convergence check and one synchronisation serve the round and captured
+// sweeps (graph_sweeps) of different environments replay side by side.
+// Returns each environment's sweep count.
+std::vector<int> Calc_CTM_Environment_lockstep(std::vector<CtmEnvironment> &envs,
+                                               const std::vector<PEPS_Parameters> &pps,
+                                               bool initialize,
+                                               std::vector<double> *sig_max = nullptr);
+
 // Singular spectra of C (L2 normalised, descending) for diagnostics/tests.
 std::vector<double> normalized_spectrum(const tensor &C);
 

@@ -192,8 +192,7 @@ Context::Context(int device, hipStream_t stream) : device_(device), alloc_(devic
 
 Context::~Context() {
   for (auto &l : lanes_) (void)hipStreamSynchronize(l.s);
-  for (auto &kv : sweep_graphs)
-    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  for (auto &kv : sweep_graphs) release_sweep_graph(kv.second);
   sweep_graphs.clear();
   // tensors held by the context return their storage to alloc_, which is
   // destroyed before the public members: release them while it is alive
@@ -217,13 +216,9 @@ Context::~Context() {
 void Context::fork(int n) {
   TN_REQUIRE(!forked_, "Context::fork: already forked");
   TN_REQUIRE(n >= 1 && n <= 16, "Context::fork: 1 to 16 lanes");
-  while ((int)lanes_.size() < n + 1) {
-    Lane l;
-    // default priority: every stream of the library runs at one priority
-    TN_HIP_CHECK(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));
-    TN_HIP_CHECK(hipMalloc(&l.scal, 4096 * sizeof(double)));
-    lanes_.push_back(l);
-  }
+  TN_REQUIRE(!capturing_ || (int)lanes_.size() >= n + 1,
+             "Context::fork: more lanes than prepared for the capture");
+  add_lanes(n);
   TN_HIP_CHECK(hipEventRecord(fork_event_, lanes_[0].s));
   for (int i = 1; i <= n; ++i)
     TN_HIP_CHECK(hipStreamWaitEvent(lanes_[i].s, fork_event_, 0));
@@ -231,6 +226,16 @@ void Context::fork(int n) {
   nforked_ = n;
   forked_ = true;
   cur_ = 0;
+}
+
+void Context::add_lanes(int n) {
+  while ((int)lanes_.size() < n + 1) {
+    Lane l;
+    // default priority: every stream of the library runs at one priority
+    TN_HIP_CHECK(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));
+    TN_HIP_CHECK(hipMalloc(&l.scal, 4096 * sizeof(double)));
+    lanes_.push_back(l);
+  }
 }
 
 void Context::use_lane(int i) {
@@ -292,15 +297,50 @@ void Context::move_issued() {
 
 void Context::begin_capture() {
   TN_REQUIRE(!capturing_ && !forked_, "begin_capture: already capturing or forked");
-  TN_HIP_CHECK(hipStreamBeginCapture(lanes_[0].s, hipStreamCaptureModeRelaxed));
+  // every lane the captured work may fork into exists before the capture
+  add_lanes(max_lanes_);
+  // the capture's own scalar areas and (empty) workspaces
+  saved_lanes_.clear();
+  for (auto &l : lanes_) {
+    saved_lanes_.push_back({l.ws, l.ws_bytes, l.scal});
+    l.ws = nullptr;
+    l.ws_bytes = 0;
+    l.scal = nullptr;
+  }
+  auto restore = [&] {
+    for (size_t i = 0; i < lanes_.size(); ++i) {
+      if (lanes_[i].scal) alloc_.free(lanes_[i].scal, 4096 * sizeof(double));
+      lanes_[i].ws = saved_lanes_[i].ws;
+      lanes_[i].ws_bytes = saved_lanes_[i].ws_bytes;
+      lanes_[i].scal = saved_lanes_[i].scal;
+    }
+    saved_lanes_.clear();
+  };
+  try {
+    for (auto &l : lanes_) l.scal = static_cast<double *>(alloc_.alloc(4096 * sizeof(double)));
+    TN_HIP_CHECK(hipStreamBeginCapture(lanes_[0].s, hipStreamCaptureModeRelaxed));
+  } catch (...) {
+    restore();
+    throw;
+  }
   capturing_ = true;
 }
 
-hipGraph_t Context::end_capture() {
+hipGraph_t Context::end_capture(std::vector<std::pair<void *, size_t>> &blocks) {
   TN_REQUIRE(capturing_, "end_capture without begin_capture");
   capturing_ = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(lanes_[0].s, &g);
+  blocks.clear();
+  for (size_t i = 0; i < lanes_.size(); ++i) {
+    Lane &l = lanes_[i];
+    if (l.ws) blocks.emplace_back(l.ws, l.ws_bytes);
+    blocks.emplace_back(l.scal, 4096 * sizeof(double));
+    l.ws = saved_lanes_[i].ws;
+    l.ws_bytes = saved_lanes_[i].ws_bytes;
+    l.scal = saved_lanes_[i].scal;
+  }
+  saved_lanes_.clear();
   if (e != hipSuccess) {
     (void)hipGetLastError();
     if (g) (void)hipGraphDestroy(g);
@@ -309,14 +349,46 @@ hipGraph_t Context::end_capture() {
   return g;
 }
 
+int Context::slot_window(int n) {
+  const int m = kSlotProjectorSync - kEagerSlots;
+  if (slot_used_.empty()) slot_used_.assign(m, false);
+  for (int i = 0; i + n <= m; ++i) {
+    int j = 0;
+    while (j < n && !slot_used_[i + j]) ++j;
+    if (j == n) {
+      for (j = 0; j < n; ++j) slot_used_[i + j] = true;
+      return kEagerSlots + i;
+    }
+    i += j;  // the slot at i + j is taken: restart after it
+  }
+  return -1;
+}
+
+void Context::release_slot_window(int first, int n) {
+  for (int j = 0; j < n; ++j) {
+    const int i = first - kEagerSlots + j;
+    if (i >= 0 && i < (int)slot_used_.size()) slot_used_[i] = false;
+  }
+}
+
+void Context::release_sweep_graph(SweepGraph &g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  for (auto &b : g.blocks) alloc_.free(b.first, b.second);
+  g.blocks.clear();
+  if (g.arena >= 0) alloc_.arena_destroy(g.arena);
+  g.arena = -1;
+  if (g.slot_first >= 0) release_slot_window(g.slot_first, g.slot_count);
+  g.slot_first = -1;
+}
+
 void Context::drop_sweep_graph(int64_t environment) {
   auto it = sweep_graphs.find(environment);
   if (it == sweep_graphs.end()) return;
-  // a replay may still run: the arena's blocks must not serve other work
-  // before it has finished
-  (void)hipStreamSynchronize(lanes_[0].s);
-  if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
-  if (it->second.arena >= 0) alloc_.arena_destroy(it->second.arena);
+  // a replay may still run (on any lane): the arena's blocks must not
+  // serve other work before it has finished
+  for (auto &l : lanes_) (void)hipStreamSynchronize(l.s);
+  release_sweep_graph(it->second);
   sweep_graphs.erase(it);
 }
 
@@ -373,6 +445,7 @@ void Context::collect_svd_stats() {
       it->second.range = h[p.nall - 1] > 0.0 ? h[0] / h[p.nall - 1] : 0.0;
   }
   pending_svd.clear();
+  eager_slot = 0;
 }
 
 void Context::sync() {

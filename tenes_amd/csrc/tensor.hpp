@@ -157,11 +157,20 @@ class Context {
   void *workspace(size_t bytes);
   double *scalars(int n);  // small device scalar area
   // Pinned host buffer for asynchronous device-to-host copies of small
-  // results (slot < kPinnedSlots, at most 512 doubles each).  The last two
-  // slots belong to the synchronous SVD and projector entry points.
-  static constexpr int kPinnedSlots = 256;
+  // results (slot < kPinnedSlots, at most 512 doubles each).  Slots below
+  // kEagerSlots serve uncaptured work in issue order (eager_slot is the next
+  // free one, reset when the pending statistics are collected); captured
+  // sweeps own windows of slots above it (slot_window), so a replay never
+  // writes a slot that other work in flight reads.  The last two slots
+  // belong to the synchronous SVD and projector entry points.
+  static constexpr int kPinnedSlots = 4096;
+  static constexpr int kEagerSlots = 256;
   static constexpr int kSlotProjectorSync = kPinnedSlots - 2;
   static constexpr int kSlotSvdSync = kPinnedSlots - 1;
+  int eager_slot = 0;
+  // First slot of n free consecutive slots of the window range (-1 if none).
+  int slot_window(int n);
+  void release_slot_window(int first, int n);
   double *pinned(int slot);
   // The same slot's address for device code (kernels write it directly).
   double *pinned_device(int slot);
@@ -186,15 +195,23 @@ class Context {
   // Stream capture of the main stream (relaxed mode; side streams join
   // through the fork events).  While capturing nothing synchronises: a new
   // contraction plan (a synchronous upload) is an error, GEMM autotuning and
-  // the event recycling of join() are skipped.  end_capture() returns the
-  // graph (nullptr if the capture was invalidated).
+  // the event recycling of join() are skipped.  Every lane (the main stream
+  // and all max_lanes() side streams) gets its own scalar area and an empty
+  // workspace for the capture, allocated from the active allocator arena,
+  // so the graph shares neither with uncaptured work nor with other graphs
+  // (a workspace that later grows cannot pull memory out from under a
+  // graph).  end_capture() returns the graph (nullptr if the capture was
+  // invalidated), restores the lanes' own areas and hands the capture's
+  // areas (pointer, bytes) to `blocks`: they belong to the graph.
   void begin_capture();
-  hipGraph_t end_capture();
+  hipGraph_t end_capture(std::vector<std::pair<void *, size_t>> &blocks);
   bool capturing() const { return capturing_; }
   // Captured CTM sweeps, one per environment (key: environment identity).
   struct SweepGraph {
     hipGraphExec_t exec = nullptr;
     int arena = -1;
+    std::vector<std::pair<void *, size_t>> blocks;  // lane areas of the graph
+    int slot_first = -1, slot_count = 0;             // its pinned slot window
     std::vector<uintptr_t> signature;
     std::vector<PendingSvdStat> pending;
     double gemm_flops = 0.0;
@@ -202,7 +219,11 @@ class Context {
   };
   std::map<int64_t, SweepGraph> sweep_graphs;
   int64_t graph_captures = 0, graph_replays = 0;
+  // Waits for every lane, then releases the environment's graph.
   void drop_sweep_graph(int64_t environment);
+  // Releases a graph's executable, lane areas, arena and slot window (the
+  // caller has made sure no replay of it runs).
+  void release_sweep_graph(SweepGraph &g);
   uint64_t next_rng_counter(uint64_t n) {
     uint64_t c = rng_counter_;
     rng_counter_ += n;
@@ -273,11 +294,19 @@ class Context {
   bool own_stream_ = false;
   Allocator alloc_;
   std::vector<Lane> lanes_;  // [0] = main stream, then the side streams
+  void add_lanes(int n);     // creates side streams until there are n
   int cur_ = 0;
   int nforked_ = 0;
   int max_lanes_ = 8;
   int max_moves_ = 0;
   bool capturing_ = false;
+  struct LaneAreas {
+    void *ws;
+    size_t ws_bytes;
+    double *scal;
+  };
+  std::vector<LaneAreas> saved_lanes_;  // the lanes' own areas during a capture
+  std::vector<bool> slot_used_;         // window range, kEagerSlots upward
   std::vector<hipEvent_t> inflight_;  // end of each issued move, oldest first
   bool forked_ = false;
   hipEvent_t fork_event_ = nullptr;

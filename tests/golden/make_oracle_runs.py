@@ -15,7 +15,16 @@ outputs (no reference code runs); the GPU tests and bench.py compare with them.
       10^4) and corner spectra.  The site tensors are stored so that the GPU
       runs the same problem.
 
-Run:  python tests/golden/make_oracle_runs.py [random|heisenberg]  (minutes to
+  tfi_batch48.json  the batched transverse-field Ising field sweep: 48
+      problems, Jz = -1, hx_k = 3k/47, 2x2 cell, (D, chi) cycling (3, 32),
+      (4, 48), (5, 64); site tensors from 300 simple-update steps (tau =
+      0.01, ferro state along +z + 0.01 noise, seed 11), then chi CTM
+      sweeps (a fixed count, epsilon = 0); each problem's energy per site.
+      The GPU runs its own simple update from the same initial tensors (the
+      bond SVD's sign convention may differ: a gauge change, the energy is
+      the same).
+
+Run:  python tests/golden/make_oracle_runs.py [random|heisenberg|batch]  (minutes to
 about an hour of CPU; not part of the test suite)."""
 from __future__ import annotations
 
@@ -87,8 +96,51 @@ def heisenberg_case(sweeps: int = 12):
     print(f"E/site = {energy:.15f}", flush=True)
 
 
+def _batch_problem(args):
+    k, n, shape, su_steps, sweeps = args
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(2):
+        lat = SquareLattice(2, 2)
+        D, chi = shape
+        hx = 3.0 * k / (n - 1)
+        Tn = inputs.initial_tensors(4, D, 2, [[1.0, 0.0]] * 4, 0.01, 11)
+        lam = [[np.ones(D) for _ in range(4)] for _ in range(4)]
+        H = inputs.bond_hamiltonian(Jz=-1.0, hx=hx)
+        g = inputs.gate(H, 0.01)
+        Tn, lam = SU.simple_update_steps(Tn, lam, lat, [(s, leg, g) for s, leg in inputs.square_bonds(2, 2)],
+                                         su_steps)
+        p = ctm.PEPSParameters(CHI=chi, Max_CTM_Iteration=sweeps, CTM_Convergence_Epsilon=0.0)
+        env, count, _ = ctm.Calc_CTM_Environment(Tn, p, lat)
+        two = [{"name": "H", "group": 0, "source": s, "dx": 1 if leg == 2 else 0, "dy": 1 if leg == 1 else 0,
+                "op": H} for s, leg in inputs.square_bonds(2, 2)]
+        w, _ = itps.measure_twosite(env, Tn, lat, two, [])
+        energy = float(np.real(sum(w[0].values()))) / 4.0
+    print(f"problem {k}: hx={hx:.4f} D={D} chi={chi} E={energy:.15f}", flush=True)
+    return {"k": k, "hx": hx, "D": D, "chi": chi, "sweeps": count, "energy_per_site": energy}
+
+
+def batch_case(n: int = 48, su_steps: int = 300, sweeps: int = 8, workers: int = 4):
+    from multiprocessing import Pool
+
+    shapes = ((3, 32), (4, 48), (5, 64))
+    jobs = [(k, n, shapes[k % 3], su_steps, sweeps) for k in range(n)]
+    # the D=5 problems first: they take longest
+    jobs.sort(key=lambda j: -j[2][0])
+    t0 = time.time()
+    with Pool(workers) as pool:
+        out = pool.map(_batch_problem, jobs, chunksize=1)
+    out.sort(key=lambda r: r["k"])
+    print(f"batch: {n} problems in {time.time() - t0:.0f} s", flush=True)
+    fx = {"problems": n, "shapes": [list(s) for s in shapes], "Jz": -1.0, "hx_max": 3.0, "su_steps": su_steps,
+          "tau": 0.01, "seed": 11, "noise": 0.01, "sweeps": sweeps, "results": out}
+    (OUT / "tfi_batch48.json").write_text(json.dumps(fx, indent=0))
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["random", "heisenberg"]
+    if "batch" in which:
+        batch_case()
     if "random" in which:
         random_case()
     if "heisenberg" in which:

@@ -66,3 +66,62 @@ def test_batch_randomized_path_and_errors():
     bad = ta.Environment(ctxs[2], 2, 2)
     with pytest.raises(ta.TenesError, match="site tensor not set"):
         ta.core.batch_calc_ctm_environment([envs[0], bad], ta.peps_parameters(CHI=48, Max_CTM_Iteration=1))
+
+
+def _batch48_fixture():
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "tfi_batch48.json")
+    if not os.path.exists(path):
+        pytest.skip("fixture missing")
+    return json.load(open(path))
+
+
+def _batch48_envs(fx):
+    """The 48 problems' site tensors from the GPU simple update (four
+    contexts), then moved to environments of one fresh context."""
+    n, shapes = fx["problems"], tuple(tuple(s) for s in fx["shapes"])
+    su_ctxs = [ta.Context(0, seed=61 + i) for i in range(4)]
+    envs, hams = batch.prepare(su_ctxs, n, su_steps=fx["su_steps"], tau=fx["tau"], seed=fx["seed"],
+                               noise=fx["noise"], shapes=shapes)
+    ctx = ta.Context(0, seed=71)
+    out = []
+    for e in envs:
+        g = ta.Environment(ctx, 2, 2)
+        g.set_site_tensors(e.site_tensors())
+        g.problem = e.problem
+        out.append(g)
+    return ctx, out, hams
+
+
+def test_tfi_batch48_workload_graph_sweeps():
+    """The batched field sweep at its workload: 48 transverse-field Ising
+    problems (Jz = -1, hx_k = 3k/47, (D, chi) cycling (3, 32), (4, 48),
+    (5, 64)), 300 GPU simple-update steps from the seeded ferro state, then
+    the fixture's sweep count on one context in lockstep with captured
+    sweeps (every problem captured once, replayed on the context's lanes).
+    Each problem's energy per site against the oracle's (numpy/LAPACK, its
+    own simple update and exact-SVD CTM: tests/golden/tfi_batch48.json) to
+    1e-10 relative; then the same batch with the contractions on the FP32
+    MFMA (contraction_fp32 = 1, projectors FP64), its energy error stated
+    and bounded."""
+    fx = _batch48_fixture()
+    ctx, envs, hams = _batch48_envs(fx)
+    sweeps = fx["sweeps"]
+    batch.run_ctm(envs, sweeps, True, graph_sweeps=1)
+    n = len(envs)
+    assert ctx.graph_stats() == (n, n * (sweeps - 1))
+    ref = [r["energy_per_site"] for r in fx["results"]]
+    got = [batch.energy(e, H) for e, H in zip(envs, hams)]
+    rel = [abs(g - r) / abs(r) for g, r in zip(got, ref)]
+    worst = int(np.argmax(rel))
+    print(f"FP64 batch: max rel energy err {max(rel):.2e} (problem {worst}, {envs[worst].problem})")
+    assert max(rel) < 1e-10, (worst, envs[worst].problem, got[worst], ref[worst])
+    batch.run_ctm(envs, sweeps, True, graph_sweeps=1, contraction_fp32=1)
+    mixed = [batch.energy(e, H) for e, H in zip(envs, hams)]
+    relm = [abs(g - r) / abs(r) for g, r in zip(mixed, ref)]
+    worst = int(np.argmax(relm))
+    print(f"mixed batch (contraction_fp32=1): max rel energy err {max(relm):.2e} "
+          f"(problem {worst}, {envs[worst].problem}), median {np.median(relm):.1e}")
+    assert max(relm) < 1e-6, (worst, envs[worst].problem, mixed[worst], ref[worst])
