@@ -96,13 +96,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
   const int lane = t & 63;
   const int w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
+  const int M = g.M, N = g.N;
+
   int bm, bn;
   tile_of(blockIdx.y * gridDim.x + blockIdx.x, gridDim.y, gridDim.x, bm, bn);
-  const int m0 = bm * BM;
-  const int n0 = bn * BN;
   const int kbeg = blockIdx.z * ka.k_per_split;
   const int kend = min(g.K, kbeg + ka.k_per_split);
-  const int M = g.M, N = g.N;
+  const int m0 = bm * BM;
+  const int n0 = bn * BN;
 
   // ---- loader geometry ----------------------------------------------------
   // Element p of this thread in the A tile (BM x BK) is e = t + 256 p of the
@@ -157,30 +158,46 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
 
   // Branch-free tile loads: out-of-range elements read offset 0 (a valid
   // address) and are replaced by zero only when the tile is stored to LDS,
-  // so all loads of a tile stay in flight during the MFMA work.
+  // so all loads of a tile stay in flight during the MFMA work.  Loads and
+  // stores are issued in four parts (part q: elements p with p % 4 == q) so
+  // that the main loop can spread them between its MFMA steps.
   double ra[A_PER], rb[B_PER];
   bool oka[A_PER], okb[B_PER];
-  auto load_tile = [&](int k0) {
+  auto load_part = [&](int k0, int q) {
 #pragma unroll
     for (int p = 0; p < A_PER; ++p) {
+      if (A_PER >= 4 && p % 4 != q) continue;
+      if (A_PER < 4 && q != 0) continue;
       const int kk = a_k_of(k0, p);
       oka[p] = a_mok[p] && kk < kend;
       const int koff = KA ? kk * g.a_k_stride : ka_off[A_MFAST ? p : 0];
-      ra[p] = g.A[oka[p] ? a_moff[p] + koff : 0];
+      const int off = a_moff[p] + koff;
+      ra[p] = g.A[oka[p] ? off : 0];
     }
 #pragma unroll
     for (int p = 0; p < B_PER; ++p) {
+      if (B_PER >= 4 && p % 4 != q) continue;
+      if (B_PER < 4 && q != 0) continue;
       const int kk = b_k_of(k0, p);
       okb[p] = b_nok[p] && kk < kend;
       const int koff = KB ? kk * g.b_k_stride : kb_off[B_NFAST ? p : 0];
-      rb[p] = g.B[okb[p] ? b_noff[p] + koff : 0];
+      const int off = b_noff[p] + koff;
+      rb[p] = g.B[okb[p] ? off : 0];
     }
   };
-  auto store_tile = [&](int buf) {
+  auto store_part = [&](int buf, int q) {
 #pragma unroll
-    for (int p = 0; p < A_PER; ++p) As[buf][a_kl[p]][a_ml[p]] = (CT)(oka[p] ? ra[p] : 0.0);
+    for (int p = 0; p < A_PER; ++p) {
+      if (A_PER >= 4 && p % 4 != q) continue;
+      if (A_PER < 4 && q != 0) continue;
+      As[buf][a_kl[p]][a_ml[p]] = (CT)(oka[p] ? ra[p] : 0.0);
+    }
 #pragma unroll
-    for (int p = 0; p < B_PER; ++p) Bs[buf][b_kl[p]][b_nl[p]] = (CT)(okb[p] ? rb[p] : 0.0);
+    for (int p = 0; p < B_PER; ++p) {
+      if (B_PER >= 4 && p % 4 != q) continue;
+      if (B_PER < 4 && q != 0) continue;
+      Bs[buf][b_kl[p]][b_nl[p]] = (CT)(okb[p] ? rb[p] : 0.0);
+    }
   };
 
   acc_t acc[TM][TN];
@@ -194,23 +211,69 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
   const int wrow = wm * (BM / 2);
   const int wcol = wn * (BN / 2);
 
-  // Pipeline: tile k+1 is loaded into registers while tile k is multiplied
-  // from LDS buffer k&1, then stored into the other buffer; one barrier per
-  // K tile.
+  if (BM == 128) {
+    // 128 x 128 tiles: tile t+1 is loaded into registers while tile t is
+    // multiplied from LDS buffer t&1, then stored into the other buffer; one
+    // barrier per K tile.  (The interleaved pipeline below needs ~45 more
+    // registers here, which costs the second workgroup per CU.)
+    if (kbeg < kend) {
+      fetch_koff(kbeg);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) load_part(kbeg, q);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) store_part(0, q);
+      fetch_koff(kbeg + BK);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool next = k0 + BK < kend;
+      if (next) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) load_part(k0 + BK, q);
+        fetch_koff(k0 + 2 * BK);
+      }
+#pragma unroll
+      for (int ks = 0; ks < BK / 4; ++ks) {
+        CT a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = As[buf][ks * 4 + lk][wrow + i * 16 + li];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = Bs[buf][ks * 4 + lk][wcol + j * 16 + li];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<CT>::run(a[i], b[j], acc[i][j]);
+      }
+      if (next) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) store_part(buf ^ 1, q);
+      }
+      __syncthreads();
+      buf ^= 1;
+    }
+  } else {
+  // Pipeline: tile t is multiplied from LDS buffer t&1 while tile t+1 (in
+  // registers since the previous tile) is stored into the other buffer and
+  // tile t+2 is loaded into the freed registers, a quarter of each after
+  // each of the four MFMA steps, so the MFMA pipe keeps running while the
+  // memory instructions issue; one barrier per K tile.  The loop body has
+  // no branches: past the end of the K range the loads are masked (zeros
+  // stored into the buffer no later tile reads).
   if (kbeg < kend) {
     fetch_koff(kbeg);
-    load_tile(kbeg);
-    store_tile(0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) load_part(kbeg, q);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) store_part(0, q);
     fetch_koff(kbeg + BK);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) load_part(kbeg + BK, q);
+    fetch_koff(kbeg + 2 * BK);
   }
   __syncthreads();
   int buf = 0;
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    const bool next = k0 + BK < kend;
-    if (next) {
-      load_tile(k0 + BK);
-      fetch_koff(k0 + 2 * BK);
-    }
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       CT a[TM], b[TN];
@@ -222,10 +285,16 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<CT>::run(a[i], b[j], acc[i][j]);
+      store_part(buf ^ 1, ks);
+      load_part(k0 + 2 * BK, ks);
+      // keep the parts in program order: the compiler would otherwise hoist
+      // all stores (and their waits on the loads) to the top of the tile
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (next) store_tile(buf ^ 1);
+    fetch_koff(k0 + 3 * BK);
     __syncthreads();
     buf ^= 1;
+  }
   }
 
   // ---- epilogue -------------------------------------------------------------
@@ -286,13 +355,26 @@ Plan choose(int M, int N, int K) {
   const int64_t tiles128 = (int64_t)ceil_div(M, 128) * ceil_div(N, 128);
   p.BM = p.BN = (M >= 128 && N >= 128 && tiles128 >= 200) ? 128 : 64;
   const int64_t tiles = (int64_t)ceil_div(M, p.BM) * ceil_div(N, p.BN);
-  // Split K when the tile grid cannot fill the 256 CUs and K is long.
+  // Split K when the tile grid cannot fill the 256 CUs and K is long: the
+  // split count s that minimises the rounds of 256 workgroups per unit of
+  // K work, ceil(tiles s / 256) / s, among those giving at least two
+  // workgroups per CU (64 x 64 tiles run up to four per CU; one alone
+  // leaves the MFMA pipe idle at every barrier), at least four K tiles per
+  // split, fewest splits on a tie.  2500 x 128 x 2500 (80 tiles): 16 splits,
+  // 51.5 us against 64.5 us at the 4 that merely reach 256 workgroups.
   p.splits = 1;
   if (tiles < 128 && K >= 4 * BK * 2) {
-    int want = (int)((256 + tiles - 1) / tiles);
-    int maxs = K / (4 * BK);
-    p.splits = want < maxs ? want : maxs;
-    if (p.splits < 1) p.splits = 1;
+    const int maxs = std::max(1, K / (4 * BK));
+    double best = 1e30;
+    for (int sp = 1; sp <= std::min(maxs, 64); ++sp) {
+      const int64_t wg = tiles * sp;
+      if (wg < 512 && sp < maxs && sp < 64) continue;
+      const double cost = (double)ceil_div(wg, (int64_t)256) / sp;
+      if (cost < best - 1e-12) {
+        best = cost;
+        p.splits = sp;
+      }
+    }
   }
   p.kps = ceil_div(K, p.splits);
   p.kps = ceil_div(p.kps, BK) * BK;
