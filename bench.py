@@ -25,7 +25,8 @@ Passes, in order:
   4. accuracy: a fresh environment run for the oracle's sweep count, energy,
      staggered magnetisation and the correlation lengths against the
      oracle's FP64 (LAPACK SVD) run of the same problem;
-  5. mixed precision (contractions on the FP32 MFMA, modes 1 and 2): the same
+  5. mixed precision (contractions of the moves on the FP32 MFMA, modes 1
+     and 2, or on the bf16 MFMA with split hi/lo operands, mode 3): the same
      timed region and accuracy check;
   6. the CPU baseline: the numpy oracle (same contraction order, LAPACK SVD)
      running one full sweep (8 moves + convergence check) on the GPU's
@@ -269,9 +270,10 @@ def main():
                     help="GEMM autotuning of the context (tuned during the warm-up sweeps)")
     ap.add_argument("--warm-iters", type=int, default=None,
                     help="range-finder iterations of a warm-started projector SVD (library default 8)")
-    ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
+    ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2, 3),
                     help="headline in mixed precision: contractions of the moves on the FP32 MFMA, 1 all but "
-                         "the projector matrix R = LT LB, 2 R as well (projector SVDs and the rest FP64)")
+                         "the projector matrix R = LT LB, 2 R as well; 3 all on the bf16 MFMA with split hi/lo "
+                         "operands (projector SVDs and the rest FP64)")
     ap.add_argument("--host-cut", action="store_true",
                     help="decide the projector cut on the host (one synchronisation per projector, the "
                          "reference's shapes) instead of on the device")
@@ -366,14 +368,17 @@ def main():
     # 5. mixed precision, measured in the same run
     mixed = []
     if not args.no_mixed and not args.mixed:
-        for mode in (1, 2):
+        for mode in (1, 2, 3):
             kwm = dict(kw, contraction_fp32=mode)
             menv = ta.Environment(ctx, 2, 2)
             menv.set_site_tensors(Tn)
             menv.calc_ctm_environment(ta.peps_parameters(Max_CTM_Iteration=args.warmup, **kwm), initialize=True)
             mdt = timed_sweeps(ctx, menv, ta, kwm, args.steps)
             rec = {"mode": mode,
-                   "fp32_contractions": "all but R = LT*LB" if mode == 1 else "all, R = LT*LB included",
+                   "fp32_contractions": {1: "FP32 MFMA: all but R = LT*LB",
+                                         2: "FP32 MFMA: all, R = LT*LB included",
+                                         3: "bf16 MFMA, split hi/lo operands (3 products, FP32 "
+                                            "accumulation): all, R = LT*LB included"}[mode],
                    "sweeps_per_s": args.steps / mdt, "ms_per_step": mdt / args.steps * 1e3}
             log(f"mixed mode {mode}: {args.steps} sweeps in {mdt:.3f} s -> {args.steps / mdt:.3f} sweeps/s")
             if fx is not None and not args.no_check:

@@ -76,9 +76,11 @@ int tenes_ctx_set_max_moves_in_flight(tenes_ctx *ctx, int n);
  * call that failed inside a move.  No counterpart in the reference. */
 int tenes_ctx_lane_state(tenes_ctx *ctx, int *forked, int *lane);
 /* Contractions issued from now on (tenes_tensordot and the rest) run on the
- * FP32 MFMA when on != 0 (FP64 data, FP32 operands and accumulation); the
- * CTM moves set it themselves from tenes_peps_parameters.contraction_fp32.
- * No counterpart in the reference. */
+ * FP32 MFMA when on == 1 (FP64 data, FP32 operands and accumulation), on
+ * the bf16 MFMA as three products of hi/lo bf16 operand halves when on == 2
+ * (FP32 accumulation), in FP64 when on == 0; the CTM moves set it
+ * themselves from tenes_peps_parameters.contraction_fp32.  No counterpart
+ * in the reference. */
 int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on);
 /* GEMM autotuning of a context (on != 0): the first contraction of every
  * plan (operand shapes and legs) with at least 1e8 flops times each candidate
@@ -165,8 +167,10 @@ typedef struct tenes_peps_parameters {
   int svd_warm_power_iters;
   /* Mixed precision (no counterpart in the reference): the contractions of
    * the moves on the FP32 MFMA (FP64 data, FP32 operands and accumulation):
-   * 1 = all but the projector matrix R = LT LB, 2 = R as well; projector
-   * SVDs, QRs, the convergence check, Arnoldi and observables stay FP64.
+   * 1 = all but the projector matrix R = LT LB, 2 = R as well; 3 = all of
+   * them, R included, on the bf16 MFMA as three products of hi/lo bf16
+   * operand halves (split precision, FP32 accumulation); projector SVDs,
+   * QRs, the convergence check, Arnoldi and observables stay FP64.
    * Default 0. */
   int contraction_fp32;
   /* Where the projector cut of ctm.cpp:275-288 is decided inside the

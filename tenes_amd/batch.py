@@ -121,8 +121,8 @@ def main(argv=None):
     ap.add_argument("--lanes", type=int, default=8, help="side streams per context")
     ap.add_argument("--compare-one-context", type=int, default=1, choices=(0, 1),
                     help="also time the problems on one context, uncaptured")
-    ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2),
-                    help="also run the batch with contraction_fp32 = 1 or 2 and report each "
+    ap.add_argument("--mixed", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="also run the batch with contraction_fp32 = 1, 2 or 3 and report each "
                          "problem's energy error against the FP64 batch")
     args = ap.parse_args(argv)
     # read by the HIP runtime when the first context initialises it

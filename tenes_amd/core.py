@@ -76,6 +76,11 @@ class Context:
         """Contractions from now on run on the FP32 MFMA (FP64 data)."""
         self.check(self.lib.tenes_ctx_set_fp32_contractions(self.h, 1 if on else 0))
 
+    def set_contraction_mode(self, mode: int):
+        """Arithmetic of the contractions from now on: 0 FP64, 1 FP32 MFMA,
+        2 bf16 MFMA with split hi/lo operands (FP32 accumulation)."""
+        self.check(self.lib.tenes_ctx_set_fp32_contractions(self.h, int(mode)))
+
     def svd_stats(self):
         """(truncated SVDs, Jacobi rounds) since the last reset_stats()."""
         a = C.c_int64()

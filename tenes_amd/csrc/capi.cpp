@@ -88,6 +88,8 @@ PEPS_Parameters to_params(const tenes_peps_parameters *p) {
   q.svd_warm_start = p->svd_warm_start != 0;
   q.svd_warm_power_iters = p->svd_warm_power_iters;
   q.contraction_fp32 = p->contraction_fp32;
+  TN_REQUIRE(q.contraction_fp32 >= 0 && q.contraction_fp32 <= 3,
+             "contraction_fp32 must be 0, 1, 2 or 3");
   q.projector_cut_on_device = p->projector_cut_on_device != 0;
   q.graph_sweeps = p->graph_sweeps != 0;
   TN_REQUIRE(q.CHI >= 1, "CHI must be positive");
@@ -295,7 +297,10 @@ int tenes_ctx_reset_stats(tenes_ctx *ctx) {
 
 int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on) {
   if (!ctx) return TENES_ERR_ARGUMENT;
-  return guard(ctx, [&] { ctx->c->set_fp32_contractions(on != 0); });
+  return guard(ctx, [&] {
+    TN_REQUIRE(on >= 0 && on <= 2, "contraction mode must be 0, 1 or 2");
+    ctx->c->set_contraction_mode(on);
+  });
 }
 
 int tenes_ctx_set_autotune(tenes_ctx *ctx, int on) {

@@ -184,6 +184,16 @@ struct ProjectorJob {
 
 constexpr int kSyncSlot = tn::Context::kSlotProjectorSync;
 
+// Context contraction modes (0 FP64, 1 FP32, 2 split bf16) of the moves'
+// contractions and of the projector matrix R = LT LB for
+// PEPS_Parameters::contraction_fp32 = 0, 1, 2, 3.
+int move_contraction_mode(const PEPS_Parameters &pp) {
+  return pp.contraction_fp32 == 3 ? 2 : (pp.contraction_fp32 != 0 ? 1 : 0);
+}
+int r_contraction_mode(const PEPS_Parameters &pp) {
+  return pp.contraction_fp32 == 3 ? 2 : (pp.contraction_fp32 == 2 ? 1 : 0);
+}
+
 // From the SVD of the (rows x cols) matrix `Rm`, build the oblique projector
 // pair: PU = Bblk * (VT_c^dagger s^-1/2), PL = Ablk * (U_c^* s^-1/2), both
 // transposed to (e_in, t, tc, cut) (ctm.cpp:290-301 / 466-475).
@@ -318,7 +328,7 @@ void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
   // mixed mode 1 the matrix the SVD truncates stays FP64
   tensor R;
   {
-    tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
+    tn::ContractionPrecision prec(ctx, r_contraction_mode(pp));
     R = tn::tensordot(LT, LB, Axes{3, 4, 5}, Axes{0, 1, 2});
   }
   projectors_begin(ctx, R, t12, e12, b12, t34, e56, b34, cap, LT,
@@ -375,7 +385,7 @@ void updown_blocks_begin(
   if (rsvd) cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * b23);
   tensor M;
   {
-    tn::ContractionPrecision prec(ctx, pp.contraction_fp32 >= 2);
+    tn::ContractionPrecision prec(ctx, r_contraction_mode(pp));
     M = tn::tensordot(R1, R2, Axes{3, 4, 5}, Axes{3, 4, 5});
   }
   projectors_begin(ctx, M, t23, e34, b23, t23, e34, R2.shape[2], cap, R1,
@@ -464,7 +474,7 @@ struct MoveSpec {
 void run_moves(tn::Context &ctx, const PEPS_Parameters &pp,
                const std::vector<MoveSpec *> &moves, int &slot,
                int slot_end = tn::Context::kEagerSlots) {
-  tn::ContractionPrecision prec(ctx, pp.contraction_fp32 != 0);
+  tn::ContractionPrecision prec(ctx, move_contraction_mode(pp));
   const bool dev = pp.projector_cut_on_device;
   int total = 0;
   for (auto *m : moves) total += m->L;

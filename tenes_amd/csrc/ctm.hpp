@@ -44,9 +44,11 @@ struct PEPS_Parameters {
   int svd_warm_power_iters = 8;
   // Mixed precision, the contractions of the four moves on the FP32 MFMA:
   // 1 = enlarged corners, projectors, Calc_Next_CTM / Calc_Next_eT, with the
-  // matrix the SVD truncates (R = LT LB) kept FP64; 2 = R as well.  The
-  // projector SVD and its QRs, the convergence check and everything outside
-  // the moves stay FP64.  0 = all FP64.
+  // matrix the SVD truncates (R = LT LB) kept FP64; 2 = R as well; 3 = all
+  // of them, R included, on the bf16 MFMA as three products of hi/lo bf16
+  // operand halves with FP32 accumulation.  The projector SVD and its QRs,
+  // the convergence check and everything outside the moves stay FP64.
+  // 0 = all FP64.
   int contraction_fp32 = 0;
   // Where the projector cut of ctm.cpp:275-288 is decided.  On the host
   // (default): the reference's variable shapes, one synchronisation per

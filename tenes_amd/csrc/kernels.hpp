@@ -39,6 +39,9 @@ struct GemmArgs {
   // Compute in FP32 on the MFMA (operands rounded to FP32 when they are
   // staged into LDS, FP32 accumulation; HBM data and the result stay FP64).
   bool fp32 = false;
+  // Compute on the bf16 MFMA as three products of hi/lo bf16 halves of the
+  // operands (split precision, FP32 accumulation; FP64 in HBM).
+  bool bf16x3 = false;
 };
 
 // A GEMM launch shape: square output tile (64 or 128) and the number of K
@@ -53,8 +56,13 @@ std::vector<GemmPlan> gemm_candidates(int M, int N, int K);
 size_t gemm_workspace_bytes(int M, int N, int K,
                             const GemmPlan *plan = nullptr);
 // Returns the name of the kernel instantiation launched (static string).
+// g.fp32 / g.bf16x3 select the FP32 or the split bf16 MFMA.
 const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
                      hipStream_t s, const GemmPlan *plan = nullptr);
+// The split bf16 variant with an explicit launch shape (square tile bm = 64
+// or 128, K splits of k_per_split each; split-K partials in workspace).
+const char *gemm_bf16x3(const GemmArgs &g, int bm, int splits, int k_per_split,
+                        void *workspace, size_t ws_bytes, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Elementwise / reduction helpers

@@ -471,6 +471,8 @@ const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
   if (g.M == 0 || g.N == 0) return "gemm_f64_empty";
   // K == 0 gives C = beta*C through a launch with an empty K range.
   Plan p = plan_for(g.M, g.N, g.K, plan);
+  if (g.bf16x3)
+    return gemm_bf16x3(g, p.BM, p.splits, p.kps, workspace, ws_bytes, s);
   KArgs ka;
   ka.g = g;
   ka.partial = nullptr;

@@ -831,7 +831,8 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
   g.a_k_stride = p.a_k_stride;
   g.b_k_stride = p.b_k_stride;
   g.b_n_stride = p.b_n_stride;
-  g.fp32 = ctx.fp32_contractions();
+  g.fp32 = ctx.contraction_mode() == 1;
+  g.bf16x3 = ctx.contraction_mode() == 2;
   const double flops = 2.0 * p.M * (double)p.N * p.K;
   // autotuning (C = A B overwrites C, so every candidate leaves a valid
   // result): time each launch shape twice on this stream, keep the fastest

@@ -260,10 +260,14 @@ class Context {
   };
   void set_profiling(bool on);
   bool profiling() const { return profiling_; }
-  // Contractions issued while this is on run on the FP32 MFMA (FP64 data in
-  // HBM, operands rounded to FP32, FP32 accumulation).
-  void set_fp32_contractions(bool on) { fp32_ = on; }
-  bool fp32_contractions() const { return fp32_; }
+  // Arithmetic of the contractions issued from now on: 0 = FP64 MFMA, 1 =
+  // FP32 MFMA (operands rounded to FP32, FP32 accumulation), 2 = bf16 MFMA
+  // with split operands (three hi/lo products, FP32 accumulation); FP64 data
+  // in HBM in every mode.
+  void set_contraction_mode(int mode) { mode_ = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
+  int contraction_mode() const { return mode_; }
+  void set_fp32_contractions(bool on) { mode_ = on ? 1 : 0; }
+  bool fp32_contractions() const { return mode_ == 1; }
   // Single-layer networks: while on, bra() of a site tensor is the unit
   // tensor of shape (1,1,1,1,1), so the double-layer contractions of the
   // CTM moves and of the environment contractions become those of the
@@ -314,7 +318,7 @@ class Context {
   double *pinned_dev_ = nullptr;
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
-  bool fp32_ = false;
+  int mode_ = 0;
   bool single_layer_ = false;
   bool autotune_ = false;
   bool profiling_ = false;
@@ -360,16 +364,16 @@ class ForkScope {
 // Sets the contraction precision of a context for a scope.
 class ContractionPrecision {
  public:
-  ContractionPrecision(Context &ctx, bool fp32) : ctx_(ctx), old_(ctx.fp32_contractions()) {
-    ctx.set_fp32_contractions(fp32);
+  ContractionPrecision(Context &ctx, int mode) : ctx_(ctx), old_(ctx.contraction_mode()) {
+    ctx.set_contraction_mode(mode);
   }
-  ~ContractionPrecision() { ctx_.set_fp32_contractions(old_); }
+  ~ContractionPrecision() { ctx_.set_contraction_mode(old_); }
   ContractionPrecision(const ContractionPrecision &) = delete;
   ContractionPrecision &operator=(const ContractionPrecision &) = delete;
 
  private:
   Context &ctx_;
-  bool old_;
+  int old_;
 };
 
 // Sets the single-layer mode of a context for a scope.

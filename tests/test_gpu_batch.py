@@ -104,7 +104,8 @@ def test_tfi_batch48_workload_graph_sweeps():
     Each problem's energy per site against the oracle's (numpy/LAPACK, its
     own simple update and exact-SVD CTM: tests/golden/tfi_batch48.json) to
     1e-10 relative; then the same batch with the contractions on the FP32
-    MFMA (contraction_fp32 = 1, projectors FP64), its energy error stated
+    MFMA (contraction_fp32 = 1, projectors FP64) and on the bf16 MFMA with
+    split hi/lo operands (contraction_fp32 = 3), their energy errors stated
     and bounded."""
     fx = _batch48_fixture()
     ctx, envs, hams = _batch48_envs(fx)
@@ -125,3 +126,12 @@ def test_tfi_batch48_workload_graph_sweeps():
     print(f"mixed batch (contraction_fp32=1): max rel energy err {max(relm):.2e} "
           f"(problem {worst}, {envs[worst].problem}), median {np.median(relm):.1e}")
     assert max(relm) < 1e-6, (worst, envs[worst].problem, mixed[worst], ref[worst])
+    # split precision: every contraction of the moves (R included) on the bf16
+    # MFMA as three hi/lo products (~2^-16 per product)
+    batch.run_ctm(envs, sweeps, True, graph_sweeps=1, contraction_fp32=3)
+    split = [batch.energy(e, H) for e, H in zip(envs, hams)]
+    rels = [abs(g - r) / abs(r) for g, r in zip(split, ref)]
+    worst = int(np.argmax(rels))
+    print(f"split bf16 batch (contraction_fp32=3): max rel energy err {max(rels):.2e} "
+          f"(problem {worst}, {envs[worst].problem}), median {np.median(rels):.1e}")
+    assert max(rels) < 1e-4, (worst, envs[worst].problem, split[worst], ref[worst])

@@ -72,6 +72,58 @@ def test_matmul_fp32_compute(gpu_ctx, M, N, K):
     assert _rel(C64, ref) < RTOL
 
 
+def _bf16(x32):
+    """float32 -> nearest-even bfloat16 value, as float32 (finite inputs)."""
+    b = x32.view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def _split_bf16(x):
+    x32 = x.astype(np.float32)
+    hi = _bf16(x32)
+    lo = _bf16((x32 - hi).astype(np.float32))
+    return hi.astype(np.float64), lo.astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K,case", [(16, 16, 32, "nn"), (5, 7, 3, "nn"), (300, 260, 129, "nn"),
+                                        (300, 260, 129, "tn"), (2500, 128, 2500, "tn"),
+                                        (2500, 2500, 64, "nn"), (64, 64, 5000, "tt")])
+def test_matmul_split_bf16(gpu_ctx, M, N, K, case):
+    """The split-precision GEMM (contraction mode 2): both operands split
+    into bf16 halves x = x_hi + x_lo (x_lo = bf16(x - x_hi), from the FP32
+    value), three products A_hi B_hi + A_hi B_lo + A_lo B_hi on the bf16 MFMA
+    with FP32 accumulation.  Against the same three products formed in FP64
+    from the same halves (numpy) the difference is FP32 accumulation only
+    (K * 2^-23 * sum|a||b|); against the FP64 product the split itself adds
+    ~2^-16 per product.  Operand layouts "n"/"t" (row- or column-contiguous
+    contracted index) exercise both loaders; a wrong MFMA lane layout would be
+    an O(1) error at every shape."""
+    rng = np.random.default_rng(M + 3 * N + 7 * K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    At = gpu_ctx.tensor(A.T.copy()) if case[0] == "t" else gpu_ctx.tensor(A)
+    Bt = gpu_ctx.tensor(B.T.copy()) if case[1] == "t" else gpu_ctx.tensor(B)
+    axa = [0] if case[0] == "t" else [1]
+    axb = [1] if case[1] == "t" else [0]
+    gpu_ctx.set_contraction_mode(2)
+    try:
+        C = ta.tensordot(At, Bt, axa, axb).numpy()
+    finally:
+        gpu_ctx.set_contraction_mode(0)
+    Ah, Al = _split_bf16(A)
+    Bh, Bl = _split_bf16(B)
+    emu = Ah @ Bh + Ah @ Bl + Al @ Bh
+    mag = np.abs(A) @ np.abs(B)
+    assert np.all(np.abs(C - emu) <= (K + 2) * 2.0 ** -23 * mag), float(np.max(np.abs(C - emu) / mag))
+    ref = A @ B
+    err = np.max(np.abs(C - ref) / mag)
+    assert err < 2.0 ** -14 + K * 2.0 ** -23, err
+    # the FP64 path is unaffected afterwards
+    C64 = ta.tensordot(At, Bt, axa, axb).numpy()
+    assert _rel(C64, ref) < RTOL
+
+
 def test_transpose(gpu_ctx):
     rng = np.random.default_rng(3)
     A = rng.standard_normal((3, 4, 5, 6, 2))
