@@ -173,6 +173,10 @@ struct ProjectorJob {
   tensor PU, PL;
   tensor Ablk, Bblk;
   Axes axA, axB;
+  // Matrix-free projector matrix (the Use_RSVD branch): PL = finish_a(U)
+  // and PU = finish_b(V) apply the blocks of R to the (r0, r1, rb, k) /
+  // (c0, c1, cb, k) singular vectors; else Ablk, Bblk with axA, axB.
+  std::function<tensor(const tensor &)> finish_a, finish_b;
   int64_t r0 = 0, r1 = 0, rb = 0, c0 = 0, c1 = 0, cb = 0;  // (t, e, tc) legs
   int kmax = 0;
   int slot = 0;  // pinned slot of the singular values
@@ -206,7 +210,8 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
                       int64_t cb, int chi_cap, const tensor &Ablk,
                       const Axes &axA, const tensor &Bblk, const Axes &axB,
                       const PEPS_Parameters &pp, bool rsvd, int slot,
-                      const ProjKey *key, ProjectorJob &job) {
+                      const ProjKey *key, ProjectorJob &job,
+                      const tn::LinearOp *Rop = nullptr) {
   const int64_t nr = r0 * r1 * rb, nc = c0 * c1 * cb;
   job.kmax = (int)std::min<int64_t>(std::min<int64_t>(nr, nc), chi_cap);
   tn::SvdOptions opts = svd_opts(pp, rsvd);
@@ -222,8 +227,8 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
       opts.start_power_iters = pp.svd_warm_power_iters;
     }
   }
-  job.sv = tn::svd_top_begin(ctx, tn::reshape(Rm, Shape{nr, nc}), job.kmax,
-                             opts, slot);
+  job.sv = Rop ? tn::svd_top_begin(ctx, *Rop, job.kmax, opts, slot)
+              : tn::svd_top_begin(ctx, tn::reshape(Rm, Shape{nr, nc}), job.kmax, opts, slot);
   job.warm_store = warm && job.sv.r.Vbasis.defined();
   if (job.warm_store) {
     ctx.svd_start[wkey] = {job.sv.r.Vbasis, job.sv.r.sbasis, 0.0};
@@ -242,6 +247,22 @@ void projectors_begin(tn::Context &ctx, const tensor &Rm, int64_t r0,
   job.cb = cb;
 }
 
+// PU = Bblk V, PL = Ablk U (ctm.cpp:290-301), legs (e_in, t, tc, cut);
+// the blocks are released afterwards.
+void projector_products(ProjectorJob &job, const tensor &Uc, const tensor &Vc) {
+  if (job.finish_a) {
+    job.PU = job.finish_b(Vc);
+    job.PL = job.finish_a(Uc);
+  } else {
+    job.PU = tn::tensordot_t(job.Bblk, Vc, job.axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+    job.PL = tn::tensordot_t(job.Ablk, Uc, job.axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+  }
+  job.Ablk = tensor();
+  job.Bblk = tensor();
+  job.finish_a = nullptr;
+  job.finish_b = nullptr;
+}
+
 void projectors_finish(tn::Context &ctx, ProjectorJob &job,
                        const PEPS_Parameters &pp) {
   if (job.done) return;
@@ -258,12 +279,9 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
                               sc.data(), ctx.stream());
     tensor Uc = tn::reshape(sv.U, Shape{job.r0, job.r1, job.rb, k});
     tensor Vc = tn::reshape(sv.V, Shape{job.c0, job.c1, job.cb, k});
-    job.PU = tn::tensordot_t(job.Bblk, Vc, job.axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
-    job.PL = tn::tensordot_t(job.Ablk, Uc, job.axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+    projector_products(job, Uc, Vc);
     tn::multiply_vector_dev(job.PU, sc.data(), 3);
     tn::multiply_vector_dev(job.PL, sc.data(), 3);
-    job.Ablk = tensor();
-    job.Bblk = tensor();
     job.done = true;
     return;
   }
@@ -287,10 +305,7 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
   tn::multiply_vector_dev(Vc, sc.data(), 1);
   Uc = tn::reshape(Uc, Shape{job.r0, job.r1, job.rb, cut});
   Vc = tn::reshape(Vc, Shape{job.c0, job.c1, job.cb, cut});
-  job.PU = tn::tensordot_t(job.Bblk, Vc, job.axB, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
-  job.PL = tn::tensordot_t(job.Ablk, Uc, job.axA, Axes{0, 1, 2}, Axes{1, 0, 2, 3});
-  job.Ablk = tensor();
-  job.Bblk = tensor();
+  projector_products(job, Uc, Vc);
   job.done = true;
 }
 
@@ -324,6 +339,27 @@ void left_block_begin(const tensor &C1, const tensor &C4, const tensor &eT1,
         std::min<int64_t>(std::min<int64_t>(pp.CHI, e78 * t41 * b41),
                           e12 * t12 * b12),
         e56 * t34 * b34);
+  if (rsvd) {
+    // Mult_col / Mult_row (ctm.cpp:59-119): the RSVD branch never forms R;
+    // the range finder multiplies by LT (LB X) and LB^T (LT^T Y)
+    const int64_t nr = t12 * e12 * b12, nc = t34 * e56 * b34, inner = LT.size() / nr;
+    const tensor LTm = tn::reshape(LT, Shape{nr, inner}), LBm = tn::reshape(LB, Shape{inner, nc});
+    const int mode = r_contraction_mode(pp);
+    tn::LinearOp op;
+    op.m = nr;
+    op.n = nc;
+    op.mul = [&ctx, LTm, LBm, mode](const tensor &X) {
+      tn::ContractionPrecision prec(ctx, mode);
+      return tn::tensordot(LTm, tn::tensordot(LBm, X, Axes{1}, Axes{0}), Axes{1}, Axes{0});
+    };
+    op.mulT = [&ctx, LTm, LBm, mode](const tensor &Y) {
+      tn::ContractionPrecision prec(ctx, mode);
+      return tn::tensordot(LBm, tn::tensordot(LTm, Y, Axes{0}, Axes{0}), Axes{0}, Axes{0});
+    };
+    projectors_begin(ctx, tensor(), t12, e12, b12, t34, e56, b34, cap, LT, Axes{0, 1, 2}, LB,
+                     Axes{3, 4, 5}, pp, rsvd, slot, key, job, &op);
+    return;
+  }
   // R = LT * LB, legs (t12, e12, tc12, t34, e56, tc34)  (ctm.cpp:273); in
   // mixed mode 1 the matrix the SVD truncates stays FP64
   tensor R;
@@ -377,6 +413,51 @@ void updown_blocks_begin(
   TN_REQUIRE(!ctx.single_layer(),
              "up-down projectors of the single-layer network are not supported");
   const bool rsvd = t23 != 1 && pp.Use_RSVD;
+  if (rsvd) {
+    // Mult_col_ud / Mult_row_ud (ctm.cpp:121-172): M = (RT LT)(RB LB)^T is
+    // applied block by block, RT (LT (LB (RB X))), and neither M nor RT LT,
+    // RB LB is formed; the projectors follow the same chains (ctm.cpp:400-439):
+    // PL = LT (RT U), PU = LB (RB V).  Stored legs: RT (t12, t23, e34, e12,
+    // tc12, tc23), RB (t34, t23, e56, e34, tc34, tc23), LT (t12, e12, tc12 |
+    // t41, e78, tc41), LB (t41, e78, tc41 | t34, e56, tc34).
+    const int64_t b23r = RT.shape[5], b23c = RB.shape[5];
+    const int cap = (int)std::min<int64_t>(pp.CHI, e34 * t23 * b23r);
+    const int mode = r_contraction_mode(pp);
+    tn::LinearOp op;
+    op.m = t23 * e34 * b23r;
+    op.n = t23 * e34 * b23c;
+    op.mul = [&ctx, LT, RT, RB, LB, mode, t23, e34, b23c](const tensor &X) {
+      tn::ContractionPrecision prec(ctx, mode);
+      const int64_t p = X.shape[1];
+      tensor W = tn::tensordot(RB, tn::reshape(X, Shape{t23, e34, b23c, p}), Axes{1, 3, 5},
+                               Axes{0, 1, 2});                            // (t34, e56, tc34, p)
+      W = tn::tensordot(LB, W, Axes{3, 4, 5}, Axes{0, 1, 2});            // (t41, e78, tc41, p)
+      W = tn::tensordot(LT, W, Axes{3, 4, 5}, Axes{0, 1, 2});            // (t12, e12, tc12, p)
+      W = tn::tensordot(RT, W, Axes{0, 3, 4}, Axes{0, 1, 2});            // (t23, e34, tc23, p)
+      return tn::reshape(W, Shape{W.size() / p, p});
+    };
+    op.mulT = [&ctx, LT, RT, RB, LB, mode, t23, e34, b23r](const tensor &Y) {
+      tn::ContractionPrecision prec(ctx, mode);
+      const int64_t p = Y.shape[1];
+      tensor W = tn::tensordot(RT, tn::reshape(Y, Shape{t23, e34, b23r, p}), Axes{1, 2, 5},
+                               Axes{0, 1, 2});                            // (t12, e12, tc12, p)
+      W = tn::tensordot(LT, W, Axes{0, 1, 2}, Axes{0, 1, 2});            // (t41, e78, tc41, p)
+      W = tn::tensordot(LB, W, Axes{0, 1, 2}, Axes{0, 1, 2});            // (t34, e56, tc34, p)
+      W = tn::tensordot(RB, W, Axes{0, 2, 4}, Axes{0, 1, 2});            // (t23, e34, tc23, p)
+      return tn::reshape(W, Shape{W.size() / p, p});
+    };
+    projectors_begin(ctx, tensor(), t23, e34, b23r, t23, e34, b23c, cap, tensor(), Axes{}, tensor(),
+                     Axes{}, pp, rsvd, slot, key, job, &op);
+    job.finish_a = [LT, RT](const tensor &U) {
+      return tn::tensordot_t(LT, tn::tensordot(RT, U, Axes{1, 2, 5}, Axes{0, 1, 2}), Axes{0, 1, 2},
+                             Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+    };
+    job.finish_b = [LB, RB](const tensor &V) {
+      return tn::tensordot_t(LB, tn::tensordot(RB, V, Axes{1, 3, 5}, Axes{0, 1, 2}), Axes{3, 4, 5},
+                             Axes{0, 1, 2}, Axes{1, 0, 2, 3});
+    };
+    return;
+  }
   // R1 = RT*LT, R2 = RB*LB, legs (t23, e34, tc23, t41, e78, tc41)
   tensor R1 = tn::tensordot(RT, LT, Axes{0, 3, 4}, Axes{0, 1, 2});
   tensor R2 = tn::tensordot(RB, LB, Axes{0, 2, 4}, Axes{3, 4, 5});

@@ -105,12 +105,54 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
   return Y;
 }
 
+namespace {
+
+// The m x n matrix of a LinearOp with min(m, n) <= 128, from its products
+// with the identity.
+Tensor form_matrix(Context &ctx, const LinearOp &op) {
+  if (op.n <= op.m) {
+    Tensor I = empty(ctx, Shape{op.n, op.n});
+    kern::eye(I.data(), op.n, ctx.stream());
+    return op.mul(I);  // m x n
+  }
+  Tensor I = empty(ctx, Shape{op.m, op.m});
+  kern::eye(I.data(), op.m, ctx.stream());
+  return transpose(op.mulT(I), Axes{1, 0});  // (n x m)^T
+}
+
+SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op,
+                              int k_target, const SvdOptions &opt, int slot);
+
+}  // namespace
+
+SvdPending svd_top_begin(Context &ctx, const LinearOp &op, int k_target,
+                         const SvdOptions &opt, int slot) {
+  TN_REQUIRE(op.m >= 1 && op.n >= 1 && op.mul && op.mulT, "svd_top: bad linear operator");
+  if (std::min(op.m, op.n) <= kJac) {
+    const Tensor A = form_matrix(ctx, op);
+    return svd_top_begin_impl(ctx, &A, nullptr, k_target, opt, slot);
+  }
+  return svd_top_begin_impl(ctx, nullptr, &op, k_target, opt, slot);
+}
+
 SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
                          const SvdOptions &opt, int slot) {
+  return svd_top_begin_impl(ctx, &A, nullptr, k_target, opt, slot);
+}
+
+namespace {
+
+SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op,
+                              int k_target, const SvdOptions &opt, int slot) {
   TN_REQUIRE(slot >= 0 && slot < Context::kPinnedSlots, "svd_top_begin: bad pinned slot");
   ContractionPrecision fp64(ctx, false);  // the SVD and its QRs stay FP64
-  TN_REQUIRE(A.rank() == 2, "svd_top: 2-D input expected");
-  const int m = (int)A.shape[0], n = (int)A.shape[1];
+  // products with A: the explicit matrix, or the operator's (which keeps
+  // the arithmetic its caller chose)
+  const Tensor A = Ap ? *Ap : Tensor();
+  auto prod = [&](const Tensor &X) { return Ap ? tensordot(A, X, Axes{1}, Axes{0}) : op->mul(X); };
+  auto prodT = [&](const Tensor &Y) { return Ap ? tensordot(A, Y, Axes{0}, Axes{0}) : op->mulT(Y); };
+  TN_REQUIRE(!Ap || A.rank() == 2, "svd_top: 2-D input expected");
+  const int m = Ap ? (int)A.shape[0] : (int)op->m, n = Ap ? (int)A.shape[1] : (int)op->n;
   TN_REQUIRE(m >= 1 && n >= 1, "svd_top: empty matrix");
   TN_REQUIRE(k_target >= 1, "svd_top: k must be positive");
   TN_REQUIRE(k_target <= kJac,
@@ -174,19 +216,15 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
     // bases); a Gaussian block mixes all directions and needs the shifted
     // passes.
     const int mid = warm ? 1 : 2, fin = warm ? 2 : 3;
-    Tensor Q = orthonormalize(ctx, tensordot(A, Om, Axes{1}, Axes{0}), nullptr,
-                              q == 0 ? fin : mid, !warm);
+    Tensor Q = orthonormalize(ctx, prod(Om), nullptr, q == 0 ? fin : mid, !warm);
     const bool skip_z =
         warm && opt.start_range > 0.0 && opt.start_range < 1e7;
     for (int it = 0; it < q; ++it) {
-      Tensor Z = skip_z ? tensordot(A, Q, Axes{0}, Axes{0})
-                        : orthonormalize(ctx, tensordot(A, Q, Axes{0}, Axes{0}),
-                                         nullptr, mid, !warm);
-      Q = orthonormalize(ctx, tensordot(A, Z, Axes{1}, Axes{0}), nullptr,
-                         it + 1 == q ? fin : mid, !warm);
+      Tensor Z = skip_z ? prodT(Q) : orthonormalize(ctx, prodT(Q), nullptr, mid, !warm);
+      Q = orthonormalize(ctx, prod(Z), nullptr, it + 1 == q ? fin : mid, !warm);
     }
     Tensor R2;
-    Tensor Bt = tensordot(A, Q, Axes{0}, Axes{0});  // n x p = A^T Q
+    Tensor Bt = prodT(Q);  // n x p = A^T Q
     Rq = orthonormalize(ctx, Bt, &R2, fin, !warm);  // Bt = Rq R2
     Lq = Q;
     have_L = have_R = randomized = true;
@@ -231,6 +269,8 @@ SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
   pd.nall = cols;
   return pd;
 }
+
+}  // namespace
 
 SvdResult svd_top_end(SvdPending &pd) {
   TN_REQUIRE(pd.ctx != nullptr, "svd_top_end: no pending SVD");

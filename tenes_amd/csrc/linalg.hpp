@@ -1,6 +1,7 @@
 // GPU FP64 SVD used by the CTM projectors and the convergence check.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "tensor.hpp"
@@ -63,6 +64,17 @@ struct SvdPending {
   int nall = 0;
 };
 SvdPending svd_top_begin(Context &ctx, const Tensor &A, int k_target,
+                         const SvdOptions &opt, int slot);
+
+// An m x n matrix known only through its products (matrix-free): mul(X) =
+// A X for X (n x p), mulT(Y) = A^T Y for Y (m x p), both row-major 2-D
+// tensors.  The randomized path calls only these; when min(m, n) <= 128 the
+// matrix is formed from the products with the identity (n or m columns).
+struct LinearOp {
+  int64_t m = 0, n = 0;
+  std::function<Tensor(const Tensor &)> mul, mulT;
+};
+SvdPending svd_top_begin(Context &ctx, const LinearOp &A, int k_target,
                          const SvdOptions &opt, int slot);
 SvdResult svd_top_end(SvdPending &pending);
 // The device results of a pending SVD without waiting for it (s and s_all

@@ -310,6 +310,42 @@ def test_ctm_environment_updown(gpu_ctx):
     assert np.max(np.abs(gmz - mz)) < 1e-10
 
 
+@pytest.mark.parametrize("corner", [1, 0])
+def test_rsvd_matrix_free_range_finder(gpu_ctx, corner):
+    """The Use_RSVD branch at a size where the projector matrices are wider
+    than 128 (unit-norm random tensors, D = 4, chi = 40: 640 x 640), so the
+    range finder runs on the matrix-free operators of ctm.cpp:59-172 —
+    LT (LB X) for the corner projectors, RT (LT (LB (RB X))) for the up-down
+    ones, and the up-down projectors through the same block chains
+    (ctm.cpp:400-439) — without forming R.  With 12 power iterations the
+    randomized SVD equals the exact one to rounding, so the run equals the
+    oracle's (exact SVD with the branch's cut): energy and <Sz> to 1e-10,
+    corner spectra to 1e-9.  (Random tensors: a symmetric state's degenerate
+    multiplets split by the cut at chi would leave the kept basis, and with
+    it the corner spectra, to the SVD's choice.)"""
+    Tn = random_tensors(4)
+    ham = models.bond_hamiltonian(Jz=1.0, Jx=1.0, Jy=1.0)
+    lat = SquareLattice(2, 2)
+    p = O.PEPSParameters(CHI=40, Use_RSVD=True, CTM_Projector_corner=bool(corner), Max_CTM_Iteration=3,
+                         CTM_Convergence_Epsilon=0.0)
+    env, count, _ = O.Calc_CTM_Environment(list(Tn), p, lat)
+    genv = ta.Environment(gpu_ctx, 2, 2)
+    genv.set_site_tensors(list(Tn))
+    gcount, _ = genv.calc_ctm_environment(ta.peps_parameters(
+        CHI=40, Use_RSVD=1, CTM_Projector_corner=corner, Max_CTM_Iteration=3, CTM_Convergence_Epsilon=0.0,
+        rsvd_power_iters=12))
+    assert gcount == count == 3
+    E, mz = observables_oracle(env, Tn, lat, ham)
+    gE, gmz = observables_gpu(genv, lat, ham)
+    assert abs(gE - E) < 1e-10 * abs(E), (gE, E)
+    assert np.max(np.abs(gmz - mz)) < 1e-10
+    for i in range(4):
+        for k in ("C1", "C2", "C3", "C4"):
+            a = O.normalized_spectrum(getattr(env, k)[i])
+            b = O.normalized_spectrum(genv.get(k, i).numpy())
+            assert spectra_diff(a, b) < 1e-9, (k, i, spectra_diff(a, b))
+
+
 def test_ctm_environment_rsvd_updown(gpu_ctx):
     """The Use_RSVD branch with up-down projectors (RSVD.toml's CTM settings:
     chi = 5, projector_corner = false; ctm.cpp:401-439), whose oracle is pinned
