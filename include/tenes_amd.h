@@ -82,6 +82,13 @@ int tenes_ctx_lane_state(tenes_ctx *ctx, int *forked, int *lane);
  * themselves from tenes_peps_parameters.contraction_fp32.  No counterpart
  * in the reference. */
 int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on);
+/* The two contractions of a block tensor with the ket and the bra site
+ * tensor in the enlarged corners and in Calc_Next_eT (ctm.cpp:209-212,
+ * 224-227, 532-536) run as one kernel that keeps their intermediate on chip
+ * (on != 0, the default; FP64 double-layer networks within its size limits)
+ * or as the reference's two contractions (on == 0).  The results agree to
+ * rounding.  No counterpart in the reference. */
+int tenes_ctx_set_fused_site(tenes_ctx *ctx, int on);
 /* GEMM autotuning of a context (on != 0): the first contraction of every
  * plan (operand shapes and legs) with at least 1e8 flops times each candidate
  * launch shape of the gather-GEMM (64 or 128 tiles, split-K counts) twice on

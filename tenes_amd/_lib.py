@@ -113,6 +113,7 @@ _SIGS = {
     "tenes_env_contract_two_sites_op12": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
     "tenes_ctx_set_fp32_contractions": (C.c_int, [C.c_void_p, C.c_int]),
+    "tenes_ctx_set_fused_site": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_lane_state": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "tenes_ctx_set_max_moves_in_flight": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_graph_stats": (C.c_int, [C.c_void_p, c_i64_p, c_i64_p]),

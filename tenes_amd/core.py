@@ -76,6 +76,11 @@ class Context:
         """Contractions from now on run on the FP32 MFMA (FP64 data)."""
         self.check(self.lib.tenes_ctx_set_fp32_contractions(self.h, 1 if on else 0))
 
+    def set_fused_site(self, on: bool):
+        """The enlarged corners' and Calc_Next_eT's pair of site contractions
+        as one kernel (default) or as the reference's two contractions."""
+        self.check(self.lib.tenes_ctx_set_fused_site(self.h, 1 if on else 0))
+
     def set_contraction_mode(self, mode: int):
         """Arithmetic of the contractions from now on: 0 FP64, 1 FP32 MFMA,
         2 bf16 MFMA with split hi/lo operands (FP32 accumulation)."""

@@ -303,6 +303,11 @@ int tenes_ctx_set_fp32_contractions(tenes_ctx *ctx, int on) {
   });
 }
 
+int tenes_ctx_set_fused_site(tenes_ctx *ctx, int on) {
+  if (!ctx) return TENES_ERR_ARGUMENT;
+  return guard(ctx, [&] { ctx->c->set_fused_site(on != 0); });
+}
+
 int tenes_ctx_set_autotune(tenes_ctx *ctx, int on) {
   if (!ctx) return TENES_ERR_ARGUMENT;
   return guard(ctx, [&] { ctx->c->set_autotune(on != 0); });

@@ -135,6 +135,54 @@ tensor identity_projector(tn::Context &ctx, int64_t e78, int64_t t41) {
   return tn::reshape(I, Shape{e78, t41, t41, e78});
 }
 
+// The two contractions of a block tensor with the ket and the bra site
+// tensor that follow each other in the enlarged corners (ctm.cpp:209-212,
+// 224-227) and in Calc_Next_eT (ctm.cpp:532-536), as one kernel
+// (kern::site_apply).  legA, legB are the legs of Tn (l, u, r, d, p) met by
+// the block tensor's ket legs kA, kB (its bra legs bA, bB meet the same
+// legs of the bra); the result keeps Tn's other two of (l, u, r, d) in
+// increasing order.  The fused kernel applies to FP64 double-layer networks
+// with all four bond dimensions equal to D <= 5 and d = 2; elsewhere the
+// callers run the two contractions.
+bool fused_site_ok(const tensor &Tn) {
+  tn::Context &ctx = *Tn.ctx();
+  if (!ctx.fused_site() || ctx.contraction_mode() != 0 || ctx.single_layer() || Tn.rank() != 5)
+    return false;
+  const int64_t D = Tn.shape[0];
+  for (int l = 1; l < 4; ++l)
+    if (Tn.shape[l] != D) return false;
+  return tn::kern::site_apply_supported((int)D, (int)Tn.shape[4]);
+}
+
+// X block-major (s, f, kA, kB, bA, bB); result corner layout (oA, s, boA,
+// oB, f, boB) or block-major (s, f, oA, oB, boA, boB) (edge).
+tensor fused_site(const tensor &X, const tensor &Tn, int legA, int legB, bool edge) {
+  tn::Context &ctx = *X.ctx();
+  int o[2], n = 0;
+  for (int l = 0; l < 4; ++l)
+    if (l != legA && l != legB) o[n++] = l;
+  const int D = (int)Tn.shape[0];
+  TN_REQUIRE(X.rank() == 6 && X.shape[2] == D && X.shape[3] == D && X.shape[4] == D &&
+                 X.shape[5] == D,
+             "fused site contraction: block tensor shape");
+  tn::kern::SiteArgs a;
+  a.Ds = (int)X.shape[0];
+  a.Df = (int)X.shape[1];
+  a.D = D;
+  a.p = (int)Tn.shape[4];
+  // operand arrays (real tensors: the bra is the ket)
+  const tensor TK = tn::transpose(Tn, Axes{legA, legB, 4, o[0], o[1]});
+  const tensor TB = tn::transpose(Tn, Axes{legA, legB, o[0], o[1], 4});
+  tensor out = edge ? tn::empty(ctx, Shape{a.Ds, a.Df, D, D, D, D})
+                    : tn::empty(ctx, Shape{D, a.Ds, D, D, a.Df, D});
+  a.X = X.data();
+  a.TK = TK.data();
+  a.TB = TB.data();
+  a.out = out.data();
+  tn::kern::site_apply(a, edge, ctx.stream());
+  return out;
+}
+
 // Enlarged corner of ctm.cpp:207-212 (and its rotated uses):
 // LT = Tn1*(((C1*eT1)*eT8)*Tn1c), reference legs (t12, t41, e12, e78, tc12,
 // tc41).  Stored with the legs grouped as a matrix,
@@ -143,23 +191,25 @@ tensor identity_projector(tn::Context &ctx, int64_t e78, int64_t t41) {
 // it then read plain row-major operands.
 tensor enlarged_LT(const tensor &C1, const tensor &eT1, const tensor &eT8,
                    const tensor &Tn1) {
-  return tn::tensordot_t(
-      Tn1,
-      tn::tensordot(tn::tensordot(tn::tensordot(C1, eT1, Axes{1}, Axes{0}),
-                                  eT8, Axes{0}, Axes{1}),
-                    tn::bra(Tn1), Axes{2, 5}, Axes{1, 0}),
-      Axes{0, 1, 4}, Axes{3, 1, 6}, Axes{0, 2, 4, 1, 3, 5});
+  const tensor T1 = tn::tensordot(C1, eT1, Axes{1}, Axes{0});
+  if (fused_site_ok(Tn1))  // (c, e, k1, k8, b1, b8): eT1's legs meet Tn1's u, eT8's its l
+    return fused_site(tn::tensordot_t(T1, eT8, Axes{0}, Axes{1}, Axes{0, 3, 1, 4, 2, 5}), Tn1, 1, 0,
+                      false);
+  const tensor T2 = tn::tensordot(T1, eT8, Axes{0}, Axes{1});  // (c, k1, b1, e, k8, b8)
+  return tn::tensordot_t(Tn1, tn::tensordot(T2, tn::bra(Tn1), Axes{2, 5}, Axes{1, 0}),
+                         Axes{0, 1, 4}, Axes{3, 1, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
 // ctm.cpp:222-227: LB = Tn4*(((C4*eT7)*eT6)*Tn4c), reference legs (t41,
 // t34, e78, e56, tc41, tc34); stored as (t41, e78, tc41 | t34, e56, tc34).
 tensor enlarged_LB(const tensor &C4, const tensor &eT7, const tensor &eT6,
                    const tensor &Tn4) {
-  return tn::tensordot_t(
-      Tn4,
-      tn::tensordot(tn::tensordot(tn::tensordot(C4, eT7, Axes{1}, Axes{0}),
-                                  eT6, Axes{0}, Axes{1}),
-                    tn::bra(Tn4), Axes{2, 5}, Axes{0, 3}),
-      Axes{0, 3, 4}, Axes{1, 3, 6}, Axes{0, 2, 4, 1, 3, 5});
+  const tensor T1 = tn::tensordot(C4, eT7, Axes{1}, Axes{0});
+  if (fused_site_ok(Tn4))  // (c, e, k7, k6, b7, b6): eT7's legs meet Tn4's l, eT6's its d
+    return fused_site(tn::tensordot_t(T1, eT6, Axes{0}, Axes{1}, Axes{0, 3, 1, 4, 2, 5}), Tn4, 0, 3,
+                      false);
+  const tensor T2 = tn::tensordot(T1, eT6, Axes{0}, Axes{1});  // (c, k7, b7, e, k6, b6)
+  return tn::tensordot_t(Tn4, tn::tensordot(T2, tn::bra(Tn4), Axes{2, 5}, Axes{0, 3}),
+                         Axes{0, 3, 4}, Axes{1, 3, 6}, Axes{0, 2, 4, 1, 3, 5});
 }
 
 // One projector computation, split at its only host synchronisation (the
@@ -521,11 +571,17 @@ void Calc_Next_CTM(const tensor &C1, const tensor &C4, const tensor &eT1,
 // fused into the last contraction's store.
 void Calc_Next_eT(const tensor &eT8, const tensor &Tn1, const tensor &PU,
                   const tensor &PL, tensor &eT_out) {
-  tensor X = tn::tensordot(eT8, PL, Axes{1}, Axes{0});
-  tensor Y = tn::tensordot(tn::bra(Tn1), X, Axes{0, 1}, Axes{2, 4});
-  tensor Z = tn::tensordot(Tn1, Y, Axes{0, 1, 4}, Axes{4, 5, 2});
-  eT_out = tn::tensordot_t(Z, PU, Axes{1, 3, 4}, Axes{1, 2, 0},
-                           Axes{3, 2, 0, 1});
+  if (fused_site_ok(Tn1)) {
+    // (a, c, k8, t, b8, tc): eT8's ket / bra legs meet Tn1's l, PL's its u
+    const tensor X = tn::tensordot_t(eT8, PL, Axes{1}, Axes{0}, Axes{0, 5, 1, 3, 2, 4});
+    const tensor Z = fused_site(X, Tn1, 0, 1, true);  // (a, c, r, d, r', d')
+    eT_out = tn::tensordot_t(Z, PU, Axes{3, 5, 0}, Axes{1, 2, 0}, Axes{3, 0, 1, 2});
+  } else {
+    const tensor X = tn::tensordot(eT8, PL, Axes{1}, Axes{0});  // (a, k8, b8, t, tc, c)
+    const tensor Y = tn::tensordot(tn::bra(Tn1), X, Axes{0, 1}, Axes{2, 4});
+    const tensor Z = tn::tensordot(Tn1, Y, Axes{0, 1, 4}, Axes{4, 5, 2});  // (r, d, r', d', a, c)
+    eT_out = tn::tensordot_t(Z, PU, Axes{1, 3, 4}, Axes{1, 2, 0}, Axes{3, 2, 0, 1});
+  }
   tn::normalize_max_abs(eT_out);
 }
 

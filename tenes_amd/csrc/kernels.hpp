@@ -65,6 +65,26 @@ const char *gemm_bf16x3(const GemmArgs &g, int bm, int splits, int k_per_split,
                         void *workspace, size_t ws_bytes, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// Double-layer site application (kernels_site.hip): for every block (s, f)
+// of the environment legs of X (Ds x Df blocks, X block-major (s, f, kA,
+// kB, bA, bB), all inner legs of dimension D),
+//   out[(oA,oB),(boA,boB)] = sum TK[(kA,kB),p,(oA,oB)] TB[(bA,bB),(boA,boB),p]
+//                                X[(kA,kB),(bA,bB)],
+// written in the corner layout (oA, s, boA, oB, f, boB) or block-major
+// (edge: s, f, oA, oB, boA, boB).  TK (kA kB, p, oA oB) and TB (bA bB,
+// boA boB, p) contiguous.  Built for D = 2..5 and physical dimension 2.
+// ---------------------------------------------------------------------------
+struct SiteArgs {
+  const double *X;
+  const double *TK;
+  const double *TB;
+  double *out;
+  int Ds, Df, D, p;
+};
+bool site_apply_supported(int D, int p);
+void site_apply(const SiteArgs &a, bool edge, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // Elementwise / reduction helpers
 // ---------------------------------------------------------------------------
 void fill(double *p, int64_t n, double v, hipStream_t s);

@@ -268,6 +268,11 @@ class Context {
   int contraction_mode() const { return mode_; }
   void set_fp32_contractions(bool on) { mode_ = on ? 1 : 0; }
   bool fp32_contractions() const { return mode_ == 1; }
+  // The HBM-bound pairs of the enlarged corners and Calc_Next_eT as one
+  // double-layer site kernel (FP64, double-layer networks; on by default);
+  // off: the two gather-GEMMs of the reference's contraction order.
+  void set_fused_site(bool on) { fused_site_ = on; }
+  bool fused_site() const { return fused_site_; }
   // Single-layer networks: while on, bra() of a site tensor is the unit
   // tensor of shape (1,1,1,1,1), so the double-layer contractions of the
   // CTM moves and of the environment contractions become those of the
@@ -319,6 +324,7 @@ class Context {
   std::map<std::string, ContractPlan> plans_;
   uint64_t rng_counter_ = 0;
   int mode_ = 0;
+  bool fused_site_ = true;
   bool single_layer_ = false;
   bool autotune_ = false;
   bool profiling_ = false;
