@@ -233,9 +233,13 @@ def timed_sweeps(ctx, env, ta, kw, steps):
 def probe_gemm():
     """Workload for counter collection (rocprofv3 --pmc): ten launches of the
     dominant CTMRG GEMM shape (R = LT*LB at D=5, chi=100: 2500 x 2500 x 2500
-    FP64, row-major operands) and five launches of a plain copy kernel of
-    known size (4096 x 4096 FP64 transpose: 128 MiB read, 128 MiB written)
-    whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    FP64, row-major operands), five Calc_Next_eT at D=5, chi=100 (each one
+    fused double-layer site kernel, edge layout: 50 MB read, 50 MB written,
+    besides its two GEMMs), one left-block projector at D=5, chi=100 (two
+    fused site kernels, corner layout, same bytes each), and five launches
+    of a plain copy kernel of known size (4096 x 4096 FP64 transpose: 128 MiB
+    read, 128 MiB written) whose counter readings calibrate FETCH_SIZE /
+    WRITE_SIZE."""
     import tenes_amd as ta
 
     ctx = ta.Context(0)
@@ -246,11 +250,24 @@ def probe_gemm():
     for _ in range(10):
         ta.tensordot(A, B, [1], [0])
     ctx.synchronize()
+    chi, D = 100, 5
+    t = ctx.tensor
+    eT = t(rng.standard_normal((chi, chi, D, D)))
+    Tn = t(rng.standard_normal((D, D, D, D, 2)))
+    PU = t(rng.standard_normal((chi, D, D, chi)))
+    PL = t(rng.standard_normal((chi, D, D, chi)))
+    for _ in range(5):
+        ta.calc_next_et(eT, Tn, PU, PL)
+    C = [t(rng.standard_normal((chi, chi))) for _ in range(2)]
+    E = [t(rng.standard_normal((chi, chi, D, D))) for _ in range(4)]
+    ta.calc_projector_left_block(C[0], C[1], E[0], E[1], E[2], E[3], Tn, Tn, ta.peps_parameters(CHI=chi))
+    ctx.synchronize()
     X = ctx.tensor(rng.standard_normal((4096, 4096)))
     for _ in range(5):
         ta.transpose(X, [1, 0])
     ctx.synchronize()
-    print("probe: 10 GEMM 2500^3 + 5 copies of 128 MiB done", flush=True)
+    print("probe: 10 GEMM 2500^3, 5 Calc_Next_eT, 1 left-block projector, 5 copies of 128 MiB done",
+          flush=True)
 
 
 def main():
