@@ -289,7 +289,8 @@ int tenes_env_save_corners(tenes_env *env);
 int tenes_env_check_convergence(tenes_env *env, const tenes_peps_parameters *p,
                                 int *converged, double *sig_max);
 /* L2-normalised singular spectrum of one corner (diagnostics):
- * writes min(chi, 128) values, *n_out receives the count. */
+ * writes the corner's min(rows, cols) <= 256 values, *n_out receives the
+ * count. */
 int tenes_env_corner_spectrum(tenes_env *env, int kind, int site, double *out,
                               int *n_out);
 

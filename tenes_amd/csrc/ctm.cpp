@@ -916,7 +916,7 @@ std::vector<std::vector<double>> spectra(const std::vector<const tensor *> &Cs) 
     for (int i : g.second) mats.push_back(*Cs[i]);
     bool big = g.first[0] > tn::kern::jacobi_max_dim() ||
                g.first[1] > tn::kern::jacobi_max_dim();
-    TN_REQUIRE(!big, "convergence check: corner matrices above 128 x 128 are "
+    TN_REQUIRE(!big, "convergence check: corner matrices above 256 x 256 are "
                      "not supported");
     tensor sv = tn::svd_values_batched(ctx, mats);
     const int64_t k = sv.shape[1];

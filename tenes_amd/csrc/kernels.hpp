@@ -151,15 +151,41 @@ void refill_columns(double *X, int64_t n, int p, const double *sigma,
 // * max diagonal drops its column (rank-deficient input): that row of R is
 // zero and that column of Rinv is zero, so Q = Y * Rinv has a zero column
 // there (the other columns of Q do not depend on the dropped one).
+// pre != nullptr: G is already scaled and shifted (a block of a larger
+// factorisation); pre[0] is the pivot threshold (< 0: none), pre[1] the
+// pivot floor, both device values written by chol_prep.
 void chol_inv_upper(const double *G, int p, double shift_coef,
-                    double rel_floor, double *R, double *Rinv, hipStream_t s);
+                    double rel_floor, double *R, double *Rinv, hipStream_t s,
+                    const double *pre = nullptr);
+// Largest p of one chol_inv_upper (the LDS-resident factorisation).
+int chol_block_dim();
+// The blocked factorisation for chol_block_dim() < p <= 256 (two blocks,
+// split at p1): chol_prep scales and shifts G as chol_inv_upper does and
+// writes its blocks G11 (p1 x p1), G12 (p1 x p2), G22 (p2 x p2), the scales
+// dg = diag(G)^{1/2}, scl = 1/dg (0 for a zero column) and params (pivot
+// threshold, pivot floor); chol_assemble writes R and R^{-1} (R may be
+// nullptr) from the block factors F11, X11 = F11^{-1}, F12 = F11^{-T} G12,
+// U = X11 F12 X22, F22, X22 = F22^{-1} of the pre-scaled blocks.
+void chol_prep(const double *G, int p, int p1, double coef, double rel_floor,
+               double *G11, double *G12, double *G22, double *dg, double *scl,
+               double *params, hipStream_t s);
+// V = F^{-T} G for the 128 x 128 upper factor F written by chol_inv_upper
+// (rows of dropped columns zero: their rows of V are zero) and G 128 x p2
+// (p2 <= 128), by forward substitution.
+void trsm_upper_t(const double *F, int p2, const double *G, double *V, hipStream_t s);
+void chol_assemble(int p, int p1, const double *F11, const double *X11,
+                   const double *F12, const double *U, const double *F22,
+                   const double *X22, const double *dg, const double *scl,
+                   double *R, double *Rinv, hipStream_t s);
 // One-sided (Hestenes) Jacobi SVD of a batch of p x p matrices with
-// rows = m (m, p <= 128), held in LDS by one workgroup per matrix.
+// rows = m, one workgroup per matrix: held in LDS for m, p <= 128
+// (jacobi_lds_dim), in global memory up to 256 (jacobi_max_dim).
 // W is column-major per matrix (col j at W + j*m); on exit its columns are
 // mutually orthogonal.  sigma[j] = ||W[:,j]||.  rotations (c,s) of every
 // step are appended to rot (size max_sweeps*(p-1)*(p/2) pairs of doubles
 // per matrix) unless rot == nullptr; nsteps[b] = number of recorded steps.
 int jacobi_max_dim();
+int jacobi_lds_dim();
 void jacobi_svd(double *W, int m, int p, int batch, double tol,
                 int max_sweeps, double *sigma, double *rot, int *nsteps,
                 hipStream_t s);

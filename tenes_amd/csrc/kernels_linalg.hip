@@ -3,11 +3,13 @@
 //  * randn            : counter-based normal numbers for the range finder.
 //  * chol_inv_upper   : shifted Cholesky G + shift I = R^T R and R^{-1}, one
 //                       workgroup with the p x p matrix resident in LDS
-//                       (p <= 128).  Used by shifted CholeskyQR.
+//                       (p <= 128).  Used by shifted CholeskyQR; wider Gram
+//                       matrices (p <= 256) are factorised as two blocks
+//                       (chol_prep, chol_assemble).
 //  * jacobi_svd       : one-sided (Hestenes) Jacobi SVD, one workgroup per
-//                       matrix, the whole m x p matrix resident in LDS, the
-//                       p/2 column pairs of a round-robin step processed by
-//                       64 groups of 16 lanes.  The (c, s) of every rotation
+//                       matrix, the whole m x p matrix resident in LDS (in
+//                       global memory above 128), the p/2 column pairs of a
+//                       round-robin step processed by 64 groups of 16 lanes.  The (c, s) of every rotation
 //                       is streamed to HBM so the accumulated right factor can
 //                       be rebuilt row-parallel afterwards (jacobi_accumulate)
 //                       without keeping a second p x p matrix in LDS.
@@ -118,11 +120,11 @@ size_t chol_lds_bytes(int p) {
 
 __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
     const double *G, int p, double shift_coef, double rel_floor, double *R,
-    double *Rinv) {
+    double *Rinv, const double *pre) {
   extern __shared__ double lds[];
   __shared__ int defic[kMaxDim];
   __shared__ double s_dg[kMaxDim], s_scl[kMaxDim];
-  __shared__ double s_shift, s_thr;
+  __shared__ double s_shift, s_thr, s_floor;
   __shared__ int s_zero;
   const int pp = (p + 15) / 16 * 16;  // padded size (identity padding)
   const int nb = pp / 16;
@@ -162,21 +164,25 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
   // norms no longer enter the conditioning (a block of nearly orthogonal
   // columns of very different lengths factors as well as an orthonormal
   // one); zero columns keep a zero diagonal.
+  // A pre-scaled input (pre != nullptr: a diagonal block or Schur
+  // complement of a larger scaled and shifted Gram matrix) keeps unit scales.
   for (int i = tid; i < pp; i += kCholThreads) {
     const double g = A[i * ld + i];
-    const double dg = i < p ? (g > 0.0 ? sqrt(g) : 0.0) : 1.0;
+    const double dg = (i < p && !pre) ? (g > 0.0 ? sqrt(g) : 0.0) : 1.0;
     s_dg[i] = dg;
     s_scl[i] = dg > 0.0 ? 1.0 / dg : 0.0;
   }
   __syncthreads();
+  if (!pre) {
 #pragma unroll 4
-  for (int u = 0; u < 32; ++u)
+    for (int u = 0; u < 32; ++u)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = r0 + 4 * u, c = c0 + 64 * h;
-      if (r < pp && c < pp) A[r * ld + c] = (A[r * ld + c] * s_scl[r]) * s_scl[c];
-    }
-  __syncthreads();
+      for (int h = 0; h < 2; ++h) {
+        const int r = r0 + 4 * u, c = c0 + 64 * h;
+        if (r < pp && c < pp) A[r * ld + c] = (A[r * ld + c] * s_scl[r]) * s_scl[c];
+      }
+    __syncthreads();
+  }
   if (w == 0) {
     double tr = 0.0, mx = 0.0;
     for (int i = lane; i < p; i += 64) {
@@ -189,11 +195,22 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
       mx = fmax(mx, __shfl_xor(mx, o, 64));
     }
     if (lane == 0) {
-      s_zero = !(tr > 0.0);
-      s_shift = shift_coef * tr;
-      // pivots at or below rel_floor * max diagonal mark a column that is
-      // numerically in the span of the previous ones (rank-deficient block)
-      s_thr = rel_floor > 0.0 ? rel_floor * (mx + s_shift) : -1.0;
+      if (pre) {
+        // shift already added; threshold and pivot floor of the whole matrix
+        s_zero = 0;
+        s_shift = 0.0;
+        s_thr = pre[0];
+        s_floor = pre[1];
+      } else {
+        s_zero = !(tr > 0.0);
+        s_shift = shift_coef * tr;
+        // pivots at or below rel_floor * max diagonal mark a column that is
+        // numerically in the span of the previous ones (rank-deficient block)
+        s_thr = rel_floor > 0.0 ? rel_floor * (mx + s_shift) : -1.0;
+        // floor for pivots that rounding pushed to <= 0 when no deficiency
+        // test is requested (first, shifted pass)
+        s_floor = s_shift > 0.0 ? s_shift : DBL_MIN;
+      }
     }
   }
   __syncthreads();
@@ -210,9 +227,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
   const double shift = s_shift, thr = s_thr;
   for (int i = tid; i < p; i += kCholThreads) A[i * ld + i] += shift;
   __syncthreads();
-  // Floor for pivots that rounding pushed to <= 0 when no deficiency test is
-  // requested (first, shifted pass).
-  const double floor_d = shift > 0.0 ? shift : DBL_MIN;
+  const double floor_d = s_floor;
 
   for (int kb = 0; kb < nb; ++kb) {
     const int K0 = kb * 16;
@@ -384,6 +399,144 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
     }
 }
 
+// Cholesky of Gram matrices wider than one LDS-resident factorisation
+// (128 < p <= 256), as two 128-blocks: with Gs = D G D + shift I (the
+// scaled, shifted matrix of chol_inv_kernel) split at p1,
+//   F11 = chol(Gs11), F12 = F11^{-T} Gs12, F22 = chol(Gs22 - F12^T F12),
+//   F^{-1} = [X11, -X11 F12 X22; 0, X22]   (X = F^{-1} of each block),
+// both block factorisations with the whole matrix's pivot threshold, so
+// that a column of the second block that is dependent on the first one is
+// dropped as the unblocked factorisation would drop it.
+// chol_prep_kernel: scales (D, D^{-1}), shift, threshold and pivot floor,
+// and the three blocks of Gs (contiguous, row-major).  A zero (or NaN) Gram
+// matrix gives Gs = I with D = 0 and D^{-1} = 1: R = 0, R^{-1} = I.
+__global__ __launch_bounds__(1024) void chol_prep_kernel(
+    const double *G, int p, int p1, double coef, double rel_floor, double *G11,
+    double *G12, double *G22, double *dg, double *scl, double *params) {
+  __shared__ double s_scl[256];
+  __shared__ double s_tr[16], s_mx[16];
+  __shared__ double s_shift;
+  __shared__ int s_zero;
+  const int tid = threadIdx.x;
+  const int p2 = p - p1;
+  double tr = 0.0, mx = 0.0;
+  for (int i = tid; i < p; i += blockDim.x) {
+    const double g = G[(int64_t)i * p + i];
+    const double d = g > 0.0 ? sqrt(g) : 0.0;
+    const double sc = d > 0.0 ? 1.0 / d : 0.0;
+    s_scl[i] = sc;
+    const double gs = (g * sc) * sc;
+    tr += fabs(gs);
+    mx = fmax(mx, gs);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    tr += __shfl_xor(tr, o, 64);
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+  }
+  if ((tid & 63) == 0) {
+    s_tr[tid >> 6] = tr;
+    s_mx[tid >> 6] = mx;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0, x = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      t += s_tr[w];
+      x = fmax(x, s_mx[w]);
+    }
+    const int zero = !(t > 0.0);
+    const double shift = coef * t;
+    s_zero = zero;
+    s_shift = zero ? 0.0 : shift;
+    params[0] = rel_floor > 0.0 ? rel_floor * (x + shift) : -1.0;
+    params[1] = shift > 0.0 ? shift : DBL_MIN;
+  }
+  __syncthreads();
+  const int zero = s_zero;
+  const double shift = s_shift;
+  for (int i = tid; i < p; i += blockDim.x) {
+    const double sc = s_scl[i];
+    dg[i] = zero ? 0.0 : (sc > 0.0 ? 1.0 / sc : 0.0);
+    scl[i] = zero ? 1.0 : sc;
+  }
+  for (int e = tid; e < p * p; e += blockDim.x) {
+    const int r = e / p, c = e - (e / p) * p;
+    const double v = zero ? (r == c ? 1.0 : 0.0)
+                          : (G[e] * s_scl[r]) * s_scl[c] + (r == c ? shift : 0.0);
+    if (r < p1) {
+      if (c < p1)
+        G11[r * p1 + c] = v;
+      else
+        G12[r * p2 + c - p1] = v;
+    } else if (c >= p1) {
+      G22[(r - p1) * p2 + c - p1] = v;
+    }
+  }
+}
+
+// F12 = F11^{-T} G12 by forward substitution (backward stable, unlike the
+// product with the explicit inverse X11^T, whose error grows with cond(F11)
+// and is large exactly where the block's columns are nearly dependent).  F11
+// is the upper factor as chol_inv_upper writes it (a dropped column has a
+// zero row, diagonal included; its row of F12 is zero).  One workgroup,
+// F11 in LDS, one thread per column of G12 holding that column in
+// registers, right-looking: v_t /= F[t][t], then v_r -= F[t][r] v_t for all
+// r > t (independent updates, the F entries LDS broadcasts).
+constexpr int kTrsmRows = 128;
+
+__global__ __launch_bounds__(128) void trsm_ut_kernel(const double *F, int p2,
+                                                      const double *G, double *V) {
+  constexpr int n = kTrsmRows;
+  extern __shared__ double Fs[];  // n x (n + 1)
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int r = e / n, c = e - (e / n) * n;
+    Fs[r * (n + 1) + c] = F[e];
+  }
+  __syncthreads();
+  if (tid >= p2) return;
+  double v[n];
+#pragma unroll
+  for (int r = 0; r < n; ++r) v[r] = G[(int64_t)r * p2 + tid];
+#pragma unroll
+  for (int t = 0; t < n; ++t) {
+    const double d = Fs[t * (n + 1) + t];
+    v[t] = d != 0.0 ? v[t] / d : 0.0;
+#pragma unroll
+    for (int r = t + 1; r < n; ++r) v[r] -= Fs[t * (n + 1) + r] * v[t];
+  }
+#pragma unroll
+  for (int r = 0; r < n; ++r) V[(int64_t)r * p2 + tid] = v[r];
+}
+
+// R and R^{-1} in the unscaled basis: R[r][c] = F[r][c] dg[c],
+// R^{-1}[r][c] = scl[r] X[r][c], from the blocks (U = X11 F12 X22).
+__global__ void chol_assemble_kernel(int p, int p1, const double *F11,
+                                     const double *X11, const double *F12,
+                                     const double *U, const double *F22,
+                                     const double *X22, const double *dg,
+                                     const double *scl, double *R, double *Rinv) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= p * p) return;
+  const int r = e / p, c = e - (e / p) * p;
+  const int p2 = p - p1;
+  double f = 0.0, x = 0.0;
+  if (r < p1) {
+    if (c < p1) {
+      f = F11[r * p1 + c];
+      x = X11[r * p1 + c];
+    } else {
+      f = F12[r * p2 + c - p1];
+      x = -U[r * p2 + c - p1];
+    }
+  } else if (c >= p1) {
+    f = F22[(r - p1) * p2 + c - p1];
+    x = X22[(r - p1) * p2 + c - p1];
+  }
+  if (R) R[e] = f * dg[c];
+  Rinv[e] = scl[r] * x;
+}
+
 // ---------------------------------------------------------------------------
 // One-sided Jacobi SVD.
 // ---------------------------------------------------------------------------
@@ -553,6 +706,112 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
   if (tid == 0 && nsteps) nsteps[b] = steps;
 }
 
+// One-sided Jacobi for matrices of up to kMaxDimBig columns and rows that
+// do not fit in LDS: the same round-robin rounds, rotation formula,
+// convergence test and rotation stream as jacobi_kernel, with the matrix
+// in global memory (it stays in L2: 512 KB at 256 x 256).  One workgroup per
+// matrix, 64 groups of 16 lanes, each group one column pair per step (the
+// two columns in registers, 16 rows per lane); the workgroup barrier
+// between rounds orders the rounds' global reads after the previous round's
+// writes (all waves of a workgroup share one CU's cache).
+constexpr int kMaxDimBig = 256;
+
+__global__ __launch_bounds__(1024) void jacobi_big_kernel(
+    double *Wg, int m, int p, double tol, int max_sweeps, double *sigma,
+    double *rot, int *nsteps) {
+  __shared__ double s_off[16];
+  __shared__ double s_tiny;
+  const int b = blockIdx.x;
+  double *W = Wg + (int64_t)b * m * p;
+  const int pe = p + (p & 1);
+  const int tid = threadIdx.x;
+  double fro = 0.0;
+  for (int i = tid; i < m * p; i += blockDim.x) fro += W[i] * W[i];
+  for (int o = 32; o > 0; o >>= 1) fro += __shfl_xor(fro, o, 64);
+  if ((tid & 63) == 0) s_off[tid >> 6] = fro;
+  __syncthreads();
+  if (tid == 0) {
+    double f = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) f += s_off[w];
+    s_tiny = f * 1e-60;  // as in jacobi_kernel
+  }
+  __syncthreads();
+  const double tiny = s_tiny;
+  const double tol2 = tol * tol;
+  const int grp = tid >> 4;
+  const int gl = tid & 15;
+  const int npairs = pe / 2;
+  const int64_t rot_stride = (int64_t)max_sweeps * (pe - 1) * npairs * 2;
+  double *rotb = rot ? rot + (int64_t)b * rot_stride : nullptr;
+  int steps = 0;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off_local = 0.0;
+    for (int s = 0; s < pe - 1; ++s) {
+      for (int k = grp; k < npairs; k += 64) {
+        int i, j;
+        rr_pair(s, k, pe, i, j);
+        // the padding column (p odd) is zero: its pairs never rotate
+        const bool real_j = j < p;
+        double *wi = W + (int64_t)i * m;
+        double *wj = W + (int64_t)j * m;
+        double xr[16], yr[16];
+        double a = 0.0, bb = 0.0, g = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int r = gl + 16 * u;
+          xr[u] = r < m ? wi[r] : 0.0;
+          yr[u] = (real_j && r < m) ? wj[r] : 0.0;
+          a += xr[u] * xr[u];
+          bb += yr[u] * yr[u];
+          g += xr[u] * yr[u];
+        }
+        a = sum16(a);
+        bb = sum16(bb);
+        g = sum16(g);
+        double c = 1.0, sn = 0.0;
+        const double g2 = g * g;
+        if (g != 0.0 && a > tiny && bb > tiny && g2 > tol2 * (a * bb)) {
+          off_local = 1.0;
+          const double dd = bb - a;
+          const double t = (dd < 0.0 ? -2.0 * g : 2.0 * g) / (fabs(dd) + sqrt(dd * dd + 4.0 * g2));
+          c = 1.0 / sqrt(1.0 + t * t);
+          sn = c * t;
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int r = gl + 16 * u;
+            if (r < m) {
+              wi[r] = c * xr[u] - sn * yr[u];
+              wj[r] = sn * xr[u] + c * yr[u];
+            }
+          }
+        }
+        if (rotb && gl == 0) {
+          double *dst = rotb + ((int64_t)steps * npairs + k) * 2;
+          dst[0] = c;
+          dst[1] = sn;
+        }
+      }
+      ++steps;
+      __syncthreads();
+    }
+    double v = off_local;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((tid & 63) == 0) s_off[tid >> 6] = v;
+    __syncthreads();
+    double off = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) off = fmax(off, s_off[w]);
+    __syncthreads();
+    if (!(off > 0.0)) break;
+  }
+  for (int jc = tid; jc < p; jc += blockDim.x) {
+    double acc = 0.0;
+    const double *wc = W + (int64_t)jc * m;
+    for (int r = 0; r < m; ++r) acc += wc[r] * wc[r];
+    sigma[(int64_t)b * p + jc] = sqrt(acc);
+  }
+  if (tid == 0 && nsteps) nsteps[b] = steps;
+}
+
 // J = I * (product of recorded rotations).  Row-parallel: one 64-lane
 // workgroup owns ROWS rows of J in LDS and replays the rotation stream.  The
 // stream is read in chunks of kAccChunk rounds: the next chunk is loaded
@@ -560,12 +819,12 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
 // replayed from LDS, so the replay does not wait on memory every round.
 constexpr int kAccChunk = 16;
 
-template <int ROWS>
+template <int ROWS, int MAXD>
 __global__ __launch_bounds__(64) void jacobi_accum_kernel(const double *rot,
                                                           const int *nsteps,
                                                           int p, double *J) {
-  __shared__ double row[ROWS][kMaxDim + 2];
-  __shared__ double cs[kAccChunk * (kMaxDim / 2) * 2];
+  __shared__ double row[ROWS][MAXD + 2];
+  __shared__ double cs[kAccChunk * (MAXD / 2) * 2];
   const int pe = p + (p & 1);
   const int npairs = pe / 2;
   const int r0 = blockIdx.x * ROWS;
@@ -577,7 +836,7 @@ __global__ __launch_bounds__(64) void jacobi_accum_kernel(const double *rot,
   const int per = kAccChunk * npairs * 2;  // doubles per chunk (<= 2048)
   const int64_t total = (int64_t)ns * npairs * 2;
   const int nchunks = (ns + kAccChunk - 1) / kAccChunk;
-  constexpr int kPerLane = kAccChunk * (kMaxDim / 2) * 2 / 64;
+  constexpr int kPerLane = kAccChunk * (MAXD / 2) * 2 / 64;
   double buf[kPerLane];
   auto fetch = [&](int ch) {
     const int64_t base = (int64_t)ch * per;
@@ -641,7 +900,8 @@ __global__ void sort_desc_kernel(const double *sigma, int p, int batch,
 
 }  // namespace
 
-int jacobi_max_dim() { return kMaxDim; }
+int jacobi_max_dim() { return kMaxDimBig; }
+int jacobi_lds_dim() { return kMaxDim; }
 
 void linalg_init_device() {
   const int lds = kMaxDim * kMaxDim * (int)sizeof(double);
@@ -652,6 +912,10 @@ void linalg_init_device() {
   TN_HIP_CHECK(hipFuncSetAttribute(
       reinterpret_cast<const void *>(&jacobi_kernel),
       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  TN_HIP_CHECK(hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&trsm_ut_kernel),
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)((size_t)kTrsmRows * (kTrsmRows + 1) * sizeof(double))));
 }
 
 void randn(double *p, int64_t n, uint64_t seed, uint64_t counter,
@@ -674,20 +938,56 @@ void refill_columns(double *X, int64_t n, int p, const double *sigma,
 }
 
 void chol_inv_upper(const double *G, int p, double shift_coef,
-                    double rel_floor, double *R, double *Rinv, hipStream_t s) {
+                    double rel_floor, double *R, double *Rinv, hipStream_t s,
+                    const double *pre) {
   TN_REQUIRE(p >= 1 && p <= kMaxDim, "chol_inv_upper: p out of range");
   const size_t lds = chol_lds_bytes(p);
   hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(kCholThreads), lds, s, G, p,
-                     shift_coef, rel_floor, R, Rinv);
+                     shift_coef, rel_floor, R, Rinv, pre);
+  TN_HIP_CHECK(hipGetLastError());
+}
+
+int chol_block_dim() { return kMaxDim; }
+
+void trsm_upper_t(const double *F, int p2, const double *G, double *V, hipStream_t s) {
+  TN_REQUIRE(p2 >= 1 && p2 <= kTrsmRows, "trsm_upper_t: p2 out of range");
+  const size_t lds = (size_t)kTrsmRows * (kTrsmRows + 1) * sizeof(double);
+  hipLaunchKernelGGL(trsm_ut_kernel, dim3(1), dim3(128), lds, s, F, p2, G, V);
+  TN_HIP_CHECK(hipGetLastError());
+}
+
+void chol_prep(const double *G, int p, int p1, double coef, double rel_floor,
+               double *G11, double *G12, double *G22, double *dg, double *scl,
+               double *params, hipStream_t s) {
+  TN_REQUIRE(p > p1 && p1 >= 1 && p <= 256, "chol_prep: sizes out of range");
+  hipLaunchKernelGGL(chol_prep_kernel, dim3(1), dim3(1024), 0, s, G, p, p1, coef,
+                     rel_floor, G11, G12, G22, dg, scl, params);
+  TN_HIP_CHECK(hipGetLastError());
+}
+
+void chol_assemble(int p, int p1, const double *F11, const double *X11,
+                   const double *F12, const double *U, const double *F22,
+                   const double *X22, const double *dg, const double *scl,
+                   double *R, double *Rinv, hipStream_t s) {
+  TN_REQUIRE(p > p1 && p1 >= 1 && p <= 256, "chol_assemble: sizes out of range");
+  hipLaunchKernelGGL(chol_assemble_kernel, dim3(ceil_div(p * p, 256)), dim3(256), 0, s,
+                     p, p1, F11, X11, F12, U, F22, X22, dg, scl, R, Rinv);
   TN_HIP_CHECK(hipGetLastError());
 }
 
 void jacobi_svd(double *W, int m, int p, int batch, double tol,
                 int max_sweeps, double *sigma, double *rot, int *nsteps,
                 hipStream_t s) {
-  TN_REQUIRE(m >= 1 && p >= 1 && m <= kMaxDim && p <= kMaxDim,
+  TN_REQUIRE(m >= 1 && p >= 1 && m <= kMaxDimBig && p <= kMaxDimBig,
              "jacobi_svd: dimension out of range");
   if (batch <= 0) return;
+  if (m > kMaxDim || p > kMaxDim) {
+    // above the LDS-resident size: the matrix stays in global memory
+    hipLaunchKernelGGL(jacobi_big_kernel, dim3(batch), dim3(1024), 0, s, W, m, p, tol,
+                       max_sweeps, sigma, rot, nsteps);
+    TN_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const int pe = p + (p & 1);
   const size_t lds = (size_t)m * pe * sizeof(double);
   hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(1024), lds, s, W, m, p,
@@ -697,10 +997,14 @@ void jacobi_svd(double *W, int m, int p, int batch, double tol,
 
 void jacobi_accumulate(const double *rot, const int *nsteps, int p, double *J,
                        hipStream_t s) {
-  TN_REQUIRE(p >= 1 && p <= kMaxDim, "jacobi_accumulate: p out of range");
+  TN_REQUIRE(p >= 1 && p <= kMaxDimBig, "jacobi_accumulate: p out of range");
   constexpr int ROWS = 4;
-  hipLaunchKernelGGL((jacobi_accum_kernel<ROWS>), dim3(ceil_div(p, ROWS)),
-                     dim3(64), 0, s, rot, nsteps, p, J);
+  if (p <= kMaxDim)
+    hipLaunchKernelGGL((jacobi_accum_kernel<ROWS, kMaxDim>), dim3(ceil_div(p, ROWS)),
+                       dim3(64), 0, s, rot, nsteps, p, J);
+  else
+    hipLaunchKernelGGL((jacobi_accum_kernel<ROWS, kMaxDimBig>), dim3(ceil_div(p, ROWS)),
+                       dim3(64), 0, s, rot, nsteps, p, J);
   TN_HIP_CHECK(hipGetLastError());
 }
 

@@ -13,7 +13,8 @@ namespace tn {
 
 namespace {
 
-constexpr int kJac = 128;  // LDS-resident Jacobi limit
+constexpr int kJac = 128;  // LDS-resident Jacobi and Cholesky limit
+constexpr int kJacBig = 256;  // global-memory Jacobi, blocked Cholesky limit
 constexpr int kMinOversample = 16;  // range-finder columns beyond k
 
 // Raw device buffer of ints carried by a Tensor (for lifetime management).
@@ -72,6 +73,39 @@ void run_jacobi(Context &ctx, Tensor &W, int rows, int cols, bool want_J,
   out.W = W;
 }
 
+// Cholesky of the p x p Gram matrix G for 128 < p <= 256 as two blocks
+// (kern::chol_prep): the same scaling, shift and pivot threshold as the
+// one-workgroup factorisation, with the second block factorised from its
+// Schur complement.  Writes R (if not null) and Ri = R^{-1}.
+void chol_inv_blocked(Context &ctx, const Tensor &G, int p, double coef,
+                      double rel_floor, Tensor *R, Tensor &Ri) {
+  hipStream_t s = ctx.stream();
+  const int p1 = kern::chol_block_dim(), p2 = p - p1;
+  Tensor G11 = empty(ctx, Shape{p1, p1}), G12 = empty(ctx, Shape{p1, p2});
+  Tensor G22 = empty(ctx, Shape{p2, p2});
+  Tensor dg = empty(ctx, Shape{p}), scl = empty(ctx, Shape{p});
+  Tensor prm = empty(ctx, Shape{2});
+  kern::chol_prep(G.data(), p, p1, coef, rel_floor, G11.data(), G12.data(),
+                  G22.data(), dg.data(), scl.data(), prm.data(), s);
+  Tensor F11 = empty(ctx, Shape{p1, p1}), X11 = empty(ctx, Shape{p1, p1});
+  kern::chol_inv_upper(G11.data(), p1, 0.0, 0.0, F11.data(), X11.data(), s,
+                       prm.data());
+  // F12 = F11^{-T} G12 by forward substitution (rows of dropped columns
+  // zero, as in the one-block factorisation)
+  Tensor F12 = empty(ctx, Shape{p1, p2});
+  kern::trsm_upper_t(F11.data(), p2, G12.data(), F12.data(), s);
+  Tensor S = tensordot(F12, F12, Axes{0}, Axes{0});    // p2 x p2
+  kern::axpy(G22.data(), S.data(), (int64_t)p2 * p2, -1.0, s);  // Schur complement
+  Tensor F22 = empty(ctx, Shape{p2, p2}), X22 = empty(ctx, Shape{p2, p2});
+  kern::chol_inv_upper(G22.data(), p2, 0.0, 0.0, F22.data(), X22.data(), s,
+                       prm.data());
+  Tensor U = tensordot(tensordot(X11, F12, Axes{1}, Axes{0}), X22, Axes{1}, Axes{0});
+  Ri = empty(ctx, Shape{p, p});
+  kern::chol_assemble(p, p1, F11.data(), X11.data(), F12.data(), U.data(),
+                      F22.data(), X22.data(), dg.data(), scl.data(),
+                      R ? R->data() : nullptr, Ri.data(), s);
+}
+
 }  // namespace
 
 Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
@@ -79,7 +113,7 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
   TN_REQUIRE(Y_in.rank() == 2, "orthonormalize: 2-D input expected");
   TN_REQUIRE(passes >= 1 && passes <= 3, "orthonormalize: 1 to 3 passes");
   const int64_t m = Y_in.shape[0], p = Y_in.shape[1];
-  TN_REQUIRE(p >= 1 && p <= kJac, "orthonormalize: block width > 128");
+  TN_REQUIRE(p >= 1 && p <= kJacBig, "orthonormalize: block width > 256");
   Tensor Y = Y_in;
   Tensor Rtot;
   const double u = 1.1102230246251565e-16;
@@ -96,8 +130,11 @@ Tensor orthonormalize(Context &ctx, const Tensor &Y_in, Tensor *R_out,
     const double coef =
         shifted ? 11.0 * (double)(m * p + p * (p + 1)) * u : 0.0;
     const double rel_floor = shifted ? 0.0 : 1e-13;
-    kern::chol_inv_upper(G.data(), (int)p, coef, rel_floor,
-                         R_out ? R.data() : nullptr, Ri.data(), ctx.stream());
+    if (p <= kJac)
+      kern::chol_inv_upper(G.data(), (int)p, coef, rel_floor,
+                           R_out ? R.data() : nullptr, Ri.data(), ctx.stream());
+    else
+      chol_inv_blocked(ctx, G, (int)p, coef, rel_floor, R_out ? &R : nullptr, Ri);
     Y = tensordot(Y, Ri, Axes{1}, Axes{0});
     if (R_out) Rtot = Rtot.defined() ? tensordot(R, Rtot, Axes{1}, Axes{0}) : R;
   }
@@ -155,8 +192,8 @@ SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op
   const int m = Ap ? (int)A.shape[0] : (int)op->m, n = Ap ? (int)A.shape[1] : (int)op->n;
   TN_REQUIRE(m >= 1 && n >= 1, "svd_top: empty matrix");
   TN_REQUIRE(k_target >= 1, "svd_top: k must be positive");
-  TN_REQUIRE(k_target <= kJac,
-             "svd_top: at most 128 singular triplets are supported");
+  TN_REQUIRE(k_target <= kJacBig,
+             "svd_top: at most 256 singular triplets are supported");
   hipStream_t s = ctx.stream();
 
   // A ~= Lq * X * Rq^T with orthonormal Lq (m x r), Rq (n x c); X is r x c.
@@ -176,13 +213,16 @@ SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op
     have_R = true;
     X = transpose(Rf, Axes{1, 0});
   } else {
-    const int p = kJac;
-    // the subspace error after q iterations is ~(s_{p+1}/s_k)^(2q+1): with
-    // fewer than 16 columns of oversampling no iteration count bounds it
+    // range-finder block: 128 columns (LDS-resident Cholesky and Jacobi)
+    // while that leaves 16 columns of oversampling, else 256 (blocked
+    // Cholesky, global-memory Jacobi).  The subspace error after q
+    // iterations is ~(s_{p+1}/s_k)^(2q+1): with fewer than 16 columns of
+    // oversampling no iteration count bounds it.
+    const int p = k_target <= kJac - kMinOversample ? kJac : kJacBig;
     if (k_target > p - kMinOversample)
       throw Error("svd_top: " + std::to_string(k_target) +
                       " leading triplets of a matrix wider than 128 need a "
-                      "range-finder block wider than 128 (at most " +
+                      "range-finder block wider than 256 (at most " +
                       std::to_string(p - kMinOversample) + " supported)",
                   -5);
     const bool warm = opt.start && opt.start->defined() && opt.start->rank() == 2 &&
@@ -210,22 +250,30 @@ SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op
     // the small kept singular directions die out over the iterations.  The
     // final range basis gets the full three passes.
     const int q = std::max(0, warm ? opt.start_power_iters : opt.power_iters);
-    // A warm block holds Ritz vectors: its images under A and A^T stay
-    // nearly orthogonal columns of very different lengths, which one plain
-    // pass on the scaled Gram matrix orthonormalises (two for the final
-    // bases); a Gaussian block mixes all directions and needs the shifted
-    // passes.
-    const int mid = warm ? 1 : 2, fin = warm ? 2 : 3;
-    Tensor Q = orthonormalize(ctx, prod(Om), nullptr, q == 0 ? fin : mid, !warm);
-    const bool skip_z =
-        warm && opt.start_range > 0.0 && opt.start_range < 1e7;
+    // A warm block holds Ritz vectors: while they stay aligned with the
+    // singular directions (a settled environment), its images under A and
+    // A^T are nearly orthogonal columns of very different lengths, which one
+    // plain pass on the scaled Gram matrix orthonormalises (two for the
+    // final bases); a Gaussian block mixes all directions and needs the
+    // shifted passes.  The 256-wide block keeps the shifted passes warm too:
+    // its Ritz values reach deeper, and with an environment still changing
+    // (random D=2 tensors, chi = 160, 1e-2 per sweep) the misaligned images
+    // lost directions to the plain pass's pivot floor (corner spectra off
+    // by 1e-7 to 1e-3 against 2.5e-9 with shifted passes).
+    const bool plain = warm && p <= kJac;
+    const int mid = plain ? 1 : 2, fin = plain ? 2 : 3;
+    Tensor Q = orthonormalize(ctx, prod(Om), nullptr, q == 0 ? fin : mid, !plain);
+    // Skipping the orthonormalisation between the A^T and the A product
+    // squares the conditioning of the next block: only for aligned warm
+    // blocks.
+    const bool skip_z = plain && opt.start_range > 0.0 && opt.start_range < 1e7;
     for (int it = 0; it < q; ++it) {
-      Tensor Z = skip_z ? prodT(Q) : orthonormalize(ctx, prodT(Q), nullptr, mid, !warm);
-      Q = orthonormalize(ctx, prod(Z), nullptr, it + 1 == q ? fin : mid, !warm);
+      Tensor Z = skip_z ? prodT(Q) : orthonormalize(ctx, prodT(Q), nullptr, mid, !plain);
+      Q = orthonormalize(ctx, prod(Z), nullptr, it + 1 == q ? fin : mid, !plain);
     }
     Tensor R2;
     Tensor Bt = prodT(Q);  // n x p = A^T Q
-    Rq = orthonormalize(ctx, Bt, &R2, fin, !warm);  // Bt = Rq R2
+    Rq = orthonormalize(ctx, Bt, &R2, fin, !plain);  // Bt = Rq R2
     Lq = Q;
     have_L = have_R = randomized = true;
     X = transpose(R2, Axes{1, 0});  // B = Q^T A = R2^T Rq^T
@@ -258,7 +306,7 @@ SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op
     r.V = have_R ? tensordot(Rq, Jp, Axes{1}, Axes{0}) : Jp;
   }
   r.s_dev = slice(jo.sigma_sorted, 0, 0, k);
-  // pinned slot: all <= 128 singular values, then the Jacobi round count,
+  // pinned slot: all <= 256 singular values, then the Jacobi round count,
   // written by one kernel through the slot's device address
   kern::write_slot(ctx.pinned_device(slot), jo.sigma_sorted.data(), cols,
                    jo.nsteps->ptr, s);
@@ -303,8 +351,8 @@ Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats) {
   TN_REQUIRE(!mats.empty(), "svd_values_batched: empty batch");
   ContractionPrecision fp64(ctx, false);
   const int m = (int)mats[0].shape[0], n = (int)mats[0].shape[1];
-  TN_REQUIRE(m <= kJac && n <= kJac,
-             "svd_values_batched: matrices larger than 128 x 128");
+  TN_REQUIRE(m <= kJacBig && n <= kJacBig,
+             "svd_values_batched: matrices larger than 256 x 256");
   const int b = (int)mats.size();
   hipStream_t s = ctx.stream();
   // Jacobi acts on the columns of the stored column-major matrix.  Use the

@@ -12,7 +12,8 @@ struct SvdOptions {
   int power_iters = 4;        // subspace iterations for the range finder
   int max_sweeps = 30;        // Jacobi sweep cap
   double jacobi_tol = 0.0;    // 0: choose from the dimension
-  // Optional start block of the range finder (n x 128, orthonormal columns,
+  // Optional start block of the range finder (n x p, p = 128 or 256 as the
+  // range finder chooses for k, orthonormal columns,
   // e.g. a previous SvdResult::Vbasis); used when its shape fits, replacing
   // the Gaussian block, with start_power_iters iterations.
   const Tensor *start = nullptr;
@@ -21,10 +22,11 @@ struct SvdOptions {
   const Tensor *start_sigma = nullptr;
   int start_power_iters = 2;
   // Largest over smallest of those singular values (0: unknown).  Below
-  // 1e7 the warm iterations skip the orthonormalisation between the A^T and
-  // the A product: (A A^T) then amplifies the rounding left in a column
-  // towards the dominant directions by at most range^2 * eps ~ 1e-2, which
-  // the next orthonormalisation removes.
+  // 1e7 the warm iterations of a 128-wide block (plain CholeskyQR passes;
+  // a 256-wide block keeps the shifted ones) skip the orthonormalisation
+  // between the A^T and the A product: (A A^T) then amplifies the rounding
+  // left in a column towards the dominant directions by at most
+  // range^2 * eps ~ 1e-2, which the next orthonormalisation removes.
   double start_range = 0.0;
 };
 
@@ -33,11 +35,11 @@ struct SvdResult {
   Tensor V;                 // (n, k): right singular vectors as columns
   Tensor s_dev;             // (k): singular values, descending
   std::vector<double> s;    // host copy of s_dev
-  // host copy of all singular values of the projected problem (the 128 Ritz
-  // values on the randomized path), descending
+  // host copy of all singular values of the projected problem (the p = 128
+  // or 256 Ritz values on the randomized path), descending
   std::vector<double> s_all;
-  // Randomized path only: all 128 right Ritz vectors (n x 128), descending,
-  // and their 128 singular values.
+  // Randomized path only: all p right Ritz vectors (n x p), descending,
+  // and their p singular values.
   Tensor Vbasis;
   Tensor sbasis;
 };
@@ -45,8 +47,10 @@ struct SvdResult {
 // Leading k singular triplets of the m x n matrix A (row-major 2-D tensor).
 // Exact one-sided Jacobi SVD when min(m, n) <= 128 (after a CholeskyQR
 // reduction when max(m, n) > 128); otherwise a randomized subspace iteration
-// with a 128-wide block followed by the same Jacobi SVD of the projected
-// problem.  The returned k is min(k_target, rank-space size).
+// with a p-wide block followed by the same Jacobi SVD of the projected
+// problem, p = 128 for k <= 112 and 256 (blocked Cholesky, global-memory
+// Jacobi) for 112 < k <= 240.  The returned k is min(k_target, rank-space
+// size).
 SvdResult svd_top(Context &ctx, const Tensor &A, int k_target,
                   const SvdOptions &opt);
 
@@ -83,7 +87,7 @@ SvdResult svd_top_end(SvdPending &pending);
 SvdResult svd_top_detach(SvdPending &pending);
 
 // Singular values (descending, all min(m,n) of them) of a batch of equal
-// shaped matrices, all of them <= 128 x 128.  Result: device tensor
+// shaped matrices, all of them <= 256 x 256.  Result: device tensor
 // (batch, min(m,n)).
 Tensor svd_values_batched(Context &ctx, const std::vector<Tensor> &mats);
 
