@@ -124,6 +124,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
   extern __shared__ double lds[];
   __shared__ int defic[kMaxDim];
   __shared__ double s_dg[kMaxDim], s_scl[kMaxDim];
+  __shared__ double s_rd[kMaxDim];  // 1 / F[i][i] (0 for a dropped column)
   __shared__ double s_shift, s_thr, s_floor;
   __shared__ int s_zero;
   const int pp = (p + 15) / 16 * 16;  // padded size (identity padding)
@@ -269,13 +270,18 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
           const double sd = sqrt(pv[t]);
           A[(K0 + r) * ld + K0 + lc] =
               lc == r ? (dfc[t] ? 1.0 : sd) : (dfc[t] ? 0.0 : d[t] / sd);
+          if (lc == r) s_rd[K0 + r] = dfc[t] ? 0.0 : 1.0 / sd;
         }
         if (lc == r) defic[K0 + r] = dfc[t] ? 1 : 0;
       }
     }
     __syncthreads();
     // ---- (2) block row of F by forward substitution, one thread per column:
-    //   F_kk^T P = A[K0:K0+16, col], rows of dropped columns set to zero
+    //   F_kk^T P = A[K0:K0+16, col], rows of dropped columns set to zero.
+    // Column-oriented: once v_t is final (times 1/F[t][t], 0 for a dropped
+    // column) it is eliminated from all later rows at once, so the chain
+    // from one row to the next is one multiply and one FMA (the row-oriented
+    // sums made every row wait for a chain of t FMAs and a division).
     const int ncol = pp - K0 - 16;
     if (tid < ncol) {
       const int col = K0 + 16 + tid;
@@ -283,11 +289,10 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) v[r] = A[(K0 + r) * ld + col];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        double acc = v[r];
+      for (int t = 0; t < 16; ++t) {
+        v[t] *= s_rd[K0 + t];
 #pragma unroll
-        for (int t = 0; t < r; ++t) acc -= A[(K0 + t) * ld + K0 + r] * v[t];
-        v[r] = defic[K0 + r] ? 0.0 : acc / A[(K0 + r) * ld + K0 + r];
+        for (int r = t + 1; r < 16; ++r) v[r] -= A[(K0 + t) * ld + K0 + r] * v[t];
       }
 #pragma unroll
       for (int r = 0; r < 16; ++r) A[(K0 + r) * ld + col] = v[r];
@@ -320,18 +325,23 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
 
   // ---- diagonal blocks of X = F^{-1}: 16-lane group g of wave w inverts
   // block 4w + g, lane lc holding column lc, rows from the bottom:
-  //   X[i][c] = (delta_ic - sum_{k>i} F[i][k] X[k][c]) / F[i][i]
+  //   X[i][c] = (delta_ic - sum_{k>i} F[i][k] X[k][c]) / F[i][i],
+  // column-oriented as the block row above (x_i final, then eliminated from
+  // the rows above it).  A dropped column i gets a zero row i here; its
+  // column of X is zeroed on the store and no other column of X depends on
+  // that row (row i of F is e_i^T, so the exact X[i][c] is 0 for c != i).
   {
     const int b = w * 4 + lr;
     if (b < nb) {
       const double *Fb = A + (size_t)b * 16 * ld + b * 16;
       double x[16];
 #pragma unroll
-      for (int i = 15; i >= 0; --i) {
-        double acc = 0.0;
+      for (int i = 0; i < 16; ++i) x[i] = lc == i ? 1.0 : 0.0;
 #pragma unroll
-        for (int k = i + 1; k < 16; ++k) acc += Fb[i * ld + k] * x[k];
-        x[i] = ((lc == i ? 1.0 : 0.0) - acc) / Fb[i * ld + i];
+      for (int i = 15; i >= 0; --i) {
+        x[i] *= s_rd[b * 16 + i];
+#pragma unroll
+        for (int r = 0; r < i; ++r) x[r] -= Fb[r * ld + i] * x[i];
       }
 #pragma unroll
       for (int r = 0; r < 16; ++r) Xd[(b * 16 + r) * kCholXdLd + lc] = x[r];
