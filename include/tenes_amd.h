@@ -380,6 +380,23 @@ int tenes_env_simple_update(tenes_env *env, int nops, const int *source_sites,
 int tenes_env_get_lambda(tenes_env *env, int site, int leg, double *out,
                          int *n_out);
 
+/* Checkpoint of the environment: iTPS::save_tensors / load_tensors
+ * (src/iTPS/saveload_tensors.cpp:38-230).  save writes dir/params.dat
+ * (format version 1: N_UNIT, CHI -- chi, or C1[0]'s dimension when chi <= 0
+ * -- and every site tensor's shape), one file per tensor (T, Et, Er, Eb,
+ * El, C1..C4 _<site>.dat; float64 with a rank/shape header, as mptensor's
+ * own format is not available) and lambda_<site>.dat (the bond weights, one
+ * per line); the directory is created if needed and every tensor must be
+ * set.  load reads such a directory, resizing each tensor to the shape the
+ * environment holds for it (zero padding, then cutting: resize_tensor,
+ * src/tensor.cpp:114-148; chi > 0 sets the environment legs of C and eT
+ * the environment does not hold yet), and fails without changing the
+ * environment when the directory or params.dat is missing, the format
+ * version is not 1, N_UNIT or a physical dimension differs, or a tensor has
+ * another rank. */
+int tenes_env_save_tensors(tenes_env *env, const char *dir, int chi);
+int tenes_env_load_tensors(tenes_env *env, const char *dir, int chi);
+
 /* Full-update fields of PEPS_Parameters (PEPS_Parameters.hpp:29-94,
  * defaults PEPS_Parameters.cpp:55-64). */
 typedef struct tenes_fu_parameters {

@@ -144,6 +144,8 @@ _SIGS = {
     "tenes_env_simple_update": (
         C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, C.POINTER(C.c_void_p), C.c_int, C.c_double]),
     "tenes_env_get_lambda": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, c_int_p]),
+    "tenes_env_save_tensors": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "tenes_env_load_tensors": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "tenes_tm_parameters_default": (None, [C.POINTER(TmParameters)]),
     "tenes_env_transfer_matrix_eigenvalues": (
         C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(TmParameters), c_dbl_p, c_int_p]),

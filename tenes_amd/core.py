@@ -6,6 +6,7 @@ argument marshalling.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Sequence
 
 import numpy as np
@@ -521,6 +522,17 @@ class Environment:
 
     def site_tensors(self) -> list:
         return [self.get("Tn", i).numpy() for i in range(self.N)]
+
+    def save_tensors(self, directory: str, chi: int = 0) -> None:
+        """Checkpoint (iTPS::save_tensors, saveload_tensors.cpp:38-86):
+        params.dat, one file per tensor and the bond weights in directory."""
+        self.ctx.check(self.ctx.lib.tenes_env_save_tensors(self.h, os.fsencode(directory), chi))
+
+    def load_tensors(self, directory: str, chi: int = 0) -> None:
+        """Restart from a checkpoint (iTPS::load_tensors, saveload_tensors.cpp:
+        88-230), each tensor resized to the shape this environment holds for
+        it; chi > 0 sets the environment legs of tensors it does not hold."""
+        self.ctx.check(self.ctx.lib.tenes_env_load_tensors(self.h, os.fsencode(directory), chi))
 
     def transfer_matrix_eigenvalues(self, direction: int, fixed: int, params=None) -> np.ndarray:
         params = params or tm_parameters()

@@ -12,6 +12,7 @@
 
 #include "../../include/tenes_amd.h"
 #include "adjoint.hpp"
+#include "checkpoint.hpp"
 #include "contract.hpp"
 #include "ctm.hpp"
 #include "full_update.hpp"
@@ -1311,6 +1312,23 @@ int tenes_env_get_lambda(tenes_env *env, int site, int leg, double *out,
     const auto &v = env->lambda[site][leg];
     if (out) std::memcpy(out, v.data(), v.size() * sizeof(double));
     if (n_out) *n_out = (int)v.size();
+  });
+}
+
+int tenes_env_save_tensors(tenes_env *env, const char *dir, int chi) {
+  if (!env || !dir) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    ensure_lambda(env);
+    tenes::itps::CheckpointState st{env->arr, &env->lambda, env->lat.N_UNIT};
+    tenes::itps::save_tensors(st, dir, chi);
+  });
+}
+
+int tenes_env_load_tensors(tenes_env *env, const char *dir, int chi) {
+  if (!env || !dir) return TENES_ERR_ARGUMENT;
+  return guard(env->ctx, [&] {
+    tenes::itps::CheckpointState st{env->arr, &env->lambda, env->lat.N_UNIT};
+    tenes::itps::load_tensors(st, *env->ctx->c, dir, chi);
   });
 }
 
