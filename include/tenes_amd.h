@@ -255,8 +255,11 @@ int tenes_calc_ctm_environment(tenes_env *env, const tenes_peps_parameters *p,
  * physical legs (Make_single_tensor_density, ctm_single.cpp:43-50).  The
  * initial corners and edges keep the site bond dimensions (no padding to
  * CHI, ctm_single.cpp:751-995); edges are stored as (chi, chi, D, 1).
- * Corner projectors only (CTM_Projector_corner != 0; the single-layer up-down
- * projectors of ctm_single.cpp:267-436 are refused with TENES_ERR_ARGUMENT).  Once the site
+ * Corner and up-down projectors (CTM_Projector_corner = 0:
+ * Calc_projector_updown_blocks_single, ctm_single.cpp:267-436, whose
+ * full-SVD branch truncates with its loop over the first e78 singular
+ * values, ctm_single.cpp:396-415; with the cut on the device the shapes stay
+ * CHI-sized with zero columns where that loop shrinks them).  Once the site
  * tensors are TPOs, the contraction entry points below (one-site, op12,
  * cluster, multisite, correlation, transfer matrix) evaluate TeNeS's
  * density versions (contract_density_ctm/): Tr(rho O) with each one-site

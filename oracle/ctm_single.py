@@ -49,6 +49,56 @@ def Calc_projector_left_block_single(C1, C4, eT1, eT6, eT7, eT8, T1, T4, p: PEPS
     return PU, PL
 
 
+def _truncate_e78(s, cut0, e78, eps):
+    """ctm_single.cpp:396-415, the full-SVD branch of the up-down projectors:
+    s_c starts as cut0 zeros and the loop runs over the first e78 values
+    (not the first cut0); the first value at or below eps (relative to s[0])
+    becomes the cut, which can exceed cut0.  Entries from cut0 on are zero
+    (the vector grows by value-initialised entries) and so are entries from
+    e78 on.  Returns (cut, s_c)."""
+    brk = None
+    for i in range(min(e78, s.size)):
+        if not (s[i] / s[0] > eps):
+            brk = i
+            break
+    cut = cut0 if brk is None else brk
+    s_c = np.zeros(cut)
+    m = min(cut, cut0, e78)
+    s_c[:m] = 1.0 / np.sqrt(s[:m])
+    return cut, s_c
+
+
+def Calc_projector_updown_blocks_single(C1, C2, C3, C4, eT1, eT2, eT3, eT4, eT5, eT6, eT7, eT8,
+                                        T1, T2, T3, T4, p: PEPSParameters):
+    """ctm_single.cpp:267-436 with the exact SVD in both branches (the RSVD
+    branch truncates as the double layer, the full-SVD branch with the e78
+    loop of _truncate_e78).  Returns PU, PL with legs (e78, t41, cut)."""
+    e12, e34, e56, e78 = eT1.shape[1], eT3.shape[1], eT5.shape[1], eT7.shape[1]
+    t12, t23, t34, t41 = T1.shape[2], T2.shape[3], T3.shape[0], T4.shape[1]
+    if t41 == 1:
+        I = np.eye(e78, dtype=C1.dtype)
+        return I.reshape(e78, 1, e78), I.reshape(e78, 1, e78)
+    # the four enlarged corners (ctm_single.cpp:287-344)
+    LT = ein("ab,bxu,yal,lurd->ydxr", C1, eT1, eT8, T1).reshape(e78 * t41, e12 * t12)
+    RT = ein("ab,bxr,yau,lurd->ylxd", C2, eT3, eT2, T2).reshape(e12 * t12, e34 * t23)
+    RB = ein("ab,bxd,yar,lurd->xlyu", C3, eT5, eT4, T3).reshape(e56 * t34, e34 * t23)
+    LB = ein("ab,bxl,yad,lurd->xuyr", C4, eT7, eT6, T4).reshape(e78 * t41, e56 * t34)
+    R1 = RT.T @ LT.T  # (e34 t23, e78 t41)
+    R2 = RB.T @ LB.T
+    U, s, VT = np.linalg.svd(R1 @ R2.T)
+    cut0 = min(p.CHI, e34 * t23)
+    if t23 != 1 and p.Use_RSVD:
+        cut, s_c = _truncate(s, cut0, p.Inverse_projector_cut)
+    else:
+        cut, s_c = _truncate_e78(s, cut0, e78, p.Inverse_projector_cut)
+    assert cut <= U.shape[1]
+    U_c = U[:, :cut] * s_c
+    VT_c = VT[:cut] * s_c[:, None]
+    PU = (R2.T @ np.conj(VT_c).T).reshape(e78, t41, cut)
+    PL = (R1.T @ np.conj(U_c)).reshape(e78, t41, cut)
+    return PU, PL
+
+
 def Calc_Next_CTM_single(C1, C4, eT1, eT6, PU, PL):
     """ctm_single.cpp:438-451: C1' = PU (C1 eT1), C4' = (eT6 C4) PL, each
     divided by its largest |entry|."""
@@ -66,12 +116,13 @@ def Calc_Next_eT_single(eT8, T1, PU, PL):
 R_RIGHT, R_TOP, R_BOTTOM = (2, 3, 0, 1), (1, 2, 3, 0), (3, 0, 1, 2)
 
 
-def _projectors(L, args_of, p):
+def _projectors(L, args_of, ud_args_of, p):
     PUs, PLs = [None] * L, [None] * L
     for n in range(L):
-        if not p.CTM_Projector_corner:
-            raise NotImplementedError("single-layer up-down projectors are not restated")
-        PUs[n], PLs[n] = Calc_projector_left_block_single(*args_of(n), p)
+        if p.CTM_Projector_corner:
+            PUs[n], PLs[n] = Calc_projector_left_block_single(*args_of(n), p)
+        else:
+            PUs[n], PLs[n] = Calc_projector_updown_blocks_single(*ud_args_of(n), p)
     return PUs, PLs
 
 
@@ -89,7 +140,12 @@ def Left_move_single(env: Env, A, ix, p, lat):
         i, j, k, l = sites(iy)
         return env.C1[i], env.C4[l], env.eTt[i], env.eTb[l], env.eTl[l], env.eTl[i], A[i], A[l]
 
-    PUs, PLs = _projectors(L, args, p)
+    def ud_args(iy):
+        i, j, k, l = sites(iy)
+        return (env.C1[i], env.C2[j], env.C3[k], env.C4[l], env.eTt[i], env.eTt[j], env.eTr[j],
+                env.eTr[k], env.eTb[k], env.eTb[l], env.eTl[l], env.eTl[i], A[i], A[j], A[k], A[l])
+
+    PUs, PLs = _projectors(L, args, ud_args, p)
     C1b, C4b, eTlb = list(env.C1), list(env.C4), list(env.eTl)
     for iy in range(L):
         i, j, k, l = sites(iy)
@@ -114,7 +170,12 @@ def Right_move_single(env: Env, A, ix, p, lat):
         i, j, k, l = sites(iy)
         return env.C3[k], env.C2[j], env.eTb[k], env.eTt[j], env.eTr[j], env.eTr[k], Ar[k], Ar[j]
 
-    PUs, PLs = _projectors(L, args, p)
+    def ud_args(iy):
+        i, j, k, l = sites(iy)
+        return (env.C3[k], env.C4[l], env.C1[i], env.C2[j], env.eTb[k], env.eTb[l], env.eTl[l],
+                env.eTl[i], env.eTt[i], env.eTt[j], env.eTr[j], env.eTr[k], Ar[k], Ar[l], Ar[i], Ar[j])
+
+    PUs, PLs = _projectors(L, args, ud_args, p)
     C2b, C3b, eTrb = list(env.C2), list(env.C3), list(env.eTr)
     for iy in range(L):
         i, j, k, l = sites(iy)
@@ -139,7 +200,12 @@ def Top_move_single(env: Env, A, iy, p, lat):
         i, j, k, l = sites(ix)
         return env.C2[j], env.C1[i], env.eTr[j], env.eTl[i], env.eTt[i], env.eTt[j], At[j], At[i]
 
-    PUs, PLs = _projectors(L, args, p)
+    def ud_args(ix):
+        i, j, k, l = sites(ix)
+        return (env.C2[j], env.C3[k], env.C4[l], env.C1[i], env.eTr[j], env.eTr[k], env.eTb[k],
+                env.eTb[l], env.eTl[l], env.eTl[i], env.eTt[i], env.eTt[j], At[j], At[k], At[l], At[i])
+
+    PUs, PLs = _projectors(L, args, ud_args, p)
     C1b, C2b, eTtb = list(env.C1), list(env.C2), list(env.eTt)
     for ix in range(L):
         i, j, k, l = sites(ix)
@@ -164,7 +230,12 @@ def Bottom_move_single(env: Env, A, iy, p, lat):
         i, j, k, l = sites(ix)
         return env.C4[l], env.C3[k], env.eTl[l], env.eTr[k], env.eTb[k], env.eTb[l], Ab[l], Ab[k]
 
-    PUs, PLs = _projectors(L, args, p)
+    def ud_args(ix):
+        i, j, k, l = sites(ix)
+        return (env.C4[l], env.C1[i], env.C2[j], env.C3[k], env.eTl[l], env.eTl[i], env.eTt[i],
+                env.eTt[j], env.eTr[j], env.eTr[k], env.eTb[k], env.eTb[l], Ab[l], Ab[i], Ab[j], Ab[k])
+
+    PUs, PLs = _projectors(L, args, ud_args, p)
     C3b, C4b, eTbb = list(env.C3), list(env.C4), list(env.eTb)
     for ix in range(L):
         i, j, k, l = sites(ix)

@@ -229,6 +229,10 @@ struct ProjectorJob {
   std::function<tensor(const tensor &)> finish_a, finish_b;
   int64_t r0 = 0, r1 = 0, rb = 0, c0 = 0, c1 = 0, cb = 0;  // (t, e, tc) legs
   int kmax = 0;
+  // > 0: the truncation of the single-layer up-down projectors' full-SVD
+  // branch (ctm_single.cpp:396-409), whose loop runs over the first e78
+  // singular values (this value) instead of the first cut
+  int e78_loop = 0;
   int slot = 0;  // pinned slot of the singular values
   tn::SvdPending sv;
   // warm-start entry to update with the Ritz-value range once known
@@ -324,8 +328,11 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
     tn::SvdResult sv = tn::svd_top_detach(job.sv);
     ctx.pending_svd.push_back({job.wkey, job.warm_store, job.slot, nall});
     const int64_t k = sv.s_dev.size();
-    tensor sc = tn::empty(ctx, Shape{k});
-    tn::kern::projector_scale(sv.s_dev.data(), (int)k, pp.Inverse_projector_cut,
+    tensor sc = job.e78_loop > 0 ? tn::zeros(ctx, Shape{k}) : tn::empty(ctx, Shape{k});
+    // the single-layer up-down rule scales only the first e78 directions
+    // (the rest stay zero columns, as the reference's zero-filled s_c)
+    const int64_t ks = job.e78_loop > 0 ? std::min<int64_t>(k, job.e78_loop) : k;
+    tn::kern::projector_scale(sv.s_dev.data(), (int)ks, pp.Inverse_projector_cut,
                               sc.data(), ctx.stream());
     tensor Uc = tn::reshape(sv.U, Shape{job.r0, job.r1, job.rb, k});
     tensor Vc = tn::reshape(sv.V, Shape{job.c0, job.c1, job.cb, k});
@@ -344,13 +351,38 @@ void projectors_finish(tn::Context &ctx, ProjectorJob &job,
                              : 0.0;
   }
   std::vector<double> s_c;
-  const int cut =
-      truncate_spectrum(sv.s, std::min<int>(job.kmax, (int)sv.s.size()),
-                        pp.Inverse_projector_cut, s_c);
+  const int cut0 = std::min<int>(job.kmax, (int)sv.s.size());
+  int cut = 0;
+  if (job.e78_loop > 0) {
+    // ctm_single.cpp:396-415: s_c starts as cut0 zeros; the loop over the
+    // first e78 values stops at the first one at or below the cutoff and
+    // makes that index the cut, which can exceed cut0: entries from cut0 on
+    // are zero (the vector's resize), and so are entries from e78 on
+    const std::vector<double> &sa = sv.s_all.size() > sv.s.size() ? sv.s_all : sv.s;
+    const int lim = std::min<int>(job.e78_loop, (int)sa.size());
+    cut = cut0;
+    for (int i = 0; i < lim; ++i)
+      if (!(sa[i] / sa[0] > pp.Inverse_projector_cut)) {
+        cut = i;
+        break;
+      }
+    s_c.assign(std::max(cut, 0), 0.0);
+    for (int j = 0; j < std::min(std::min(cut, cut0), job.e78_loop); ++j)
+      s_c[j] = 1.0 / std::sqrt(sa[j]);
+  } else {
+    cut = truncate_spectrum(sv.s, cut0, pp.Inverse_projector_cut, s_c);
+  }
   TN_REQUIRE(cut >= 1, "projector: no singular value above the cutoff");
   tensor sc = tn::from_host(ctx, Shape{cut}, s_c.data());
-  tensor Uc = tn::slice(sv.U, 1, 0, cut);
-  tensor Vc = tn::slice(sv.V, 1, 0, cut);
+  // a cut beyond the computed triplets (single-layer up-down rule) adds
+  // zero-scaled columns: zero columns
+  const int kc = std::min<int>(cut, (int)sv.U.shape[1]);
+  tensor Uc = tn::slice(sv.U, 1, 0, kc);
+  tensor Vc = tn::slice(sv.V, 1, 0, kc);
+  if (kc < cut) {
+    Uc = tn::extend(Uc, Shape{Uc.shape[0], cut});
+    Vc = tn::extend(Vc, Shape{Vc.shape[0], cut});
+  }
   tn::multiply_vector_dev(Uc, sc.data(), 1);
   tn::multiply_vector_dev(Vc, sc.data(), 1);
   Uc = tn::reshape(Uc, Shape{job.r0, job.r1, job.rb, cut});
@@ -456,13 +488,12 @@ void updown_blocks_begin(
                     tn::bra(Tn3), Axes{2, 5}, Axes{3, 2}),
       Axes{2, 3, 4}, Axes{3, 1, 6});
   tensor LB = enlarged_LB(C4, eT7, eT6, Tn4);
-  // The single-layer version (ctm_single.cpp:267-436) truncates its full SVD
-  // with a loop over the first e78 singular values only (ctm_single.cpp:400),
-  // which keeps at most e78 directions and zero-pads the projectors to the
-  // cut; that behaviour is not reproduced here.
-  TN_REQUIRE(!ctx.single_layer(),
-             "up-down projectors of the single-layer network are not supported");
+  // The single-layer network (ctm_single.cpp:267-436) takes the same blocks
+  // and products with unit bra legs; its full-SVD branch truncates with a
+  // loop over the first e78 singular values (ctm_single.cpp:400), its RSVD
+  // branch as the double layer.
   const bool rsvd = t23 != 1 && pp.Use_RSVD;
+  job.e78_loop = (ctx.single_layer() && !rsvd) ? (int)e78 : 0;
   if (rsvd) {
     // Mult_col_ud / Mult_row_ud (ctm.cpp:121-172): M = (RT LT)(RB LB)^T is
     // applied block by block, RT (LT (LB (RB X))), and neither M nor RT LT,

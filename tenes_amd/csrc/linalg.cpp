@@ -192,8 +192,10 @@ SvdPending svd_top_begin_impl(Context &ctx, const Tensor *Ap, const LinearOp *op
   const int m = Ap ? (int)A.shape[0] : (int)op->m, n = Ap ? (int)A.shape[1] : (int)op->n;
   TN_REQUIRE(m >= 1 && n >= 1, "svd_top: empty matrix");
   TN_REQUIRE(k_target >= 1, "svd_top: k must be positive");
-  TN_REQUIRE(k_target <= kJacBig,
-             "svd_top: at most 256 singular triplets are supported");
+  if (k_target > kJacBig)
+    throw Error("svd_top: at most 256 singular triplets are supported (" +
+                    std::to_string(k_target) + " asked)",
+                -5);
   hipStream_t s = ctx.stream();
 
   // A ~= Lq * X * Rq^T with orthonormal Lq (m x r), Rq (n x c); X is r x c.

@@ -65,9 +65,13 @@ def test_svd_randomized_k240_gap(gpu_ctx):
 
 
 def test_svd_k_above_240_rejected(gpu_ctx):
+    """k = 241 (fewer than 16 columns of oversampling in the 256-wide block)
+    and k = 300 are refused with TENES_ERR_UNSUPPORTED."""
     A = np.random.default_rng(23).standard_normal((600, 600))
-    with pytest.raises(ta.TenesError, match="range-finder block"):
+    with pytest.raises(ta.TenesError, match="error -5: .*range-finder block"):
         ta.svd_top(gpu_ctx.tensor(A), 241)
+    with pytest.raises(ta.TenesError, match="error -5: .*at most 256"):
+        ta.svd_top(gpu_ctx.tensor(A), 300)
 
 
 def test_svd_randomized_k160_rank_deficient(gpu_ctx):

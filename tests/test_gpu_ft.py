@@ -110,23 +110,66 @@ def test_gpu_single_layer_ctm_vs_oracle(gpu_ctx):
         assert abs(got - ref[key].real) < 1e-10, (key, got, ref[key])
 
 
-def test_gpu_single_layer_updown_projectors_refused(gpu_ctx):
-    """CTM_Projector_corner = 0 on an iTPO environment is an argument error
-    (TeNeS's single-layer up-down truncation, ctm_single.cpp:400, is not
-    reproduced), raised inside a move whose rows run on side streams; the
-    context is back on its own stream afterwards (no fork left open) and the
-    environment stays usable.  A context of its own, so that a failure here
-    cannot leak into the other tests' session context."""
-    ctx = ta.Context(0, seed=99)
+@pytest.mark.parametrize("steps", [100, 300])
+@pytest.mark.parametrize("device_cut", [0, 1], ids=["host-cut", "device-cut"])
+def test_gpu_single_layer_updown_vs_oracle(gpu_ctx, steps, device_cut):
+    """Up-down projectors of the single-layer network (CTM_Projector_corner =
+    0, Calc_projector_updown_blocks_single, ctm_single.cpp:267-436, with its
+    truncation loop over the first e78 singular values): FT_TFI_square TPO
+    tensors after 100 and 300 oracle simple-update steps (beta = 1, 3), CHI
+    = 10, run to the convergence test (epsilon 1e-10) on the GPU and in the
+    oracle: the same sweep count, corner spectra to 1e-10 (measured 2e-11 at
+    beta = 1: the runs stop at a change of 1e-10) and every one-site and
+    two-site observable to 1e-10.  With the cut on the device the
+    projectors keep CHI-sized shapes with zero columns where the host cut
+    shrinks them (the spectra are compared with zero padding)."""
+    from oracle import ctm, ctm_single as CS, itps, contract_density as KD
+    from tenes_amd import finite_temperature as FTG
+
     case = F.setup("FT_TFI_square")
-    Tn, _ = _oracle_tpo(case, 50)
-    genv = ta.Environment(ctx, 2, 2)
+    lat = case["lat"]
+    Tn, _ = _oracle_tpo(case, steps)
+    p = ctm.PEPSParameters(CHI=10, Max_CTM_Iteration=100, CTM_Convergence_Epsilon=1e-10,
+                           CTM_Projector_corner=False)
+    env, count, _ = CS.Calc_CTM_Environment_density(Tn, p, lat)
+    genv = ta.Environment(gpu_ctx, 2, 2)
     genv.set_site_tensors(Tn)
+    params = ta.peps_parameters(CHI=10, Max_CTM_Iteration=100, CTM_Convergence_Epsilon=1e-10,
+                                CTM_Projector_corner=0, projector_cut_on_device=device_cut)
+    rec = FTG.measure(genv, 2, 2, 0, case["onesite"], case["twosite"], case["site_ops"], 0, [],
+                      params, 1.0)
+    assert rec["ctm_count"] == count
+    for i in range(4):
+        for k in KINDS[:4]:
+            a = np.linalg.svd(getattr(env, k)[i], compute_uv=False)
+            b = np.linalg.svd(genv.get(k, i).numpy(), compute_uv=False)
+            assert spectra_diff(a / np.linalg.norm(a), b / np.linalg.norm(b)) < 1e-10, (k, i)
+    one, _ = itps.measure_onesite(env, Tn, lat, case["onesite"], K=KD)
+    two, _ = itps.measure_twosite(env, Tn, lat, case["twosite"], case["onesite"], K=KD)
+    for g in one:
+        for i in one[g]:
+            assert abs(rec["onesite"][g][i] - one[g][i]) < 1e-10, (g, i)
+    for g in two:
+        for k in two[g]:
+            assert abs(rec["twosite"][g][k] - two[g][k]) < 1e-10, (g, k)
+
+
+def test_gpu_error_inside_a_forked_move_restores_the_context():
+    """An error raised inside a move whose rows run on side streams (here
+    the refusal of a projector SVD beyond 240 triplets: CHI = 250 with
+    projector matrices 256 wide from the first sweep, random D=4 tensors):
+    the context is back on its own stream afterwards (no fork left open) and
+    the environment stays usable.  A context of its own, so that a failure
+    here cannot leak into the other tests' session context."""
+    from tenes_amd import inputs
+
+    ctx = ta.Context(0, seed=99)
+    genv = ta.Environment(ctx, 2, 2)
+    genv.set_site_tensors(inputs.random_tensors(4, 4, 2, seed=3))
     assert ctx.lane_state() == (False, -1)
-    with pytest.raises(ta.TenesError, match="up-down projectors"):
-        genv.calc_ctm_environment_density(ta.peps_parameters(CHI=10, Max_CTM_Iteration=2,
-                                                             CTM_Projector_corner=0))
+    with pytest.raises(ta.TenesError, match="error -5: .*range-finder block"):
+        genv.calc_ctm_environment(ta.peps_parameters(CHI=250, Max_CTM_Iteration=2))
     assert ctx.lane_state() == (False, -1)
-    n, _ = genv.calc_ctm_environment_density(ta.peps_parameters(CHI=10, Max_CTM_Iteration=2))
+    n, _ = genv.calc_ctm_environment(ta.peps_parameters(CHI=20, Max_CTM_Iteration=2))
     assert ctx.lane_state() == (False, -1)
     assert n >= 1 and np.isfinite(genv.contract_one_site(0))
