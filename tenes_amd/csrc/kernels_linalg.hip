@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
   __shared__ double s_dg[kMaxDim], s_scl[kMaxDim];
   __shared__ double s_rd[kMaxDim];  // 1 / F[i][i] (0 for a dropped column)
   __shared__ double s_shift, s_thr, s_floor;
-  __shared__ int s_zero;
+  __shared__ int s_zero, s_near;
   const int pp = (p + 15) / 16 * 16;  // padded size (identity padding)
   const int nb = pp / 16;
   const int ld = pp + 1;
@@ -224,6 +224,41 @@ __global__ __launch_bounds__(kCholThreads) void chol_inv_kernel(
         Rinv[r * p + c] = (r == c) ? 1.0 : 0.0;
       }
     return;
+  }
+  // Near-identity Gram matrix (the second pass of a two-pass CholeskyQR):
+  // with Gs = I + E, every |E_ij| <= 1e-10 and no shift, the factor is
+  // R~ = I + U, U = triu(E) with halved diagonal, and R~^{-1} = I - U, both
+  // exact to O(||U||^2) <= 2e-16 (||U||_2 <= 128 * 1e-10): no pivot chain.
+  // No pivot can fall to the drop threshold (pivots are 1 +- 1e-10), so the
+  // factorisation would drop nothing either.
+  if (!pre && shift_coef == 0.0) {
+    if (tid == 0) s_near = 1;
+    __syncthreads();
+    bool near = true;
+#pragma unroll 4
+    for (int u = 0; u < 32; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = r0 + 4 * u, c = c0 + 64 * h;
+        if (r < p && c < p) near = near && fabs(A[r * ld + c] - (r == c ? 1.0 : 0.0)) <= 1e-10;
+      }
+    if (!near) s_near = 0;
+    __syncthreads();
+    if (s_near) {
+#pragma unroll 4
+      for (int u = 0; u < 32; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = r0 + 4 * u, c = c0 + 64 * h;
+          if (r < p && c < p) {
+            const double e = c > r ? A[r * ld + c] : (c == r ? 0.5 * (A[r * ld + r] - 1.0) : 0.0);
+            const double id = r == c ? 1.0 : 0.0;
+            if (R) R[r * p + c] = (id + e) * s_dg[c];
+            Rinv[r * p + c] = s_scl[r] * (id - e);
+          }
+        }
+      return;
+    }
   }
   const double shift = s_shift, thr = s_thr;
   for (int i = tid; i < p; i += kCholThreads) A[i * ld + i] += shift;
