@@ -626,9 +626,13 @@ __device__ __forceinline__ void rr_pair(int s, int k, int pe, int &i, int &j) {
   }
 }
 
+// FULL: m == kMaxDim (the randomized path's projected problems), so the
+// row guards of the inner loops are compile-time true.
+template <bool FULL>
 __global__ __launch_bounds__(1024) void jacobi_kernel(
-    double *Wg, int m, int p, double tol, int max_sweeps, double *sigma,
+    double *Wg, int m_arg, int p, double tol, int max_sweeps, double *sigma,
     double *rot, int *nsteps) {
+  const int m = FULL ? kMaxDim : m_arg;
   extern __shared__ double W[];  // pe columns of length m (col-major)
   __shared__ double s_off[64];
   __shared__ double s_tiny;
@@ -690,8 +694,8 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int r = gl + 16 * u;
-          xr[u] = r < m ? wi[r] : 0.0;
-          yr[u] = r < m ? wj[r] : 0.0;
+          xr[u] = (FULL || r < m) ? wi[r] : 0.0;
+          yr[u] = (FULL || r < m) ? wj[r] : 0.0;
           a += xr[u] * xr[u];
           bb += yr[u] * yr[u];
           g += xr[u] * yr[u];
@@ -715,7 +719,7 @@ __global__ __launch_bounds__(1024) void jacobi_kernel(
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
               const int r = gl + 16 * u;
-              if (r < m) {
+              if (FULL || r < m) {
                 wi[r] = c * xr[u] - sn * yr[u];
                 wj[r] = sn * xr[u] + c * yr[u];
               }
@@ -962,7 +966,10 @@ void linalg_init_device() {
       hipFuncAttributeMaxDynamicSharedMemorySize,
       (int)chol_lds_bytes(kMaxDim)));
   TN_HIP_CHECK(hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&jacobi_kernel),
+      reinterpret_cast<const void *>(&jacobi_kernel<true>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  TN_HIP_CHECK(hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&jacobi_kernel<false>),
       hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   TN_HIP_CHECK(hipFuncSetAttribute(
       reinterpret_cast<const void *>(&trsm_ut_kernel),
@@ -1042,8 +1049,12 @@ void jacobi_svd(double *W, int m, int p, int batch, double tol,
   }
   const int pe = p + (p & 1);
   const size_t lds = (size_t)m * pe * sizeof(double);
-  hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(1024), lds, s, W, m, p,
-                     tol, max_sweeps, sigma, rot, nsteps);
+  if (m == kMaxDim)
+    hipLaunchKernelGGL((jacobi_kernel<true>), dim3(batch), dim3(1024), lds, s, W, m, p,
+                       tol, max_sweeps, sigma, rot, nsteps);
+  else
+    hipLaunchKernelGGL((jacobi_kernel<false>), dim3(batch), dim3(1024), lds, s, W, m, p,
+                       tol, max_sweeps, sigma, rot, nsteps);
   TN_HIP_CHECK(hipGetLastError());
 }
 
