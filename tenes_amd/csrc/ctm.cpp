@@ -1441,6 +1441,24 @@ void run_ctm(std::vector<CtmRun> &runs) {
         ctx.svd_calls += g->svd_calls;
       }
     }
+    // A run that cannot converge (epsilon <= 0: sig_max < epsilon never
+    // holds) observes its checks only through the last one's sig_max (and
+    // the per-sweep print at print level 3): when no active run needs this
+    // round's check, it is skipped, and with it the round's synchronisation.
+    // Runs with the cut on the device read their projector statistics at
+    // that synchronisation and always take it.
+    bool need_check = false;
+    for (CtmRun *r : act)
+      need_check = need_check || r->pp.CTM_Convergence_Epsilon > 0.0 ||
+                   r->count + 1 >= r->pp.Max_CTM_Iteration || r->pp.print_level >= 3 ||
+                   r->ps.projector_cut_on_device || r->graphs;
+    if (!need_check) {
+      for (CtmRun *r : act) {
+        r->count += 1;
+        r->keep_old();
+      }
+      continue;
+    }
     std::vector<const tensor *> all;
     for (CtmRun *r : act) push_corner_pairs(all, r->A, r->C_old, r->lat->N_UNIT);
     const auto lam = spectra(all);
