@@ -11,7 +11,7 @@ the site tensors and `--warmup` sweeps are run untimed (chi reaches 100 in
 the first move); then `--steps` sweeps are timed, each = 2 left + 2 right +
 2 top + 2 bottom moves (16 projectors with a truncated FP64 SVD of a
 2500 x 2500 matrix, 16 Calc_Next_CTM, 32 Calc_Next_eT) + the convergence
-check (epsilon = 0: no early exit).
+check of every sweep (epsilon = 1e-300: checked, never reached).
 
 Passes, in order:
   1. timed region (the `value`): no per-launch instrumentation;
@@ -320,7 +320,11 @@ def main():
         ctx.set_max_moves_in_flight(args.moves_in_flight)
     fx = load_fixture(args.D, args.chi, args.su_steps)
     env, H, Tn = build_inputs(ta, ctx, args.D, args.su_steps, 0.01, fx)
-    kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=0.0)
+    # a sweep is TeNeS's loop body: the moves and Check_Convergence_CTM.  An
+    # epsilon of 1e-300 makes every sweep take the convergence check (the
+    # library skips the checks a run with epsilon <= 0 cannot observe) and
+    # never stops early (sig_max stays far above it)
+    kw = dict(CHI=args.chi, CTM_Convergence_Epsilon=1e-300)
     if args.host_cut:
         kw["projector_cut_on_device"] = 0
     if args.graph:
