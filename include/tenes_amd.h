@@ -446,9 +446,11 @@ typedef struct tenes_tm_parameters {
 } tenes_tm_parameters;
 void tenes_tm_parameters_default(tenes_tm_parameters *p);
 /* TransferMatrix_ctm::eigenvalues, transfer_matrix.cpp:84-142.
- * arnoldi_maxiter > 1 on the Arnoldi path (dimension above
- * maxdim_dense_eigensolver and at least arnoldi_maxdim) returns
- * TENES_ERR_UNSUPPORTED: the restarts of arnoldi.cpp:117-167 are not built.
+ * On the Arnoldi path (dimension above maxdim_dense_eigensolver) up to
+ * arnoldi_maxiter passes over a basis of arnoldi_maxdim vectors, restarted
+ * between passes as arnoldi.cpp:117-167 does (shifts: the largest
+ * arnoldi_maxdim - arnoldi_restartdim Ritz values); num_eigvals above
+ * arnoldi_maxdim there returns TENES_ERR_ARGUMENT.
  * eig: 2*num_eigvals doubles (re, im), sorted by |lambda| descending;
  * *n_out receives the number of eigenvalues written. */
 int tenes_env_transfer_matrix_eigenvalues(tenes_env *env, int dir,

@@ -126,47 +126,107 @@ class TransferMatrixCTM:
         return res.reshape(c0 * c1, c0 * c1)
 
     def eigenvalues(self, d, fixed, prm: TMParameters):
-        """transfer_matrix.cpp:84-142 with Arnoldi (arnoldi.cpp:56-181,
-        maxiter = 1: no restart)."""
+        """transfer_matrix.cpp:84-142: dense eigensolver up to
+        maxdim_dense_eigensolver, Arnoldi (with restarts) above."""
         N = self.dim(d, fixed)
         nev = min(prm.num_eigvals, N)
         if N == 1:
             return np.array([1.0 + 0j])
         if N <= prm.maxdim_dense_eigensolver:
             return eigen(self.matrix(d, fixed), nev)[0]
-        maxvec = prm.arnoldi_maxdim
+        maxvec, maxiter = prm.arnoldi_maxdim, prm.arnoldi_maxiter
         if N < maxvec:
-            maxvec = N
-        v0 = self.initial_vector(d, fixed)
-        dt = np.result_type(v0.dtype, np.float64)
-        Q = np.zeros((maxvec + 1, N), dtype=dt)
-        H = np.zeros((maxvec + 1, maxvec), dtype=dt)
-        Q[0] = v0
+            maxvec, maxiter = N, 1
+        return arnoldi(lambda v: self.matvec(d, v, fixed), self.initial_vector(d, fixed), nev,
+                       maxvec, prm.arnoldi_restartdim, maxiter, prm.arnoldi_rtol)
 
-        def orthonormalize(k):
-            for j in range(k):
-                x = np.vdot(Q[j], Q[k])
-                H[j, k - 1] = x
-                Q[k] = Q[k] - Q[j] * x
-            x = np.sqrt(np.real(np.vdot(Q[k], Q[k])))
-            Q[k] = Q[k] * (1.0 / x)
-            if k > 0:
-                H[k, k - 1] = x
 
-        def residue(k):
-            w, last = eigen(H[:k, :k], nev)
-            return np.abs(H[k, k - 1] * last) / np.abs(w)
+class ArnoldiState:
+    """Krylov basis Q ((maxvec+1) x N) and Hessenberg matrix H ((maxvec+1) x
+    maxvec) of arnoldi.cpp:34-41, one array each as there: a restart writes
+    its maxvec x maxvec block of H and the later steps overwrite only the
+    entries arnoldi.cpp's orthonormalize writes."""
 
-        orthonormalize(0)
-        for k in range(1, nev + 1):
-            Q[k] = self.matvec(d, Q[k - 1], fixed)
-            orthonormalize(k)
+    def __init__(self, N, maxvec, dtype):
+        self.maxvec = maxvec
+        self.Q = np.zeros((maxvec + 1, N), dtype=dtype)
+        self.H = np.zeros((maxvec + 1, maxvec), dtype=dtype)
+
+    def orthonormalize(self, k):
+        """arnoldi.cpp:98-112: modified Gram-Schmidt of Q[k] against
+        Q[0..k-1] (each projection taken from the already updated Q[k]),
+        H[j, k-1] = <Q[j], Q[k]>, H[k, k-1] = ||Q[k]||."""
+        Q, H = self.Q, self.H
+        for j in range(k):
+            x = np.vdot(Q[j], Q[k])
+            H[j, k - 1] = x
+            Q[k] = Q[k] - Q[j] * x
+        x = np.sqrt(np.real(np.vdot(Q[k], Q[k])))
+        Q[k] = Q[k] * (1.0 / x)
+        if k > 0:
+            H[k, k - 1] = x
+
+    def residue(self, k, nev):
+        """arnoldi.cpp:169-181: |H[k, k-1] * last component of the Ritz
+        vector| / |Ritz value| of the nev largest Ritz values of H[:k, :k]."""
+        w, last = eigen(self.H[:k, :k], nev)
+        return np.abs(self.H[k, k - 1] * last) / np.abs(w)
+
+    def restart(self, minvec, cutoff=1.0e-12):
+        """arnoldi.cpp:117-167: maxvec - minvec QR steps h <- q^H h q of the
+        Hessenberg block, each applied to the basis (Q[j] <- sum_i Q[i] q[i, j],
+        j < maxvec); then entries of the block below cutoff in modulus are
+        zeroed.
+
+        Each step factors h - identity(k, sigma), sigma the i-th largest Ritz
+        value.  identity (tensor.hpp:58-64) writes its value at index (k, k),
+        outside the k x k matrix, in place of (i, i), so the matrix it returns
+        is zero and every step is an unshifted QR step: q's first column is
+        h e_0 / r_00 and the new start vector Q[0] is A Q[0] / r_00, a power
+        step.  That is what the reference executes, and what this does (the
+        Ritz values it computes for the shifts have no effect and are not
+        computed here)."""
+        k = self.maxvec
+        h = self.H[:k, :k].copy()
+        for _ in range(k - minvec):
+            q, _ = np.linalg.qr(h)
+            h = q.conj().T @ h @ q
+            self.Q[:k] = q.T @ self.Q[:k]
+        h[np.abs(h) ** 2 < cutoff * cutoff] = 0.0
+        self.H[:k, :k] = h
+
+
+def arnoldi(matvec, v0, nev, maxvec, mindim, maxiter, rtol, states=None):
+    """arnoldi.cpp:43-96 (initialize + run) and :85-96 (eigenvalues): the nev
+    eigenvalues of largest modulus of the operator matvec from the start
+    vector v0, a Krylov basis of at most maxvec vectors and up to maxiter
+    passes, restarted (ArnoldiState.restart) with mindim kept directions
+    between passes.  `states`, when a list, receives a copy of (Q, H) after
+    every restart."""
+    N = v0.size
+    if nev > maxvec:
+        raise ValueError("Arnoldi: more eigenvalues than basis vectors")
+    mindim = max(mindim, nev)
+    st = ArnoldiState(N, maxvec, np.result_type(v0.dtype, np.float64))
+    st.Q[0] = v0
+    st.orthonormalize(0)
+    for k in range(1, nev + 1):
+        st.Q[k] = matvec(st.Q[k - 1])
+        st.orthonormalize(k)
+    res_nev = 1.0
+    for it in range(1, maxiter + 1):
         for k in range(nev + 1, maxvec + 1):
-            Q[k] = self.matvec(d, Q[k - 1], fixed)
-            orthonormalize(k)
-            if residue(k)[nev - 1] <= prm.arnoldi_rtol:
+            st.Q[k] = matvec(st.Q[k - 1])
+            st.orthonormalize(k)
+            res_nev = st.residue(k, nev)[nev - 1]
+            if res_nev <= rtol:
                 break
-        return eigen(H[:maxvec, :maxvec], nev)[0]
+        if res_nev <= rtol or it == maxiter:
+            break
+        st.restart(mindim)
+        if states is not None:
+            states.append((st.Q.copy(), st.H.copy()))
+    return eigen(st.H[:maxvec, :maxvec], nev)[0]
 
 
 def correlation_length(eigvals, L):

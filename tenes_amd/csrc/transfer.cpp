@@ -4,10 +4,13 @@
 // host, where its eigenvalues are computed (TeNeS does the same with dgeev).
 #include "transfer.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <string>
 
 #include "eig_host.hpp"
 #include "kernels.hpp"
+#include "small_dense.hpp"
 
 namespace tenes {
 namespace itps {
@@ -147,12 +150,13 @@ std::vector<std::complex<double>> TransferMatrix_ctm::eigenvalues(
     maxvec = (int)N;
     maxiter = 1;  // transfer_matrix.cpp:117-120
   }
-  // The restarts of arnoldi.cpp:72-86, 117-167 (maxiter > 1) are not built:
-  // refused rather than run as a single pass.
-  if (maxiter > 1)
-    throw tn::Error("transfer matrix: arnoldi_maxiter > 1 (Arnoldi restarts) is "
-                    "not supported",
-                    -5);
+  // arnoldi.cpp:60-67 writes Q[1..nev] before it looks at maxvec: more
+  // eigenvalues than basis vectors would run past the basis
+  if (nev > maxvec)
+    throw tn::Error("transfer matrix: num_eigvals (" + std::to_string(nev) +
+                        ") exceeds arnoldi_maxdim (" + std::to_string(maxvec) + ")",
+                    -3);
+  const int mindim = std::max(params.arnoldi_restartdim, nev);  // arnoldi.cpp:63-65
   hipStream_t s = ctx.stream();
   tensor Qall = tn::empty(ctx, Shape{maxvec + 1, (int64_t)N});
   auto Qp = [&](int k) { return Qall.data() + (int64_t)k * N; };
@@ -204,6 +208,68 @@ std::vector<std::complex<double>> TransferMatrix_ctm::eigenvalues(
     return res;
   };
 
+  // arnoldi.cpp:117-167: maxvec - minvec QR steps h <- q^T h q of the
+  // Hessenberg block, applied to the basis, then entries of the block below
+  // 1e-12 in modulus zeroed.  Each reference step factors
+  // h - identity(k, sigma) with sigma a Ritz value, but identity
+  // (tensor.hpp:58-64) sets the out-of-range element (k, k) in place of
+  // (i, i) and returns a zero matrix: every step is an unshifted QR step (the
+  // new start vector is A Q[0] / r_00, a power step), and so it is here.  The
+  // product of the steps' q's turns the basis once, Q[j] <- sum_i Q[i] qt[i][j]
+  // for j < maxvec, as a (maxvec x maxvec) x (maxvec x N) product on the
+  // device.  Later steps overwrite only what orthonormalize writes; the rest
+  // of the restarted H stays in place, as in the single H array of
+  // arnoldi.cpp.
+  auto restart = [&](int minvec) {
+    const int k = maxvec;
+    std::vector<double> h((size_t)k * k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) h[(size_t)i * k + j] = Hh[(size_t)i * maxvec + j];
+    std::vector<double> qt((size_t)k * k, 0.0), q, r, t((size_t)k * k);
+    for (int i = 0; i < k; ++i) qt[(size_t)i * k + i] = 1.0;
+    for (int step = 0; step < k - minvec; ++step) {
+      tn::host::qr(h, k, k, q, r);
+      // h <- q^T (h q)
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+          double acc = 0.0;
+          for (int l = 0; l < k; ++l) acc += h[(size_t)i * k + l] * q[(size_t)l * k + j];
+          t[(size_t)i * k + j] = acc;
+        }
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+          double acc = 0.0;
+          for (int l = 0; l < k; ++l) acc += q[(size_t)l * k + i] * t[(size_t)l * k + j];
+          h[(size_t)i * k + j] = acc;
+        }
+      // qt <- qt q
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+          double acc = 0.0;
+          for (int l = 0; l < k; ++l) acc += qt[(size_t)i * k + l] * q[(size_t)l * k + j];
+          t[(size_t)i * k + j] = acc;
+        }
+      qt.swap(t);
+    }
+    if (k > minvec) {
+      // new Q[j] = sum_i qt[i][j] Q[i]: (qt^T) x Q
+      std::vector<double> qtt((size_t)k * k);
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) qtt[(size_t)j * k + i] = qt[(size_t)i * k + j];
+      tensor Qt = tn::from_host(ctx, Shape{k, k}, qtt.data());
+      tensor Qk = tn::borrow(ctx, Shape{k, (int64_t)N}, Qall.data());
+      tensor Qn = tn::tensordot(Qt, Qk, Axes{1}, Axes{0});
+      TN_HIP_CHECK(hipMemcpyAsync(Qall.data(), Qn.data(), (size_t)k * N * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+    }
+    const double cut2 = 1.0e-12 * 1.0e-12;
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        double v = h[(size_t)i * k + j];
+        Hh[(size_t)i * maxvec + j] = v * v < cut2 ? 0.0 : v;
+      }
+  };
+
   tensor v0 = initial_vector(dir, fixed_coord);
   TN_HIP_CHECK(hipMemcpyAsync(Qp(0), v0.data(), N * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
@@ -212,11 +278,17 @@ std::vector<std::complex<double>> TransferMatrix_ctm::eigenvalues(
     apply(k);
     orthonormalize(k);
   }
-  for (int k = nev + 1; k <= maxvec; ++k) {
-    apply(k);
-    orthonormalize(k);
-    auto res = residue(k);
-    if (res[nev - 1] <= params.arnoldi_rtol) break;
+  // arnoldi.cpp:68-84
+  double res_nev = 1.0;
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    for (int k = nev + 1; k <= maxvec; ++k) {
+      apply(k);
+      orthonormalize(k);
+      res_nev = residue(k)[nev - 1];
+      if (res_nev <= params.arnoldi_rtol) break;
+    }
+    if (res_nev <= params.arnoldi_rtol || iter == maxiter) break;
+    restart(mindim);
   }
   std::vector<double> h((size_t)maxvec * maxvec);
   for (int i = 0; i < maxvec; ++i)

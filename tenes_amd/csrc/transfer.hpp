@@ -36,9 +36,10 @@ class TransferMatrix_ctm {
       : lattice(lattice), Tn(Tn), C1(C1), C2(C2), C3(C3), C4(C4), eTl(eTl),
         eTt(eTt), eTr(eTr), eTb(eTb) {}
 
-  // transfer_matrix.cpp:84-142.  Arnoldi restarts (maxiter > 1 on the
-  // Arnoldi path) are not implemented and raise an error with status -5
-  // (TENES_ERR_UNSUPPORTED); maxiter = 1 is the reference default.
+  // transfer_matrix.cpp:84-142 with the Arnoldi passes and restarts of
+  // arnoldi.cpp:56-167 (maxiter = 1, the reference default, never restarts).
+  // num_eigvals above arnoldi_maxdim on the Arnoldi path raises an error with
+  // status -3 (the reference would index past its Krylov basis).
   std::vector<std::complex<double>> eigenvalues(
       int dir, int fixed_coord, const TransferMatrix_Parameters &params) const;
 

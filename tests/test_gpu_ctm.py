@@ -237,6 +237,35 @@ def test_transfer_matrix_arnoldi(gpu_ctx):
         assert abs(abs(got[0]) - abs(dense[0])) < 1e-10 * abs(dense[0])
 
 
+@pytest.mark.parametrize("maxdim,restartdim,maxiter", [(12, 8, 3), (12, 8, 10), (16, 6, 10),
+                                                         (10, 4, 20)])
+def test_transfer_matrix_arnoldi_restarts(gpu_ctx, maxdim, restartdim, maxiter):
+    """Restarted Arnoldi (arnoldi.cpp:56-167, arnoldi_maxiter > 1) on the
+    chi = 20 transfer matrix (N = 400) with a basis too small to converge in
+    one pass: the 4 eigenvalues agree with the oracle's restarted run to 1e-8
+    (moduli, relative; 1e-15 noise in the oracle's matvec moves them by at most
+    5e-10 here) and, after 10 passes that keep more directions than
+    eigenvalues, with the dense eigenvalues to 1e-11 (one pass of a 12-vector
+    basis leaves them at 2e-4).  With restartdim = num_eigvals (10, 4, 20) the
+    column before the first recomputed one misses the residual term the
+    restart drops (as in the reference) and the run stalls at 4e-7 of the
+    dense values, in the oracle alike."""
+    Tn, lat, env, _ = oracle_env("heisenberg", 2, 20, 60, 6)
+    genv = upload_env(gpu_ctx, Tn, env)
+    tm = TMo.TransferMatrixCTM(lat, Tn, env)
+    kw = dict(num_eigvals=4, arnoldi_maxdim=maxdim, arnoldi_restartdim=restartdim,
+              arnoldi_maxiter=maxiter)
+    for d in (0, 1):
+        ref = tm.eigenvalues(d, 0, TMo.TMParameters(**kw))
+        got = genv.transfer_matrix_eigenvalues(d, 0, ta.tm_parameters(**kw))
+        assert np.max(np.abs(np.abs(got) - np.abs(ref)) / np.abs(ref)) < 1e-8, (got, ref)
+        if maxiter >= 10 and restartdim > 4:
+            dense = TMo.eigen(tm.matrix(d, 0), 4)[0]
+            assert np.max(np.abs(np.abs(got) - np.abs(dense)) / np.abs(dense)) < 1e-11
+    with pytest.raises(ta.TenesError, match="exceeds arnoldi_maxdim"):
+        genv.transfer_matrix_eigenvalues(0, 0, ta.tm_parameters(num_eigvals=6, arnoldi_maxdim=5))
+
+
 @pytest.mark.parametrize("model,D,chi,sweeps", [("heisenberg", 2, 10, 6), ("tfi", 2, 16, 5),
                                                 ("heisenberg", 3, 20, 4)])
 def test_ctm_environment_observables(gpu_ctx, model, D, chi, sweeps):
