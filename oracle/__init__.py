@@ -19,7 +19,10 @@ Pinning (what checks each part besides retyped formulas):
                     and gauge invariants (tests/test_oracle_*.py).
   * full_update     test/data/full_update.dat golden (G6).
   * transfer        dense eigendecomposition of the full transfer matrix and
-                    the correlation_length.dat goldens.
+                    the correlation_length.dat goldens; the Arnoldi restarts
+                    (no golden runs them) by the power-step, similarity,
+                    Arnoldi-relation and convergence checks of
+                    tests/test_oracle_arnoldi.py.
   * contract (clusters up to 4x4)  the contraction trees of the 1x1, 1x2,
                     2x1 and 2x2 clusters, a D = 1 closed form and the
                     correlation functions (tests/test_oracle_cluster.py);
