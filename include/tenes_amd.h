@@ -97,6 +97,14 @@ int tenes_ctx_set_fused_site(tenes_ctx *ctx, int on);
  * rounding (split-K sums in another order).  Off by default.  No
  * counterpart in the reference. */
 int tenes_ctx_set_autotune(tenes_ctx *ctx, int on);
+/* Stream-K launches (on != 0) for the FP64 products whose 128 x 128 tile
+ * grid leaves at least half a round of 512 workgroups after its whole rounds
+ * (2500 x 2500: 400 tiles) with K >= 1024: the K iterations are split
+ * evenly over 512 workgroups and a split tile is summed, in a fixed order,
+ * by the last workgroup of its share to finish (results change only at
+ * rounding, identical from launch to launch).  Off by default; the
+ * autotuner tries it as one candidate.  No counterpart in the reference. */
+int tenes_ctx_set_stream_k(tenes_ctx *ctx, int on);
 /* Truncated SVDs (tenes_svd_top and the projectors) since the last reset and
  * the Jacobi rounds they took (a round orthogonalises p/2 disjoint column
  * pairs of the p x p projected matrix; a sweep is p - 1 rounds). */

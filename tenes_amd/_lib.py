@@ -64,6 +64,7 @@ class TmParameters(C.Structure):
 _SIGS = {
     "tenes_abi_version": (C.c_int, []),
     "tenes_ctx_set_autotune": (C.c_int, [C.c_void_p, C.c_int]),
+    "tenes_ctx_set_stream_k": (C.c_int, [C.c_void_p, C.c_int]),
     "tenes_ctx_create": (C.c_int, [C.c_int, C.c_void_p, c_void_pp]),
     "tenes_ctx_destroy": (C.c_int, [C.c_void_p]),
     "tenes_ctx_last_error": (C.c_char_p, [C.c_void_p]),

@@ -73,6 +73,10 @@ class Context:
         """GEMM autotuning (tenes_ctx_set_autotune)."""
         self.check(self.lib.tenes_ctx_set_autotune(self.h, 1 if on else 0))
 
+    def set_stream_k(self, on: bool):
+        """Stream-K launches of the qualifying FP64 GEMMs (tenes_ctx_set_stream_k)."""
+        self.check(self.lib.tenes_ctx_set_stream_k(self.h, 1 if on else 0))
+
     def set_fp32_contractions(self, on: bool):
         """Contractions from now on run on the FP32 MFMA (FP64 data)."""
         self.check(self.lib.tenes_ctx_set_fp32_contractions(self.h, 1 if on else 0))

@@ -314,6 +314,11 @@ int tenes_ctx_set_autotune(tenes_ctx *ctx, int on) {
   return guard(ctx, [&] { ctx->c->set_autotune(on != 0); });
 }
 
+int tenes_ctx_set_stream_k(tenes_ctx *ctx, int on) {
+  if (!ctx) return TENES_ERR_ARGUMENT;
+  return guard(ctx, [&] { ctx->c->set_stream_k(on != 0); });
+}
+
 int tenes_ctx_set_max_lanes(tenes_ctx *ctx, int n) {
   return guard(ctx, [&] {
     TN_REQUIRE(n >= 1, "max lanes must be >= 1");

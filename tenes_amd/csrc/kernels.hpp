@@ -51,6 +51,10 @@ struct GemmPlan {
   int splits = 0;
 };
 // Launch shapes worth timing for (M, N, K), the heuristic's first.
+// The stream-K launch of an M x N x K FP64 product (GemmPlan{128, 0}) exists:
+// 128 x 128 tiles leaving at least half a round of 512 workgroups after the
+// whole rounds, and K of at least 64 tiles of 16.
+bool gemm_stream_k(int M, int N, int K);
 std::vector<GemmPlan> gemm_candidates(int M, int N, int K);
 // Scratch needed for split-K on the given shape (bytes, 0 if none).
 size_t gemm_workspace_bytes(int M, int N, int K,

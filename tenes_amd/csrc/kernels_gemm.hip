@@ -52,6 +52,17 @@ struct KArgs {
   GemmArgs g;
   int k_per_split;
   double *partial;  // split-K partials [z][M][N] or nullptr
+  // Stream-K (gemm_kernel<..., SK = true>, 128 x 128 FP64 tiles, a grid of
+  // kSkRanks workgroups): the first sk_dp tiles of the logical tile order run
+  // whole, kSkRanks at a time; the K iterations (BK each) of the remaining
+  // sk_tiles tiles are split evenly over the ranks.  A tile split between
+  // ranks is summed by the last of them to finish, from sk_S partial slots
+  // per rank (sk_part, 64 x 256 doubles each) and one arrival counter per
+  // tile (sk_cnt, zero on entry, zeroed again by that last rank).
+  int sk_dp = 0, sk_tiles = 0, sk_nk = 0, sk_S = 0;
+  int64_t sk_iters = 0;
+  double *sk_part = nullptr;
+  int *sk_cnt = nullptr;
 };
 
 // XCD-aware tile order.  Workgroups are dealt to the 8 XCDs round-robin by
@@ -62,11 +73,17 @@ struct KArgs {
 // A row panels and B column panels instead of touching all of them.
 constexpr int kXcds = 8;
 constexpr int kGroupM = 8;
+__device__ __forceinline__ void tile_coords(int t, int nbm, int nbn, int &bm, int &bn);
 __device__ __forceinline__ void tile_of(int id, int nbm, int nbn, int &bm, int &bn) {
   const int ntiles = nbm * nbn;
   const int xcd = id % kXcds, slot = id / kXcds;
   const int base = ntiles / kXcds, extra = ntiles % kXcds;
   const int t = xcd * base + min(xcd, extra) + slot;  // bijection on [0, ntiles)
+  tile_coords(t, nbm, nbn, bm, bn);
+}
+// Logical tile t -> (tile row, tile column): groups of kGroupM tile rows,
+// column by column inside a group.
+__device__ __forceinline__ void tile_coords(int t, int nbm, int nbn, int &bm, int &bn) {
   const int per_group = kGroupM * nbn;
   const int g = t / per_group;
   const int first_m = g * kGroupM;
@@ -76,8 +93,11 @@ __device__ __forceinline__ void tile_of(int id, int nbm, int nbn, int &bm, int &
   bn = r / gsize;
 }
 
-template <class CT, int BM, int BN, bool A_MFAST, bool B_NFAST, bool KA, bool KB>
-__global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
+constexpr int kSkRanks = 512;  // stream-K grid: two workgroups per CU
+
+template <class CT, int BM, int BN, bool A_MFAST, bool B_NFAST, bool KA, bool KB,
+          bool SK = false>
+__global__ __launch_bounds__(256, SK ? 2 : 1) void gemm_kernel(KArgs ka) {
   // KA / KB: the contracted-index offsets of A / B form an arithmetic
   // progression (computed), otherwise they come from the table, prefetched
   // one K tile ahead so that the tile loads never wait on a table read.
@@ -98,12 +118,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
   const int wm = w >> 1, wn = w & 1;
   const int M = g.M, N = g.N;
 
-  int bm, bn;
-  tile_of(blockIdx.y * gridDim.x + blockIdx.x, gridDim.y, gridDim.x, bm, bn);
-  const int kbeg = blockIdx.z * ka.k_per_split;
-  const int kend = min(g.K, kbeg + ka.k_per_split);
-  const int m0 = bm * BM;
-  const int n0 = bn * BN;
+  // the work item: output tile (m0, n0) over the K range [kbeg, kend)
+  int kbeg, kend, m0, n0;
 
   // ---- loader geometry ----------------------------------------------------
   // Element p of this thread in the A tile (BM x BK) is e = t + 256 p of the
@@ -113,26 +129,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
   // whole K loop and kept in registers.
   int a_ml[A_PER], a_kl[A_PER], a_moff[A_PER];
   bool a_mok[A_PER];
-#pragma unroll
-  for (int p = 0; p < A_PER; ++p) {
-    const int e = t + 256 * p;
-    a_ml[p] = A_MFAST ? e % BM : e / BK;
-    a_kl[p] = A_MFAST ? e / BM : e % BK;
-    const int m = m0 + a_ml[p];
-    a_mok[p] = m < M;
-    a_moff[p] = a_mok[p] ? (g.a_m_stride ? m * g.a_m_stride : g.a_m[m]) : 0;
-  }
   int b_nl[B_PER], b_kl[B_PER], b_noff[B_PER];
   bool b_nok[B_PER];
+  auto setup = [&]() {
 #pragma unroll
-  for (int p = 0; p < B_PER; ++p) {
-    const int e = t + 256 * p;
-    b_nl[p] = B_NFAST ? e % BN : e / BK;
-    b_kl[p] = B_NFAST ? e / BN : e % BK;
-    const int n = n0 + b_nl[p];
-    b_nok[p] = n < N;
-    b_noff[p] = b_nok[p] ? (g.b_n_stride ? n * g.b_n_stride : g.b_n[n]) : 0;
-  }
+    for (int p = 0; p < A_PER; ++p) {
+      const int e = t + 256 * p;
+      a_ml[p] = A_MFAST ? e % BM : e / BK;
+      a_kl[p] = A_MFAST ? e / BM : e % BK;
+      const int m = m0 + a_ml[p];
+      a_mok[p] = m < M;
+      a_moff[p] = a_mok[p] ? (g.a_m_stride ? m * g.a_m_stride : g.a_m[m]) : 0;
+    }
+#pragma unroll
+    for (int p = 0; p < B_PER; ++p) {
+      const int e = t + 256 * p;
+      b_nl[p] = B_NFAST ? e % BN : e / BK;
+      b_kl[p] = B_NFAST ? e / BN : e % BK;
+      const int n = n0 + b_nl[p];
+      b_nok[p] = n < N;
+      b_noff[p] = b_nok[p] ? (g.b_n_stride ? n * g.b_n_stride : g.b_n[n]) : 0;
+    }
+  };
 
   // k of load p of this thread within the tile starting at k0 (for the
   // k-fast layouts all loads of a thread share one k)
@@ -201,15 +219,18 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
   };
 
   acc_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
   const int li = lane & 15;  // row (A) / column (B) inside a 16x16 block
   const int lk = lane >> 4;  // k inside the 16x16x4 step
   const int wrow = wm * (BM / 2);
   const int wcol = wn * (BN / 2);
+
+  // acc = the (m0, n0) tile's product over [kbeg, kend); ends with a barrier
+  // after the last LDS read, so the next work item may refill the buffers
+  auto mainloop = [&]() {
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   if (BM == 128) {
     // 128 x 128 tiles: tile t+1 is loaded into registers while tile t is
@@ -296,11 +317,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
     buf ^= 1;
   }
   }
+  };
 
   // ---- epilogue -------------------------------------------------------------
   // Output layouts (lane l, register r, column l%16): v_mfma_f64_16x16x4
   // row (l/16) + 4*r (measured on gfx950); v_mfma_f32_16x16x4 row
   // 4*(l/16) + r.
+  auto epilogue = [&](int z) {
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -314,7 +337,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
         if (m >= M) continue;
         const double v = (double)acc[i][j][r];
         if (ka.partial) {
-          ka.partial[((int64_t)blockIdx.z * M + m) * N + n] = v;
+          ka.partial[((int64_t)z * M + m) * N + n] = v;
         } else {
           double *dst = g.C + g.c_m[m] + g.c_n[n];
           *dst = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * (*dst);
@@ -322,6 +345,110 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs ka) {
       }
     }
   }
+  };
+
+  if (!SK) {
+    int bm, bn;
+    tile_of(blockIdx.y * gridDim.x + blockIdx.x, gridDim.y, gridDim.x, bm, bn);
+    kbeg = blockIdx.z * ka.k_per_split;
+    kend = min(g.K, kbeg + ka.k_per_split);
+    m0 = bm * BM;
+    n0 = bn * BN;
+    setup();
+    mainloop();
+    epilogue(blockIdx.z);
+    return;
+  }
+
+  // ---- stream-K -------------------------------------------------------------
+  // Rank r of this workgroup: the ranks of one XCD (ids x, x+8, ...) are
+  // consecutive, so its data-parallel tiles and its stretch of the split
+  // iterations are neighbours in the logical tile order (shared panels in
+  // that XCD's L2).  One loop over the rank's work items (its data-parallel
+  // tiles, then its pieces of split tiles) so the tile code is inlined once.
+  __shared__ int s_last;
+  const int nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+  const int nk = ka.sk_nk;
+  const int rank = (blockIdx.x % kXcds) * (kSkRanks / kXcds) + blockIdx.x / kXcds;
+  const int64_t I = ka.sk_iters;
+  const int64_t beg = (int64_t)rank * I / kSkRanks;
+  const int64_t end = (int64_t)(rank + 1) * I / kSkRanks;
+  // the rank whose range holds split iteration x (largest r, r I / R <= x)
+  auto rank_of = [&](int64_t x) { return (int)(((x + 1) * kSkRanks + I - 1) / I - 1); };
+  constexpr int NREG = TM * TN * 4;
+  int tt = rank;
+  int64_t x = beg;
+  while (tt < ka.sk_dp || x < end) {
+    const bool dp = tt < ka.sk_dp;
+    int tile, ts = 0, kb = 0, ke = nk;
+    if (dp) {
+      tile = tt;
+      tt += kSkRanks;
+    } else {
+      ts = (int)(x / nk);
+      kb = (int)(x - (int64_t)ts * nk);
+      ke = (int)min((int64_t)nk, end - (int64_t)ts * nk);
+      tile = ka.sk_dp + ts;
+      x = (int64_t)ts * nk + ke;
+    }
+    int bm, bn;
+    tile_coords(tile, nbm, nbn, bm, bn);
+    m0 = bm * BM;
+    n0 = bn * BN;
+    kbeg = kb * BK;
+    kend = min(g.K, ke * BK);
+    setup();
+    mainloop();
+    const bool shared = !dp && !(kb == 0 && ke == nk);
+    if (shared) {
+      // this rank's share of tile ts: its partial slot, then the arrival
+      // count; the last of the tile's ranks to arrive writes the tile
+      const int first = rank_of((int64_t)ts * nk);
+      const int last = rank_of((int64_t)ts * nk + nk - 1);
+      auto slot_ptr = [&](int r) {
+        const int64_t rb = (int64_t)r * I / kSkRanks;
+        const int64_t sl = (int64_t)r * ka.sk_S + (ts - (int)(rb / nk));
+        return ka.sk_part + sl * NREG * 256;
+      };
+      double *mine = slot_ptr(rank);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            mine[((i * TN + j) * 4 + r) * 256 + t] = (double)acc[i][j][r];
+      __threadfence();
+      __syncthreads();
+      if (t == 0) s_last = atomicAdd(ka.sk_cnt + ts, 1) == last - first;
+      __syncthreads();
+      if (!s_last) continue;
+      __threadfence();
+      // sum the shares from their slots (this rank's own included) in rank
+      // order: the same sum whichever rank arrives last
+      for (int r = first; r <= last; ++r) {
+        const double *src = slot_ptr(r);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const CT v = (CT)src[((i * TN + j) * 4 + q) * 256 + t];
+              acc[i][j][q] = r == first ? v : acc[i][j][q] + v;
+            }
+      }
+      if (t == 0) ka.sk_cnt[ts] = 0;  // ready for the next launch
+    }
+    epilogue(0);
+  }
+}
+
+// Zero the stream-K arrival counters (a kernel, not a memset, so a captured
+// graph replays it correctly).
+__global__ void zero_ints_kernel(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 __global__ void splitk_reduce_kernel(const double *P, int splits, int M, int N,
@@ -339,7 +466,38 @@ __global__ void splitk_reduce_kernel(const double *P, int splits, int M, int N,
 
 struct Plan {
   int BM, BN, splits, kps;
+  bool sk = false;  // stream-K (128 x 128 tiles, kSkRanks workgroups)
 };
+
+// Stream-K layout of an M x N x K product on 128 x 128 tiles: whole rounds
+// of kSkRanks tiles run data-parallel, the K iterations of the remaining
+// tiles are split over the kSkRanks ranks.  Used when the remainder is at
+// least half a round (below that a split tile has more than ~4 shares; the
+// classic grid of two workgroups per CU leaves some CUs with one tile and
+// others with two) and K is long enough (>= 64 iterations) for the shares to
+// amortise their partial tiles.
+struct SkLayout {
+  int dp = 0, tiles = 0, nk = 0, S = 0;
+  int64_t iters = 0;
+  size_t part_bytes = 0, bytes = 0;
+};
+bool sk_layout(int M, int N, int K, SkLayout &L) {
+  if (M < 128 || N < 128 || K <= 0) return false;
+  const int64_t ntiles = (int64_t)ceil_div(M, 128) * ceil_div(N, 128);
+  const int nk = ceil_div(K, BK);
+  const int64_t dp = (ntiles / kSkRanks) * kSkRanks;
+  const int64_t rest = ntiles - dp;
+  if (rest < kSkRanks / 2 || nk < 64 || ntiles > (1 << 20)) return false;
+  L.dp = (int)dp;
+  L.tiles = (int)rest;
+  L.nk = nk;
+  L.iters = rest * nk;
+  const int64_t lmax = ceil_div(L.iters, (int64_t)kSkRanks);
+  L.S = (int)ceil_div(lmax, (int64_t)nk) + 1;
+  L.part_bytes = (size_t)kSkRanks * L.S * 64 * 256 * sizeof(double);
+  L.bytes = L.part_bytes + (size_t)L.tiles * sizeof(int);
+  return true;
+}
 
 Plan choose(int M, int N, int K) {
   Plan p;
@@ -379,6 +537,8 @@ Plan choose(int M, int N, int K) {
   p.kps = ceil_div(K, p.splits);
   p.kps = ceil_div(p.kps, BK) * BK;
   p.splits = ceil_div(K, p.kps);
+  // stream-K only on request (GemmPlan{128, 0}): at 2500 x 2500 x 2500 it
+  // measured 0.82 ms against 0.74 ms for the data-parallel grid
   return p;
 }
 
@@ -426,6 +586,13 @@ Plan plan_for(int M, int N, int K, const GemmPlan *gp) {
   if (gp == nullptr || gp->bm == 0 || K <= 0) return choose(M, N, K);
   Plan p;
   p.BM = p.BN = gp->bm == 128 ? 128 : 64;
+  SkLayout L;
+  if (p.BM == 128 && gp->splits == 0 && sk_layout(M, N, K, L)) {
+    p.splits = 1;
+    p.kps = ceil_div(K, BK) * BK;
+    p.sk = true;
+    return p;
+  }
   const int maxs = std::max(1, ceil_div(K, BK));
   p.splits = std::min(std::max(gp->splits, 1), maxs);
   p.kps = ceil_div(ceil_div(K, p.splits), BK) * BK;
@@ -434,6 +601,11 @@ Plan plan_for(int M, int N, int K, const GemmPlan *gp) {
 }
 
 }  // namespace
+
+bool gemm_stream_k(int M, int N, int K) {
+  SkLayout L;
+  return sk_layout(M, N, K, L);
+}
 
 std::vector<GemmPlan> gemm_candidates(int M, int N, int K) {
   std::vector<GemmPlan> out;
@@ -444,13 +616,14 @@ std::vector<GemmPlan> gemm_candidates(int M, int N, int K) {
     const Plan r = plan_for(M, N, K, &g);
     for (const auto &o : out) {
       const Plan e = plan_for(M, N, K, &o);
-      if (e.BM == r.BM && e.splits == r.splits) return;
+      if (e.BM == r.BM && e.splits == r.splits && e.sk == r.sk) return;
     }
     out.push_back(g);
   };
   add(h.BM, h.splits);
   add(64, 1);
   add(128, 1);
+  if (gemm_stream_k(M, N, K)) add(128, 0);  // splits 0 with bm 128: stream-K
   if (K >= 8 * BK) {
     add(64, std::max(2, h.splits * 2));
     add(64, std::max(2, h.splits / 2));
@@ -461,6 +634,8 @@ std::vector<GemmPlan> gemm_candidates(int M, int N, int K) {
 
 size_t gemm_workspace_bytes(int M, int N, int K, const GemmPlan *plan) {
   Plan p = plan_for(M, N, K, plan);
+  SkLayout L;
+  if (p.sk && sk_layout(M, N, K, L)) return L.bytes;
   if (p.splits <= 1) return 0;
   return (size_t)p.splits * M * N * sizeof(double);
 }
@@ -488,6 +663,27 @@ const char *gemm_f64(const GemmArgs &g, void *workspace, size_t ws_bytes,
     }
   }
   if (g.K == 0) ka.k_per_split = 0;
+  // stream-K: FP64, the row-major R = LT * LB operand layout (the
+  // instantiation built for it), the workspace for its partial tiles
+  SkLayout L;
+  if (p.sk && !g.fp32 && !g.a_mfast && g.b_nfast && g.a_k_stride != 0 && g.b_k_stride != 0 &&
+      sk_layout(g.M, g.N, g.K, L) && workspace != nullptr && ws_bytes >= L.bytes) {
+    ka.sk_dp = L.dp;
+    ka.sk_tiles = L.tiles;
+    ka.sk_nk = L.nk;
+    ka.sk_S = L.S;
+    ka.sk_iters = L.iters;
+    ka.sk_part = static_cast<double *>(workspace);
+    ka.sk_cnt = reinterpret_cast<int *>(static_cast<char *>(workspace) + L.part_bytes);
+    ka.k_per_split = g.K;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(ceil_div(L.tiles, 256)), dim3(256), 0, s,
+                       ka.sk_cnt, L.tiles);
+    hipLaunchKernelGGL((gemm_kernel<double, 128, 128, false, true, true, true, true>),
+                       dim3(kSkRanks), dim3(256), 0, s, ka);
+    TN_HIP_CHECK(hipGetLastError());
+    static const std::string name = std::string(kernel_name<double, 128, false, true, true, true>()) + "+sk";
+    return name.c_str();
+  }
   dim3 grid(ceil_div(g.N, p.BN), ceil_div(g.M, p.BM), splits);
   const char *name =
       g.fp32 ? ((p.BM == 128) ? launch_cfg<float, 128, 128>(ka, grid, s)

@@ -863,6 +863,9 @@ void tensordot_into(const Tensor &A, const Tensor &B, const Axes &axA,
   }
   kern::GemmPlan tuned{p.tuned_bm, p.tuned_splits};
   const kern::GemmPlan *plan = p.tuned_bm ? &tuned : nullptr;
+  static const kern::GemmPlan kStreamK{128, 0};
+  if (!plan && ctx.stream_k() && !g.fp32 && !g.bf16x3 && kern::gemm_stream_k(p.M, p.N, p.K))
+    plan = &kStreamK;
   const size_t ws = kern::gemm_workspace_bytes(p.M, p.N, p.K, plan);
   void *w = ws ? ctx.workspace(ws) : nullptr;
   if (ctx.profiling()) {

@@ -287,6 +287,12 @@ class Context {
   void set_autotune(bool on) { autotune_ = on; }
   bool autotune() const { return autotune_; }
   static constexpr double kAutotuneFlops = 1e8;
+  // Stream-K launches for the FP64 GEMMs that qualify (kern::gemm_stream_k):
+  // 128 x 128 tiles whose grid leaves a partial round.  Off by default
+  // (slower than the data-parallel grid at 2500^3 on this part); the
+  // autotuner also tries it.
+  void set_stream_k(bool on) { stream_k_ = on; }
+  bool stream_k() const { return stream_k_; }
   void profile_begin(hipEvent_t *a);
   void profile_end(hipEvent_t a, const char *name, double flops);
   std::map<std::string, KernelStat> kernel_stats();  // flushes pending
@@ -327,6 +333,7 @@ class Context {
   bool fused_site_ = true;
   bool single_layer_ = false;
   bool autotune_ = false;
+  bool stream_k_ = false;
   bool profiling_ = false;
   struct Pending {
     hipEvent_t a, b;

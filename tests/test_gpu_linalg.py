@@ -26,6 +26,44 @@ def test_matmul_shapes(gpu_ctx, M, N, K):
     assert _rel(C, A @ B) < RTOL
 
 
+@pytest.mark.parametrize("M,N,K", [(2500, 2500, 2500), (3500, 3500, 2000), (2100, 2100, 1024),
+                                   (2000, 2300, 1500), (2500, 2500, 1100)])
+def test_matmul_stream_k(gpu_ctx, M, N, K):
+    """tenes_ctx_set_stream_k: row-major FP64 products whose 128 x 128 tile
+    grid leaves a partial round (2500^2: 400 tiles on 512 workgroup slots)
+    run stream-K: K
+    iterations split evenly over 512 workgroups, split tiles summed by
+    their last contributor.  Checked against numpy (1e-12 of max|C|), over
+    three launches in a row (the arrival counters must come back to zero),
+    bit-identical between launches (the shares are summed in rank order),
+    and the launch must be the stream-K one (kernel report); (3500, 3500,
+    2000) has a data-parallel round (512 tiles) before the split ones."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    ref = A @ B
+    ta_, tb_ = gpu_ctx.tensor(A), gpu_ctx.tensor(B)
+    gpu_ctx.set_profiling(True)
+    gpu_ctx.set_stream_k(True)
+    try:
+        first = None
+        for _ in range(3):
+            C = ta.tensordot(ta_, tb_, [1], [0]).numpy()
+            assert _rel(C, ref) < RTOL
+            if first is None:
+                first = C
+            else:
+                assert np.array_equal(C, first)
+        names = [k for k in gpu_ctx.kernel_report() if f"[{M}x{N}x{K}]" in k]
+    finally:
+        gpu_ctx.set_profiling(False)
+        gpu_ctx.set_stream_k(False)
+    assert names and all("+sk" in k for k in names), names
+    # and off (the default) the same product takes the data-parallel grid
+    C = ta.tensordot(ta_, tb_, [1], [0]).numpy()
+    assert _rel(C, ref) < RTOL
+
+
 @pytest.mark.parametrize("case", [
     # (shapeA, shapeB, axA, axB, perm)
     ((4, 5, 6), (6, 5, 3), [1, 2], [1, 0], None),
