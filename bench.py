@@ -273,8 +273,8 @@ def probe_gemm():
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="timed CTM sweeps")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed CTM sweeps")
+    ap.add_argument("--steps", type=int, default=20, help="timed CTM sweeps")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed CTM sweeps")
     ap.add_argument("--D", type=int, default=5)
     ap.add_argument("--chi", type=int, default=100)
     ap.add_argument("--su-steps", type=int, default=500)
