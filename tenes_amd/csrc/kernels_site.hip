@@ -37,7 +37,7 @@ constexpr int kSiteNB = 4;
 
 // Store the NB staged output blocks os[q][(oA,oB)][(boA,boB)] of workgroup
 // (s, f0) in the edge (block-major) or corner layout.
-template <bool EDGE, int D>
+template <bool EDGE, int D, int NT = kSiteThreads>
 __device__ __forceinline__ void store_out(const SiteArgs &a, const double *os, int s, int f0,
                                           int nb, int t) {
   constexpr int D2 = D * D;
@@ -46,11 +46,11 @@ __device__ __forceinline__ void store_out(const SiteArgs &a, const double *os, i
   if (EDGE) {
     double *dst = a.out + ((int64_t)s * a.Df + f0) * (KO * BO);
     const int n = nb * KO * BO;
-    for (int e = t; e < n; e += kSiteThreads) dst[e] = os[e];
+    for (int e = t; e < n; e += NT) dst[e] = os[e];
   } else {
     // order (oA, boA, oB, q, boB): runs of nb D contiguous doubles
     const int n = D * D * D * NB * D;
-    for (int e = t; e < n; e += kSiteThreads) {
+    for (int e = t; e < n; e += NT) {
       int r = e;
       const int boB = r % D;
       r /= D;
@@ -141,8 +141,10 @@ __global__ __launch_bounds__(kSiteThreads) void site_apply_kernel(SiteArgs a) {
 
 typedef double site_d4 __attribute__((ext_vector_type(4)));
 
+constexpr int kSiteMfmaThreads = 512;  // 8 waves: 4 per SIMD at two workgroups per CU
+
 template <bool EDGE, int D, int P>
-__global__ __launch_bounds__(kSiteThreads) void site_mfma_kernel(SiteArgs a) {
+__global__ __launch_bounds__(kSiteMfmaThreads) __attribute__((amdgpu_waves_per_eu(4))) void site_mfma_kernel(SiteArgs a) {
   constexpr int D2 = D * D;
   constexpr int KI = D2, BI = D2, KO = D2, BO = D2;
   constexpr int NCB = BO * P;  // intermediate columns (boA, boB, p)
@@ -155,9 +157,14 @@ __global__ __launch_bounds__(kSiteThreads) void site_mfma_kernel(SiteArgs a) {
   // banks: at 112 doubles the lanes of a row group met on two bank offsets
   constexpr int LA1 = MT1 * 16 + 1;  // row length of A1s (m)
   constexpr int LB2 = NT2 * 16 + 1;  // row length of B2s (n)
-  constexpr int NW = kSiteThreads / 64;
-  constexpr int NTW1 = (NT1 + NW - 1) / NW;  // product-1 N tiles per wave
-  constexpr int NTW2 = (NT2 + NW - 1) / NW;  // product-2 N tiles per wave
+  constexpr int NTH = kSiteMfmaThreads;
+  constexpr int NW = NTH / 64;
+  // product 1: output tiles tau = w, w + NW, ...; tile tau is (mt, nt) =
+  // (tau / NT1, tau % NT1), so a wave keeps one N tile (NW % NT1 == 0) and
+  // its TB columns in registers
+  static_assert(NW % NT1 == 0, "one product-1 N tile per wave");
+  constexpr int MTW1 = (MT1 * NT1 + NW - 1) / NW;  // product-1 tiles per wave
+  constexpr int NTW2 = (NT2 + NW - 1) / NW;        // product-2 N tiles per wave
   // A1s[k = bi][m = q KI + ki] (zero padded); B2s[k = ki P + p][n = q BO + bo];
   // the output staging os aliases A1s once the first product has read it
   __shared__ double A1s[KS1 * 4 * LA1];
@@ -172,18 +179,18 @@ __global__ __launch_bounds__(kSiteThreads) void site_mfma_kernel(SiteArgs a) {
   const int f0 = (blockIdx.x - s * ngrp) * NB;
   const int nb = min(NB, a.Df - f0);
 
-  for (int e = t; e < KS1 * 4 * LA1; e += kSiteThreads) A1s[e] = 0.0;
-  for (int e = t; e < KS2 * 4 * LB2; e += kSiteThreads) B2s[e] = 0.0;
+  for (int e = t; e < KS1 * 4 * LA1; e += NTH) A1s[e] = 0.0;
+  for (int e = t; e < KS2 * 4 * LB2; e += NTH) B2s[e] = 0.0;
   // operands that stay in registers: this wave's TB columns (product 1) and
   // all of TK^T (product 2)
-  double b1[NTW1][KS1];
-#pragma unroll
-  for (int j = 0; j < NTW1; ++j) {
-    const int c = 16 * (w + NW * j) + li;
+  const int nt1 = w % NT1;
+  double b1[KS1];
+  {
+    const int c = 16 * nt1 + li;
 #pragma unroll
     for (int ks = 0; ks < KS1; ++ks) {
       const int k = 4 * ks + lk;
-      b1[j][ks] = (k < BI && c < NCB) ? a.TB[k * NCB + c] : 0.0;
+      b1[ks] = (k < BI && c < NCB) ? a.TB[k * NCB + c] : 0.0;
     }
   }
   double a2[MT2][KS2];
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(kSiteThreads) void site_mfma_kernel(SiteArgs a) {
   {
     const double *src = a.X + ((int64_t)s * a.Df + f0) * (KI * BI);
     const int n = nb * KI * BI;
-    for (int e = t; e < n; e += kSiteThreads) {
+    for (int e = t; e < n; e += NTH) {
       const int q = e / (KI * BI), r = e - q * (KI * BI);
       const int ki = r / BI, bi = r - ki * BI;
       A1s[bi * LA1 + q * KI + ki] = src[e];
@@ -209,34 +216,35 @@ __global__ __launch_bounds__(kSiteThreads) void site_mfma_kernel(SiteArgs a) {
   }
   __syncthreads();
   // ---- product 1 -> B2s -----------------------------------------------------
+  {
+    site_d4 acc[MTW1];
 #pragma unroll
-  for (int j = 0; j < NTW1; ++j) {
-    const int nt = w + NW * j;
-    if (nt < NT1) {
-      site_d4 acc[MT1];
+    for (int j = 0; j < MTW1; ++j) acc[j] = site_d4{0, 0, 0, 0};
 #pragma unroll
-      for (int mt = 0; mt < MT1; ++mt) acc[mt] = site_d4{0, 0, 0, 0};
+    for (int ks = 0; ks < KS1; ++ks) {
 #pragma unroll
-      for (int ks = 0; ks < KS1; ++ks) {
-#pragma unroll
-        for (int mt = 0; mt < MT1; ++mt) {
+      for (int j = 0; j < MTW1; ++j) {
+        const int mt = (w + NW * j) / NT1;
+        if (mt < MT1) {
           const double av = A1s[(4 * ks + lk) * LA1 + 16 * mt + li];
-          acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1[j][ks], acc[mt], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1[ks], acc[j], 0, 0, 0);
         }
       }
-      // lane l, register r: row 16 mt + l/16 + 4 r, column 16 nt + l%16
-      const int c = 16 * nt + li;
-      const int p = c % P, bo = c / P;
+    }
+    // lane l, register r: row 16 mt + l/16 + 4 r, column 16 nt1 + l%16
+    const int c = 16 * nt1 + li;
+    const int p = c % P, bo = c / P;
 #pragma unroll
-      for (int mt = 0; mt < MT1; ++mt)
+    for (int j = 0; j < MTW1; ++j) {
+      const int mt = (w + NW * j) / NT1;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = 16 * mt + lk + 4 * r;
-          if (m < NB * KI && c < NCB) {
-            const int q = m / KI, ki = m - q * KI;
-            B2s[(ki * P + p) * LB2 + q * BO + bo] = acc[mt][r];
-          }
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * mt + lk + 4 * r;
+        if (mt < MT1 && m < NB * KI && c < NCB) {
+          const int q = m / KI, ki = m - q * KI;
+          B2s[(ki * P + p) * LB2 + q * BO + bo] = acc[j][r];
         }
+      }
     }
   }
   __syncthreads();
@@ -269,7 +277,7 @@ __global__ __launch_bounds__(kSiteThreads) void site_mfma_kernel(SiteArgs a) {
     }
   }
   __syncthreads();
-  store_out<EDGE, D>(a, os, s, f0, nb, t);
+  store_out<EDGE, D, NTH>(a, os, s, f0, nb, t);
 }
 
 template <bool EDGE>
@@ -282,10 +290,10 @@ void launch_site(const SiteArgs &a, dim3 grid, hipStream_t s) {
       hipLaunchKernelGGL((site_apply_kernel<EDGE, 3, 2>), grid, dim3(kSiteThreads), 0, s, a);
       break;
     case 4:
-      hipLaunchKernelGGL((site_mfma_kernel<EDGE, 4, 2>), grid, dim3(kSiteThreads), 0, s, a);
+      hipLaunchKernelGGL((site_mfma_kernel<EDGE, 4, 2>), grid, dim3(kSiteMfmaThreads), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((site_mfma_kernel<EDGE, 5, 2>), grid, dim3(kSiteThreads), 0, s, a);
+      hipLaunchKernelGGL((site_mfma_kernel<EDGE, 5, 2>), grid, dim3(kSiteMfmaThreads), 0, s, a);
       break;
   }
 }
