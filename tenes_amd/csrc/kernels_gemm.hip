@@ -507,11 +507,14 @@ Plan choose(int M, int N, int K) {
     p.kps = 0;
     return p;
   }
-  // Large outputs: 128 x 128 tiles.  (160 x 160 tiles would fill the 256
-  // CUs in one round at 2500 x 2500, but measured 1.03 ms against 0.78 ms:
-  // their 25 accumulator tiles per wave leave too few registers.)
+  // Large outputs with a long K: 128 x 128 tiles.  (160 x 160 tiles would
+  // fill the 256 CUs in one round at 2500 x 2500, but measured 1.03 ms
+  // against 0.78 ms: their 25 accumulator tiles per wave leave too few
+  // registers.)  With K of at most 8 tiles the 64 x 64 grid, four times the
+  // workgroups at twice the occupancy, hides more of the short main loop's
+  // prologue and epilogue: 2500 x 2500 x 100 in 47 us against 53 us.
   const int64_t tiles128 = (int64_t)ceil_div(M, 128) * ceil_div(N, 128);
-  p.BM = p.BN = (M >= 128 && N >= 128 && tiles128 >= 200) ? 128 : 64;
+  p.BM = p.BN = (M >= 128 && N >= 128 && tiles128 >= 200 && K > 8 * BK) ? 128 : 64;
   const int64_t tiles = (int64_t)ceil_div(M, p.BM) * ceil_div(N, p.BN);
   // Split K when the tile grid cannot fill the 256 CUs and K is long: the
   // split count s that minimises the rounds of 256 workgroups per unit of
