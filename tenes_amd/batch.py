@@ -8,7 +8,7 @@ sweep: n problems (default 48), hx_k = 3k/(n-1), Jz = -1, 2x2 unit cell, D
 and chi cycling in lockstep through (3, 32), (4, 48), (5, 64), site tensors
 from 300 simple-update steps (tau = 0.01) on the GPU from the ferro state.
 
-    python -m tenes_amd.batch [--problems 48] [--contexts 4] [--sweeps 5]
+    python -m tenes_amd.batch [--problems 48] [--contexts 1] [--sweeps 5]
                               [--graph 1] [--lanes 8]
 
 prints one JSON line: aggregate sweeps/s over all problems (batched and, for
@@ -112,7 +112,10 @@ def run_ctm(envs, sweeps: int, initialize: bool, **kw):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--problems", type=int, default=48)
-    ap.add_argument("--contexts", type=int, default=4)
+    # one context with 8 lanes and captured sweeps measured fastest (38
+    # sweeps/s); captured sweeps on several contexts oversubscribe the
+    # hardware queues (4 contexts: 25-26)
+    ap.add_argument("--contexts", type=int, default=1)
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--su-steps", type=int, default=300)
