@@ -47,11 +47,12 @@ def test_calc_next_et(gpu_ctx, chi, D, cut):
     assert rel(g.numpy(), ref) < 1e-11
 
 
-@pytest.mark.parametrize("chi,D,cut", [(6, 2, 5), (40, 4, 33), (100, 5, 100)])
+@pytest.mark.parametrize("chi,D,cut", [(6, 2, 5), (40, 4, 33), (100, 5, 100), (30, 5, 27)])
 def test_calc_next_et_fused_and_unfused(gpu_ctx, chi, D, cut):
     """Calc_Next_eT with its two site contractions fused into one kernel (the
     default) and as two gather-GEMMs: both against the oracle to 1e-11 and
-    against each other to rounding."""
+    against each other to rounding.  The fused kernel takes the blocks of
+    the cut leg four at a time: cut = 33 and 27 end on a group of 1 and 3."""
     rng = np.random.default_rng(5 * chi + D)
     eT, Tn, PU, PL = random_env_pieces(rng, chi, D, cut=cut)
     ref = O.Calc_Next_eT(eT, Tn, PU, PL)
@@ -149,6 +150,34 @@ def test_projector_left_block(gpu_ctx, model, D, chi, cut):
     P = np.tensordot(PU, PL, ([3], [3]))
     gP = np.tensordot(gPU.numpy(), gPL.numpy(), ([3], [3]))
     assert rel(gP, P) < 1e-10
+
+
+@pytest.mark.parametrize("D,chi", [(4, 7), (5, 5)])
+def test_projector_left_block_fused_partial_groups(gpu_ctx, D, chi):
+    """The enlarged corners through the fused site kernel on the MFMA (D = 4,
+    5) when chi is not a multiple of its 4-block groups (chi = 7: a last group
+    of 3; chi = 5: of 1), random environment pieces, chi D^2 <= 128 (exact
+    Jacobi SVD): PU PL^T fused against unfused to 1e-12 and against the
+    oracle to 1e-10."""
+    rng = np.random.default_rng(11 * D + chi)
+    C1, C4 = rng.standard_normal((chi, chi)), rng.standard_normal((chi, chi))
+    eTt, eTb, eTl1, eTl0 = (rng.standard_normal((chi, chi, D, D)) for _ in range(4))
+    Tn0, Tn1 = (rng.standard_normal((D, D, D, D, 2)) for _ in range(2))
+    args = (C1, C4, eTt, eTb, eTl1, eTl0, Tn0, Tn1)
+    PU, PL = O.Calc_projector_left_block(*args, O.PEPSParameters(CHI=chi))
+    P = np.tensordot(PU, PL, ([3], [3]))
+    gps = []
+    for fused in (True, False):
+        gpu_ctx.set_fused_site(fused)
+        try:
+            gPU, gPL = ta.calc_projector_left_block(*[gpu_ctx.tensor(a) for a in args],
+                                                    ta.peps_parameters(CHI=chi))
+        finally:
+            gpu_ctx.set_fused_site(True)
+        gps.append(np.tensordot(gPU.numpy(), gPL.numpy(), ([3], [3])))
+    assert rel(gps[0], gps[1]) < 1e-12
+    for gP in gps:
+        assert rel(gP, P) < 1e-10
 
 
 def test_projector_updown_blocks(gpu_ctx):
